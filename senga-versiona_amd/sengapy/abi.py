@@ -1,0 +1,245 @@
+"""ctypes binding of the C ABI in include/senga_gpu.h.
+
+``Config`` mirrors ``sg_config`` field by field; ``config_from_case`` fills it
+from a ``sengapy.case.Case`` (cont.dat control data plus the chem.dat/diff.dat
+derived tables).  ``Library`` loads libsenga_gpu.so and wraps the entry points;
+it raises if the library is missing (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+MAX_SPEC = 15
+MAX_STEP = 128
+MAX_SSL = 10
+MAX_BODY = 16
+MAX_FALL = 32
+NCOF = 7
+NTINT = 2
+NTRC = 4
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBDIR = os.path.join(PKG, "lib")
+LIBPATH = os.path.join(LIBDIR, "libsenga_gpu.so")
+
+d = C.c_double
+i32 = C.c_int32
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("nx", i32), ("ny", i32), ("nz", i32), ("nspec", i32), ("nstep", i32),
+        ("mixavg", i32), ("precision", i32), ("reserved0", i32),
+        ("xgdlen", d), ("ygdlen", d), ("zgdlen", d),
+        ("bc_type", i32 * 6), ("bc_ipar", (i32 * 4) * 6), ("bc_rpar", (d * 4) * 6),
+        ("prin", d), ("trin", d), ("urin", d), ("vrin", d), ("wrin", d),
+        ("yrin", d * MAX_SPEC),
+        ("alamda", d), ("rlamda", d), ("prantl", d), ("olewis", d * MAX_SPEC),
+        ("wmolar", d * MAX_SPEC), ("ovwmol", d * MAX_SPEC), ("rgspec", d * MAX_SPEC),
+        ("ntint", i32 * MAX_SPEC),
+        ("ncpoly", (i32 * NTINT) * MAX_SPEC), ("ncenth", (i32 * NTINT) * MAX_SPEC),
+        ("ncenpy", (i32 * NTINT) * MAX_SPEC), ("tinthi", (d * NTINT) * MAX_SPEC),
+        ("amascp", ((d * NCOF) * NTINT) * MAX_SPEC), ("amasct", ((d * NCOF) * NTINT) * MAX_SPEC),
+        ("amasch", ((d * NCOF) * NTINT) * MAX_SPEC), ("amolgb", ((d * NCOF) * NTINT) * MAX_SPEC),
+        ("rparam", (d * 3) * MAX_STEP),
+        ("nsslen", i32 * MAX_STEP), ("nsspec", (i32 * MAX_SSL) * MAX_STEP),
+        ("diffmu", (d * MAX_SSL) * MAX_STEP), ("diffmw", (d * MAX_SSL) * MAX_STEP),
+        ("nrslen", i32 * MAX_STEP), ("nrspec", (i32 * MAX_SSL) * MAX_STEP),
+        ("npslen", i32 * MAX_STEP), ("npspec", (i32 * MAX_SSL) * MAX_STEP),
+        ("nrclen", i32 * MAX_STEP), ("nrcpec", (i32 * MAX_SSL) * MAX_STEP),
+        ("crspec", (d * MAX_SSL) * MAX_STEP),
+        ("npclen", i32 * MAX_STEP), ("npcpec", (i32 * MAX_SSL) * MAX_STEP),
+        ("cpspec", (d * MAX_SSL) * MAX_STEP),
+        ("mblist", i32 * MAX_STEP), ("mglist", i32 * MAX_STEP), ("mllist", i32 * MAX_STEP),
+        ("mtlist", i32 * MAX_STEP), ("mslist", i32 * MAX_STEP),
+        ("effy3b", (d * MAX_SPEC) * MAX_BODY),
+        ("rclind", (d * 4) * MAX_FALL), ("rctroe", (d * 12) * MAX_FALL), ("rcsrif", (d * 8) * MAX_FALL),
+        ("clnten", d),
+        ("pdifgb", d), ("tdifgb", d),
+        ("ncovis", i32), ("ncocon", i32), ("ncodif", i32), ("reserved1", i32),
+        ("viscco", (d * NTRC) * MAX_SPEC), ("condco", (d * NTRC) * MAX_SPEC),
+        ("diffco", ((d * NTRC) * MAX_SPEC) * MAX_SPEC),
+        ("wilko1", (d * MAX_SPEC) * MAX_SPEC), ("wilko2", (d * MAX_SPEC) * MAX_SPEC),
+    ]
+
+
+def config_from_case(case) -> Config:
+    """Fill sg_config from a Case (control + derived tables)."""
+    c = Config()
+    ctl, t, m = case.control, case.tables, case.tables.mech
+    ns, nst = t.nspec, t.nstep
+    if ns > MAX_SPEC or nst > MAX_STEP:
+        raise ValueError("mechanism too large for sg_config")
+    c.nx, c.ny, c.nz = ctl.nx, ctl.ny, ctl.nz
+    c.nspec, c.nstep = ns, nst
+    c.mixavg = 1 if ctl.nfmavt == 1 else 0
+    c.precision = 0
+    c.xgdlen, c.ygdlen, c.zgdlen = ctl.xgdlen, ctl.ygdlen, ctl.zgdlen
+    for f, (typ, ip, rp) in enumerate(ctl.bc):
+        c.bc_type[f] = typ
+        for q in range(4):
+            c.bc_ipar[f][q] = ip[q]
+            c.bc_rpar[f][q] = rp[q]
+    c.prin, c.trin, c.urin, c.vrin, c.wrin = ctl.prin, ctl.trin, ctl.urin, ctl.vrin, ctl.wrin
+    for s in range(ns):
+        c.yrin[s] = ctl.yrin[s] if s < len(ctl.yrin) else 0.0
+    from .chem import RLAMDA, PRANTL
+    c.alamda, c.rlamda, c.prantl = t.alamda, RLAMDA, PRANTL
+    for s in range(ns):
+        c.olewis[s] = t.olewis[s]
+        c.wmolar[s], c.ovwmol[s], c.rgspec[s] = t.wmolar[s], t.ovwmol[s], t.rgspec[s]
+        c.ntint[s] = int(t.ntint[s])
+        for it in range(int(t.ntint[s])):
+            c.ncpoly[s][it] = int(t.ncpoly[it, s])
+            c.ncenth[s][it] = int(t.ncenth[it, s])
+            c.ncenpy[s][it] = int(t.ncenpy[it, s])
+            c.tinthi[s][it] = t.tinthi[it, s]
+            for q in range(NCOF):
+                c.amascp[s][it][q] = t.amascp[q, it, s]
+                c.amasct[s][it][q] = t.amasct[q, it, s]
+                c.amasch[s][it][q] = t.amasch[q, it, s]
+                c.amolgb[s][it][q] = t.amolgb[q, it, s]
+    for st in range(nst):
+        c.rparam[st][0], c.rparam[st][1], c.rparam[st][2] = t.rparam[:, st]
+        c.nsslen[st] = len(m.nsspec[st])
+        for q, sp in enumerate(m.nsspec[st]):
+            c.nsspec[st][q] = sp - 1
+            c.diffmu[st][q] = t.diffmu[st][q]
+            c.diffmw[st][q] = t.diffmw[st][q]
+        c.nrslen[st] = len(m.nrspec[st])
+        for q, sp in enumerate(m.nrspec[st]):
+            c.nrspec[st][q] = sp - 1
+        c.npslen[st] = len(m.npspec[st])
+        for q, sp in enumerate(m.npspec[st]):
+            c.npspec[st][q] = sp - 1
+        c.nrclen[st] = len(m.nrcpec[st])
+        for q, (sp, cf) in enumerate(m.nrcpec[st]):
+            c.nrcpec[st][q] = sp - 1
+            c.crspec[st][q] = cf
+        c.npclen[st] = len(m.npcpec[st])
+        for q, (sp, cf) in enumerate(m.npcpec[st]):
+            c.npcpec[st][q] = sp - 1
+            c.cpspec[st][q] = cf
+        c.mblist[st] = m.mblist[st] if m.bodies else 0
+        c.mglist[st] = m.mglist[st]
+        c.mllist[st] = m.mllist[st]
+        c.mtlist[st] = m.mtlist[st]
+        c.mslist[st] = m.mslist[st]
+    for b in range(len(m.bodies)):
+        for s in range(ns):
+            c.effy3b[b][s] = m.effy3b[s, b]
+    for j in range(t.rclind.shape[1]):
+        for q in range(4):
+            c.rclind[j][q] = t.rclind[q, j]
+    for j in range(t.rctroe.shape[1]):
+        for q in range(12):
+            c.rctroe[j][q] = t.rctroe[q, j]
+    for j in range(t.rcsrif.shape[1]):
+        for q in range(8):
+            c.rcsrif[j][q] = t.rcsrif[q, j]
+    c.clnten = math.log(1.0e1)          # src/indata.f:1071
+    dt = case.diff
+    if dt is not None:
+        c.pdifgb, c.tdifgb = dt.pdifgb, dt.tdifgb
+        c.ncovis, c.ncocon, c.ncodif = dt.ncovis, dt.ncocon, dt.ncodif
+        for s in range(ns):
+            for q in range(dt.ncovis):
+                c.viscco[s][q] = dt.viscco[q, s]
+            for q in range(dt.ncocon):
+                c.condco[s][q] = dt.condco[q, s]
+            for j in range(ns):
+                for q in range(dt.ncodif):
+                    c.diffco[s][j][q] = dt.diffco[q, j, s]
+                c.wilko1[s][j] = dt.wilko1[j, s]
+                c.wilko2[s][j] = dt.wilko2[j, s]
+    return c
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    """The C restatement (oracle/lib/liboracle.so) -- test infrastructure."""
+
+    def __init__(self, case, libpath=None):
+        root = os.path.dirname(PKG)
+        libpath = libpath or os.path.join(root, "oracle", "lib", "liboracle.so")
+        lib = C.CDLL(libpath)
+        lib.orc_create.restype = C.c_void_p
+        lib.orc_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_int)]
+        for n in ("orc_set_state",):
+            getattr(lib, n).argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_double]
+        lib.orc_set_timestep.argtypes = [C.c_void_p, C.c_double]
+        lib.orc_rhs.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+        lib.orc_substage.argtypes = [C.c_void_p, C.c_int]
+        lib.orc_end_step.argtypes = [C.c_void_p]
+        lib.orc_step.argtypes = [C.c_void_p, C.c_int]
+        lib.orc_get.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
+        lib.orc_get_stage.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        lib.orc_get_itndex.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        lib.orc_derivatives.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.orc_destroy.argtypes = [C.c_void_p]
+        self.lib = lib
+        self.case = case
+        self.cfg = config_from_case(case)
+        err = C.c_int(0)
+        self.h = lib.orc_create(C.byref(self.cfg), C.byref(err))
+        if not self.h:
+            raise RuntimeError(f"orc_create failed: {err.value}")
+        self.nx, self.ny, self.nz = case.shape
+        self.ns = case.nspec
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed: {rc}")
+
+    def set_state(self, run, etime=0.0):
+        run = np.ascontiguousarray(run, dtype=np.float64)
+        self._chk(self.lib.orc_set_state(self.h, _dp(run), etime), "orc_set_state")
+
+    def set_timestep(self, dt):
+        self.lib.orc_set_timestep(self.h, dt)
+
+    def rhs(self, irk=1):
+        out = np.zeros((5 + self.ns, self.nz, self.ny, self.nx))
+        self._chk(self.lib.orc_rhs(self.h, irk, _dp(out)), "orc_rhs")
+        return out
+
+    def substage(self, irk):
+        self._chk(self.lib.orc_substage(self.h, irk), "orc_substage")
+
+    def step(self, n=1):
+        self._chk(self.lib.orc_step(self.h, n), "orc_step")
+
+    def get(self):
+        run = np.zeros((5 + self.ns, self.nz, self.ny, self.nx))
+        T = np.zeros((self.nz, self.ny, self.nx)); P = np.zeros_like(T)
+        rate = np.zeros((self.ns, self.nz, self.ny, self.nx))
+        self.lib.orc_get(self.h, _dp(run), _dp(T), _dp(P), _dp(rate))
+        return run, T, P, rate
+
+    def stage(self):
+        st = np.zeros((5 + self.ns, self.nz, self.ny, self.nx))
+        self.lib.orc_get_stage(self.h, _dp(st))
+        return st
+
+    def itndex(self):
+        a = np.zeros((self.nz, self.ny, self.nx), dtype=np.int32)
+        self.lib.orc_get_itndex(self.h, a.ctypes.data_as(C.POINTER(C.c_int)))
+        return a
+
+    def derivatives(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        out = np.zeros((9,) + f.shape)
+        self.lib.orc_derivatives(self.h, _dp(f), _dp(out))
+        return out

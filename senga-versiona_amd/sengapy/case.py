@@ -1,0 +1,244 @@
+"""Case builders: control data, mechanism, transport data and a seeded initial dump.
+
+Two case families match the workloads this library is measured on:
+
+* ``periodic_1step``: triply periodic box, 2-species single-step irreversible
+  Arrhenius chemistry (R -> P, Zeldovich number 6, heat-release parameter 0.75,
+  unity Lewis number, Pr = 0.7), a planar flame slab in progress variable along
+  x (two erf fronts so the profile is periodic) and seeded solenoidal
+  turbulence.  At 32^3 with L = 2*pi mm this is the CPU-runnable workload.
+* ``h2_nscbc``: 9-species 21-step H2-air mechanism (the reference's
+  input/chem.dat), NSCBC inflow (type 13, specified density/velocity/species) at
+  x-left and partially non-reflecting outflow (type 21) at x-right, periodic in
+  y and z, premixed phi = 0.7 at T_u = 300 K and 1 atm, a planar erf flame with
+  radical bumps, and seeded Passot-Pouquet turbulence.  At 128^3 this is the
+  FP64 GPU workload.
+
+The time step is fixed from the initial field: dt = CFL * min(dx) / max(|u|+a)
+with CFL = 0.5 (the reference has no CFL logic; it takes a fixed TSTEP from
+cont.dat when the step switch is 0).
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import chem as _chem
+from . import diff as _diff
+from .cont import Control
+from .dump import Dump, write_dump
+from . import ic as _ic
+
+DATA = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "data")
+
+
+@dataclass
+class Case:
+    name: str
+    control: Control
+    mech: _chem.Mechanism
+    tables: _chem.ChemTables
+    diff: Optional[_diff.DiffTables]
+    dump: Dump
+    chem_path: str
+    diff_path: Optional[str]
+
+    @property
+    def nspec(self):
+        return self.tables.nspec
+
+    @property
+    def shape(self):
+        return (self.control.nx, self.control.ny, self.control.nz)
+
+    def write_inputs(self, rundir: str) -> None:
+        """Lay out input/ and output/ as the reference expects (src/indata.f:54-136)."""
+        os.makedirs(os.path.join(rundir, "input"), exist_ok=True)
+        os.makedirs(os.path.join(rundir, "output"), exist_ok=True)
+        self.control.write(os.path.join(rundir, "input", "cont.dat"))
+        _chem.write_chem(self.mech, os.path.join(rundir, "input", "chem.dat"))
+        if self.diff_path:
+            with open(self.diff_path) as f, open(os.path.join(rundir, "input", "diff.dat"), "w") as g:
+                g.write(f.read())
+        write_dump(os.path.join(rundir, "output", "dmpi0000000.dat"), self.dump)
+
+
+def sound_speed_max(tables, state):
+    """max(|u|+a) over the field, a^2 = gamma R T with gamma from the mixture cp."""
+    rho = state[0]
+    u = state[1] / rho; v = state[2] / rho; w = state[3] / rho
+    Y = [state[5 + i] / rho for i in range(tables.nspec)]
+    T = _case_temperature(tables, state)
+    rmix = sum(tables.rgspec[i] * Y[i] for i in range(tables.nspec))
+    cp = np.zeros_like(T)
+    for i in range(tables.nspec):
+        it = np.where(T > tables.tinthi[0, i], 1, 0) if tables.ntint[i] > 1 else np.zeros(T.shape, int)
+        f = tables.amascp[4, it, i]
+        for c in range(3, -1, -1):
+            f = f * T + tables.amascp[c, it, i]
+        cp += f * Y[i]
+    a = np.sqrt(cp / (cp - rmix) * rmix * T)
+    return float(np.max(np.maximum(np.abs(u), np.maximum(np.abs(v), np.abs(w))) + a))
+
+
+def _case_temperature(tables, state):
+    """Temperature from the conservative state by bisection on E(T) (host side,
+    used only to size the time step)."""
+    rho = state[0]
+    Y = [state[5 + i] / rho for i in range(tables.nspec)]
+    ke = 0.5 * (state[1] ** 2 + state[2] ** 2 + state[3] ** 2) / rho ** 2
+    e = state[4] / rho - ke
+    rmix = sum(tables.rgspec[i] * Y[i] for i in range(tables.nspec))
+    lo = np.full(rho.shape, 200.0); hi = np.full(rho.shape, 3500.0)
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        em = _ic.enthalpy(tables, mid, Y) - rmix * mid
+        hi = np.where(em > e, mid, hi)
+        lo = np.where(em > e, lo, mid)
+    return 0.5 * (lo + hi)
+
+
+def _f32(x: float) -> float:
+    """Round a time step to the nearest float32 value.  On a restart with a
+    fixed step the reference keeps cont.dat's TSTEP through an implicitly typed
+    REAL variable (TSOLD, src/indata.f:1423,1481), so only steps exactly
+    representable in single precision reach its RK coefficients unchanged."""
+    return float(np.float32(x))
+
+
+def one_step_mechanism() -> _chem.Mechanism:
+    """R -> P, single irreversible step.  Thermo: the reference's 1-step case
+    (src/casefiles/1step/chem.dat, W = 28.84 for both species); the product's
+    enthalpy constant is set so that T_b = T_u/(1-alpha) = 1200 K from T_u = 300 K
+    (alpha = 0.75), and E/R0 = Ze*T_b^2/(T_b-T_u) = 9600 K with Ze = 6."""
+    base = _chem.read_chem(os.path.join(DATA, "onestep_base", "chem.dat"))
+    m = base
+    t = _chem.derive(m)
+    Tu, Tb = 300.0, 1200.0
+    # sensible enthalpy rise of the reactant from Tu to Tb sets the heat release
+    hR_u = float(_ic.enthalpy(t, np.array([Tu]), [np.array([1.0]), np.array([0.0])])[0])
+    hR_b = float(_ic.enthalpy(t, np.array([Tb]), [np.array([1.0]), np.array([0.0])])[0])
+    hP_b = float(_ic.enthalpy(t, np.array([Tb]), [np.array([0.0]), np.array([1.0])])[0])
+    # want h_P(Tb) = h_R(Tu): shift the product's a6 (both intervals) by dh*W/R0
+    dh = hR_u - hP_b
+    shift = dh * m.wmolar[1] / _chem.RGUNIV
+    m.amolcp[5, :, 1] += shift
+    m.clewis[:] = 1.0
+    ze = 6.0
+    m.rparam[:, 0] = [2.75e7, 0.0, ze * Tb * Tb / (Tb - Tu) * _chem.RGUNIV]
+    del hR_b
+    return m
+
+
+def _fmt_roundtrip(m: _chem.Mechanism, path: str) -> _chem.Mechanism:
+    """Write the mechanism to chem.dat and read it back, so that every consumer
+    (reference, oracle, GPU) sees the values as printed in the file."""
+    _chem.write_chem(m, path)
+    return _chem.read_chem(path)
+
+
+def periodic_1step(n=(32, 32, 32), length=2.0e-3 * math.pi, seed=1, uprime=2.0,
+                   workdir=None, cfl=0.5, nsteps=10) -> Case:
+    nx, ny, nz = n
+    workdir = workdir or os.path.join(DATA, "_gen")
+    os.makedirs(workdir, exist_ok=True)
+    mech = _fmt_roundtrip(one_step_mechanism(), os.path.join(workdir, f"onestep_{nx}.chem.dat"))
+    tab = _chem.derive(mech)
+    L = (length, length, length)
+    d = tuple(L[i] / (n[i] - 1) for i in range(3))
+    x = _ic.grid(nx, L[0])
+    period = nx * d[0]
+    delta = 3.0 * d[0]
+    c = 0.5 * (np.vectorize(math.erf)((x - 0.3 * period) / delta)
+               - np.vectorize(math.erf)((x - 0.7 * period) / delta))
+    Tu, Tb = 300.0, 1200.0
+    shape = (nz, ny, nx)
+    C = np.broadcast_to(c, shape)
+    T = Tu + (Tb - Tu) * C
+    Y = [1.0 - C, C.copy()]
+    u, v, w = _ic.solenoidal_field(shape, (d[2], d[1], d[0]), uprime, 2.0 * 2.0 * delta, seed,
+                                   spectrum="batchelor-townsend")
+    st = _ic.conservative_state(tab, T, np.full(shape, 1.0e5), Y, u, v, w)
+    dt = _f32(cfl * min(d) / sound_speed_max(tab, st))
+    ctl = Control(L[0], L[1], L[2], nx, ny, nz, dt, 1, nsteps, 0, prin=1.0e5, trin=Tu,
+                  yrin=[1.0, 0.0])
+    ctl.bc = [(1, [0, 0, 0, 0], [0.0] * 4)] * 6
+    chem_path = os.path.join(workdir, f"onestep_{nx}.chem.dat")
+    return Case(f"periodic_1step_{nx}x{ny}x{nz}", ctl, mech, tab, None,
+                Dump(st, 0.0, dt), chem_path, None)
+
+
+def h2_mechanism() -> _chem.Mechanism:
+    return _chem.read_chem(os.path.join(DATA, "h2_9sp", "chem.dat"))
+
+
+def h2_nscbc(n=(128, 128, 128), length=(5.0e-3, 5.0e-3, 5.0e-3), seed=7, phi=0.7,
+             uin=1.25, uprime_over_sl=5.0, sl=1.2, mixavg=True, cfl=0.5, nsteps=100,
+             periodic_x=False, radicals=True) -> Case:
+    nx, ny, nz = n
+    mech = h2_mechanism()
+    tab = _chem.derive(mech)
+    dif = _diff.read_diff(os.path.join(DATA, "h2_9sp", "diff.dat"), mech.wmolar) if mixavg else None
+    W = mech.wmolar
+    # unburnt phi-mixture of H2 + air(O2 + 3.76 N2), products H2O + excess O2 + N2
+    nO2 = 0.5 / phi
+    mol_u = {"H2": 1.0, "O2": nO2, "N2": 3.76 * nO2}
+    mol_b = {"H2O": 1.0, "O2": nO2 - 0.5, "N2": 3.76 * nO2}
+    idx = {s: i for i, s in enumerate(mech.species)}
+
+    def massfrac(mol):
+        y = np.zeros(mech.nspec)
+        for s, nm in mol.items():
+            y[idx[s]] = nm * W[idx[s]]
+        return y / y.sum()
+
+    Yu, Yb = massfrac(mol_u), massfrac(mol_b)
+    Tu = 300.0
+    Tb = _ic.adiabatic_temperature(tab, Yu, Yb, Tu)
+    d = tuple(length[i] / (n[i] - 1) for i in range(3))
+    x = _ic.grid(nx, length[0])
+    shape = (nz, ny, nx)
+    delta = 0.35e-3
+    if periodic_x:
+        period = nx * d[0]
+        c = 0.5 * (np.vectorize(math.erf)((x - 0.3 * period) / delta)
+                   - np.vectorize(math.erf)((x - 0.7 * period) / delta))
+    else:
+        c = _ic.erf_profile(x, 0.5 * length[0], delta)
+    C = np.broadcast_to(c, shape)
+    T = Tu + (Tb - Tu) * C
+    Y = [Yu[i] * (1.0 - C) + Yb[i] * C for i in range(mech.nspec)]
+    if radicals:
+        bump = np.exp(-((C - 0.5) / 0.2) ** 2)
+        for s, amp in (("O", 2e-3), ("OH", 5e-3), ("H", 3e-4), ("HO2", 1e-4), ("H2O2", 2e-5)):
+            Y[idx[s]] = Y[idx[s]] + amp * bump
+        tot = sum(Y)
+        # remove the radical mass from N2 so that sum(Y) = 1
+        Y[idx["N2"]] = Y[idx["N2"]] - (tot - 1.0)
+    up = uprime_over_sl * sl
+    u, v, w = _ic.solenoidal_field(shape, (d[2], d[1], d[0]), up, 1.0e-3, seed,
+                                   spectrum="passot-pouquet")
+    if not periodic_x:
+        # taper the fluctuation to zero at the x boundaries (inflow/outflow planes)
+        taper = np.sin(np.pi * x / length[0]) ** 2
+        u = u * taper; v = v * taper; w = w * taper
+    u = u + uin
+    st = _ic.conservative_state(tab, T, np.full(shape, 1.0e5), Y, u, v, w)
+    dt = _f32(cfl * min(d) / sound_speed_max(tab, st))
+    ctl = Control(length[0], length[1], length[2], nx, ny, nz, dt, 1, nsteps, 0, prin=1.0e5,
+                  trin=Tu, urin=uin, yrin=[float(y) for y in Yu])
+    per = (1, [0, 0, 0, 0], [0.0] * 4)
+    if periodic_x:
+        ctl.bc = [per] * 6
+    else:
+        ctl.bc = [(13, [1, 0, 0, 0], [uin, 0.0, 0.0, 0.0]),
+                  (21, [0, 0, 0, 0], [1.0e5, 0.287, 1.0e-3, 0.0]), per, per, per, per]
+    ctl.nfmavt = 1 if mixavg else 0
+    name = f"h2_{'per' if periodic_x else 'nscbc'}_{'mix' if mixavg else 'cle'}_{nx}x{ny}x{nz}"
+    return Case(name, ctl, mech, tab, dif, Dump(st, 0.0, dt),
+                os.path.join(DATA, "h2_9sp", "chem.dat"),
+                os.path.join(DATA, "h2_9sp", "diff.dat") if mixavg else None)
