@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Throughput of the SENGA+ RK substage on one MI355X.
+
+Workload: the 128^3 H2-air case with NSCBC inflow/outflow in x, periodic y/z,
+9 species / 21 steps with Lindemann falloff, NASA-7 thermodynamics and
+mixture-averaged transport, FP64, seeded Passot-Pouquet turbulence
+(sengapy.case.h2_nscbc).  A "step" is one full RK4(3)5 time step (five
+right-hand-side evaluations and register updates); the metric counts
+grid-point-substages per second with the state resident in HBM.
+
+Output: one JSON line.  roofline: the dominant kernel's algorithmic HBM bytes
+per launch over its mean duration (HIP events on the library's stream);
+cpu_baseline: the C oracle (a scalar restatement of the reference, one core)
+on a bounded sample of the same physics.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "senga-versiona_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK = 8.0e12   # MI355X HBM3E, bytes/s
+
+
+def hip():
+    lib = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    lib.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+    lib.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+    lib.hipEventSynchronize.argtypes = [C.c_void_p]
+    lib.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    lib.hipDeviceSynchronize.argtypes = []
+    return lib
+
+
+def kernel_bytes(name, ns, bounded):
+    """Algorithmic bytes per grid point of one launch: each distinct input
+    field read once and each output written once (FP64 fields, 4-byte index)."""
+    f = 8
+    nprim = 9 + 3 * ns
+    if name == "prim":          # reads stage (5+N), T, index; writes primitives, rate, index
+        return f * (5 + ns + 1) + 4 + f * (nprim + ns) + 4
+    if name == "pass_x":        # rho, 3 momenta, primitives, rate -> accumulators, 4 gradients
+        return f * (4 + nprim + ns) + f * (5 + ns) + f * 4
+    if name in ("pass_y", "pass_z"):   # accumulators read-modify-write
+        return f * (4 + nprim) + 2 * f * (5 + ns) + f * 4
+    if name == "final":         # accumulators, 12 gradients, u,v,w,mu, run (r/w), stage (w)
+        return f * (5 + ns) + f * 12 + f * 4 + 2 * f * (5 + ns) + f * (5 + ns)
+    return 0
+
+
+def cpu_baseline(seconds_target=15.0):
+    """Oracle (scalar C restatement, one thread) on a 32^3 sample of the same
+    H2 NSCBC mixture-averaged physics; whole RK steps."""
+    from pyoracle import Oracle
+    from sengapy import case as Cs
+    c = Cs.h2_nscbc(n=(32, 32, 32), length=(1.25e-3, 1.25e-3, 1.25e-3), mixavg=True, nsteps=1)
+    o = Oracle(c)
+    o.set_timestep(c.dump.tstep)
+    o.set_state(c.dump.state, 0.0)
+    npts = 32 ** 3
+    t0 = time.perf_counter()
+    nsub = 0
+    while True:
+        for r in range(1, 6):
+            o.substage(r)
+            nsub += 1
+        o.lib.orc_end_step(o.h)
+        if time.perf_counter() - t0 > seconds_target:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": npts * nsub / dt / 1e6, "unit": "Mpoint-RK-substages/s", "cores": 1, "kind": "port",
+            "sample": f"C oracle, 32^3 H2-air NSCBC mixture-averaged FP64, {nsub} substages "
+                      f"({nsub // 5} RK steps) in {dt:.1f} s on one host core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=2)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (multi-GPU sharding is "
+              "design-only in this build)", file=sys.stderr)
+        sys.exit(2)
+    from sengapy import abi
+    from sengapy import case as Cs
+    n = a.n
+    t0 = time.perf_counter()
+    c = Cs.h2_nscbc(n=(n, n, n), length=(5.0e-3, 5.0e-3, 5.0e-3), mixavg=True, nsteps=a.steps)
+    print(f"case built in {time.perf_counter() - t0:.1f} s, dt = {c.dump.tstep:.3e}", file=sys.stderr, flush=True)
+    s = abi.Solver(c)
+    s.set_timestep(c.dump.tstep)
+    s.upload(c.dump.state, 0.0)
+    h = hip()
+    stream = C.c_void_p(s.stream())
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    h.hipEventCreate(C.byref(ev0))
+    h.hipEventCreate(C.byref(ev1))
+    s.step(a.warmup)
+    s.sync()
+    h.hipDeviceSynchronize()
+    h.hipEventRecord(ev0, stream)
+    s.step(a.steps)
+    h.hipEventRecord(ev1, stream)
+    h.hipEventSynchronize(ev1)
+    s.sync()
+    ms = C.c_float(0)
+    h.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+    total_s = ms.value / 1e3
+    npts = n ** 3
+    value = npts * 5 * a.steps / total_s / 1e6
+    print(f"timed {a.steps} steps: {ms.value:.2f} ms", file=sys.stderr, flush=True)
+    # per-kernel durations with events around each launch (separate, profiled steps)
+    s.profile(True)
+    s.step(a.profile_steps)
+    s.sync()
+    prof = s.profile_read()
+    s.profile(False)
+    ns = c.nspec
+    kern = {k: v for k, v in prof.items() if v[1] > 0 and kernel_bytes(k, ns, True) > 0}
+    dom = max(kern, key=lambda k: kern[k][0])
+    dom_ms = kern[dom][0] / kern[dom][1]
+    dom_bytes = kernel_bytes(dom, ns, True) * npts
+    achieved = dom_bytes / (dom_ms / 1e3)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        t = json.load(open(tfile))
+        if t.get("workload") == f"h2_nscbc_mix_{n}^3" and dom in t.get("kernels", {}):
+            traffic = t["kernels"][dom]
+    substage_bytes = 32 * (5 + ns) + 16      # compulsory state traffic of one fused substage
+    out = {
+        "metric": "Mpoint-RK-substages/s",
+        "value": round(value, 3),
+        "unit": "Mpoint-RK-substages/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms.value / a.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded turbulence + planar erf flame initial dump)",
+        "config": {"workload": f"H2-air 9-species 21-step, {n}^3, NSCBC x inflow/outflow, periodic y/z, "
+                               "mixture-averaged transport, FP64, RK4(3)5",
+                   "grid": [n, n, n], "nspec": ns, "nstep_chem": c.tables.nstep, "tstep": c.dump.tstep},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "algorithmic_bytes_per_point": kernel_bytes(dom, ns, True),
+                     "mean_launch_ms": dom_ms},
+        "substage_roofline_frac": substage_bytes * value * 1e6 / HBM_PEAK,
+        "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kern.items()},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,468 @@
+// sg_api.hip -- C ABI of the MI355X SENGA+ right-hand-side library.
+//
+// The handle owns the device registers of one case and one HIP stream.  The
+// substage sequence mirrors src/senga2.f:113-159; see include/senga_gpu.h for
+// the contract of each entry point.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "sg_internal.h"
+
+using namespace sg;
+
+namespace {
+
+enum KernelId { K_TEMPIN = 0, K_BOUNDT, K_PRIM, K_PASSX, K_PASSY, K_PASSZ, K_FINAL, K_BOUNTT, K_DERIVS, K_COUNT };
+const char *kKernelNames[K_COUNT] = {"tempin", "boundt", "prim", "pass_x", "pass_y", "pass_z",
+                                     "final", "bountt", "derivs"};
+
+}  // namespace
+
+struct sg_handle {
+  sg_config cfg;       // host copy (species lists 0-based, REAL-rounded falloff constants)
+  sg_config *dcfg;     // device copy
+  Params P;
+  Fields F;
+  hipStream_t stream;
+  int device;
+  void *base;          // single device allocation
+  size_t bytes;
+  double etime, tstep;
+  double rklhs[5], rkrhs[5], rkerr[5], rktim[5];
+  int have_state;
+  int err_code;
+  int err_ijk[3];
+  std::string err_msg;
+  // profiling
+  int prof_on;
+  hipEvent_t ev[2];
+  double prof_ms[K_COUNT];
+  int64_t prof_n[K_COUNT];
+};
+
+static int set_err(sg_handle *h, int code, const char *msg, int i = 0, int j = 0, int k = 0) {
+  if (h) {
+    h->err_code = code;
+    h->err_ijk[0] = i;
+    h->err_ijk[1] = j;
+    h->err_ijk[2] = k;
+    h->err_msg = msg;
+  }
+  return code;
+}
+
+#define HIPCHK(h, call)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      char b_[256];                                                                  \
+      snprintf(b_, sizeof b_, "%s: %s", #call, hipGetErrorString(e_));               \
+      return set_err((h), SG_EDEVICE, b_);                                           \
+    }                                                                                \
+  } while (0)
+
+// Launch a kernel through fn, timing it with events on the handle's stream
+// when profiling is on.
+template <class Fn>
+static int timed(sg_handle *h, int kid, Fn fn) {
+  if (h->prof_on) hipEventRecord(h->ev[0], h->stream);
+  fn();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    char b[256];
+    snprintf(b, sizeof b, "launch of %s failed: %s", kKernelNames[kid], hipGetErrorString(e));
+    return set_err(h, SG_EDEVICE, b);
+  }
+  if (h->prof_on) {
+    hipEventRecord(h->ev[1], h->stream);
+    hipEventSynchronize(h->ev[1]);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+    h->prof_ms[kid] += ms;
+    h->prof_n[kid] += 1;
+  }
+  return 0;
+}
+
+// Stencil coefficients (src/dfinit.f:61-477), flags and ranges (src/bcinit.f).
+static int setup_params(sg_handle *h) {
+  const sg_config &c = h->cfg;
+  Params &P = h->P;
+  memset(&P, 0, sizeof P);
+  P.nx = c.nx; P.ny = c.ny; P.nz = c.nz; P.ns = c.nspec;
+  P.sy = c.nx; P.sz = (long)c.nx * c.ny;
+  P.npts = (long)c.nx * c.ny * c.nz;
+  P.fstride = (P.npts + 31) / 32 * 32;   // 256-byte aligned fields
+  const double L[3] = {c.xgdlen, c.ygdlen, c.zgdlen};
+  const int N[3] = {c.nx, c.ny, c.nz};
+  for (int d = 0; d < 3; d++) {
+    double delta = L[d] / (double)(float)(N[d] - 1);   // src/dfinit.f:42-50
+    P.ovdel[d] = 1.0 / delta;
+    P.ovdl2[d] = P.ovdel[d] * P.ovdel[d];
+  }
+  const double acoeff = 5.0 / 3.0, bcoeff = -2.0e1 / 2.1e1, ccoeff = 5.0 / 1.4e1,
+               dcoeff = -5.0 / 6.3e1, ecoeff = 1.0 / 1.26e2;
+  P.ac[0] = acoeff / 2.0; P.ac[1] = bcoeff / 4.0; P.ac[2] = ccoeff / 6.0; P.ac[3] = dcoeff / 8.0;
+  P.ac[4] = ecoeff / 1.0e1;
+  const double acoef1 = 4.0, bcoef1 = -3.0, ccoef1 = 4.0 / 3.0, dcoef1 = -1.0 / 4.0;
+  const double acoef2 = -1.0 / 4.0, bcoef2 = 3.0 / 2.0, ccoef2 = -1.0 / 2.0, dcoef2 = 1.0 / 1.2e1;
+  const double acoef3 = 4.0 / 3.0, bcoef3 = -1.0 / 3.0;
+  const double acoef4 = 3.0 / 2.0, bcoef4 = -3.0 / 5.0, ccoef4 = 1.0 / 1.0e1;
+  const double acoef5 = 8.0 / 5.0, bcoef5 = -4.0 / 5.0, ccoef5 = 8.0 / 3.5e1, dcoef5 = -1.0 / 3.5e1;
+  double c1[4] = {acoef1, bcoef1, ccoef1, dcoef1}, c2[4] = {acoef2, bcoef2, ccoef2, dcoef2};
+  memcpy(P.c1, c1, sizeof c1); memcpy(P.c2, c2, sizeof c2);
+  P.c3[0] = acoef3 / 2.0; P.c3[1] = bcoef3 / 4.0;
+  P.c4[0] = acoef4 / 2.0; P.c4[1] = bcoef4 / 4.0; P.c4[2] = ccoef4 / 6.0;
+  P.c5[0] = acoef5 / 2.0; P.c5[1] = bcoef5 / 4.0; P.c5[2] = ccoef5 / 6.0; P.c5[3] = dcoef5 / 8.0;
+  P.sc[0] = acoeff; P.sc[1] = bcoeff / 4.0; P.sc[2] = ccoeff / 9.0; P.sc[3] = dcoeff / 1.6e1;
+  P.sc[4] = ecoeff / 2.5e1;
+  double s1[5] = {-7.7e1 / 6.0, 1.07e2 / 6.0, -1.3e1, 6.1e1 / 1.2e1, -5.0 / 6.0};
+  double s2[5] = {5.0 / 6.0, -1.0 / 3.0, 7.0 / 6.0, -1.0 / 2.0, 1.0 / 1.2e1};
+  memcpy(P.s1, s1, sizeof s1); memcpy(P.s2, s2, sizeof s2);
+  P.s3[0] = acoef3; P.s3[1] = bcoef3 / 4.0;
+  P.s4[0] = acoef4; P.s4[1] = bcoef4 / 4.0; P.s4[2] = ccoef4 / 9.0;
+  P.s5[0] = acoef5; P.s5[1] = bcoef5 / 4.0; P.s5[2] = ccoef5 / 9.0; P.s5[3] = dcoef5 / 1.6e1;
+  P.mc[0] = acoeff / 4.0; P.mc[1] = bcoeff / 4.0 / 4.0; P.mc[2] = ccoeff / 4.0 / 9.0;
+  P.mc[3] = dcoeff / 4.0 / 1.6e1; P.mc[4] = ecoeff / 4.0 / 2.5e1;
+  P.my1[0] = acoef3 / 2.0; P.my1[1] = bcoef3 / 4.0;
+  memcpy(P.m1, c1, sizeof c1); memcpy(P.m2, c2, sizeof c2);
+  P.m3[0] = acoef3 / 4.0; P.m3[1] = bcoef3 / 4.0 / 4.0;
+  P.m4[0] = acoef4 / 4.0; P.m4[1] = bcoef4 / 4.0 / 4.0; P.m4[2] = ccoef4 / 4.0 / 9.0;
+  P.m5[0] = acoef5 / 4.0; P.m5[1] = bcoef5 / 4.0 / 4.0; P.m5[2] = ccoef5 / 4.0 / 9.0;
+  P.m5[3] = dcoef5 / 4.0 / 1.6e1;
+  P.thrd = 1.0 / 3.0; P.tthd = 2.0 * P.thrd; P.fthd = 2.0 * P.tthd;   // src/com_senga2.h:271-272
+  // boundaries: y, z periodic; x periodic or NSBCI3 (left) + NSBCO1 (right)
+  for (int f = 2; f < 6; f++)
+    if (c.bc_type[f] != SG_BC_PERIODIC) return set_err(h, SG_EINVAL, "y and z must be periodic");
+  bool perx = c.bc_type[0] == SG_BC_PERIODIC;
+  if (perx != (c.bc_type[1] == SG_BC_PERIODIC)) return set_err(h, SG_EINVAL, "x ends must both be periodic or both bounded");
+  for (int v = 0; v < 6; v++) P.rklo[v] = 0;
+  if (!perx) {
+    if (c.bc_type[0] != SG_BC_INFLOW3 || c.bc_type[1] != SG_BC_OUTFLOW1)
+      return set_err(h, SG_EINVAL, "bounded x supports NSBCI3 (13) at x-left with NSBCO1 (21) at x-right");
+    if (c.bc_ipar[0][0] != 1) return set_err(h, SG_EINVAL, "NSBCI3 supports the constant-velocity option (I1=1) only");
+    if (c.bc_ipar[1][0] != 0 || c.bc_ipar[1][1] != 0)
+      return set_err(h, SG_EINVAL, "NSBCO1 supports I1=I2=0 only");
+    P.bnd[0] = 1;
+    P.inflow_xl = 1; P.outflow_xr = 1;
+    P.fcnv[0] = 1; P.fvsn[0] = 1;
+    P.rklo[0] = P.rklo[1] = P.rklo[2] = P.rklo[3] = P.rklo[5] = 1;
+    P.fcnv[1] = 1; P.fvst[1] = 1; P.fcon[1] = 1; P.fadb[1] = 1; P.fdif[1] = 1;
+    P.cobcxr = c.bc_rpar[1][1] * (1.0 - c.bc_rpar[1][2]) / (2.0 * c.xgdlen);
+    P.pinfxr = c.bc_rpar[1][0];
+    P.uin = c.bc_rpar[0][0];
+  }
+  double rtin = 0.0;   // src/indata.f:1178-1183
+  for (int s = 0; s < c.nspec; s++) rtin = rtin + c.rgspec[s] * c.yrin[s];
+  rtin = rtin * c.trin;
+  P.drin = c.prin / rtin;
+  P.mixavg = c.mixavg;
+  P.alamda = c.alamda; P.rlamda = c.rlamda; P.prantl = c.prantl;
+  P.tdifgb = c.tdifgb; P.pdifgb = c.pdifgb;
+  return 0;
+}
+
+extern "C" {
+
+int sg_abi_version(void) { return SG_ABI_VERSION; }
+size_t sg_config_size(void) { return sizeof(sg_config); }
+
+int sg_create(const sg_config *cfg, sg_handle **out) {
+  if (!cfg || !out) return SG_EINVAL;
+  *out = nullptr;
+  if (!species_supported(cfg->nspec)) return SG_EINVAL;   // compiled species counts
+  if (cfg->nstep < 0 || cfg->nstep > SG_MAX_STEP) return SG_EINVAL;
+  if (cfg->nx < 11 || cfg->ny < 11 || cfg->nz < 11) return SG_EINVAL;
+  if ((long)cfg->nx * cfg->ny * 8 >= (1L << 31)) return SG_EINVAL;
+  if (cfg->precision != 0) return SG_EINVAL;
+  if (cfg->mixavg && (cfg->ncovis < 1 || cfg->ncocon < 1 || cfg->ncodif < 1 || cfg->ncovis > SG_NTRC ||
+                      cfg->ncocon > SG_NTRC || cfg->ncodif > SG_NTRC))
+    return SG_EINVAL;
+  sg_handle *h = new sg_handle();
+  h->cfg = *cfg;
+  // The reference reads these falloff constants through implicitly typed
+  // REAL locals (FLCNST, TALPHA, CFCST1/2, ENCST1/2, DTCNST in
+  // src/chrate.f:249,319-330); round them once here.
+  for (int j = 0; j < SG_MAX_FALL; j++) {
+    h->cfg.rclind[j][3] = (double)(float)h->cfg.rclind[j][3];
+    const int tr[6] = {3, 7, 8, 9, 10, 11};
+    for (int q = 0; q < 6; q++) h->cfg.rctroe[j][tr[q]] = (double)(float)h->cfg.rctroe[j][tr[q]];
+  }
+  int rc = setup_params(h);
+  if (rc) { delete h; return rc; }
+  if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return SG_EDEVICE;
+  }
+  const Params &P = h->P;
+  int ns = P.ns;
+  long fs = P.fstride;
+  long nfields = (5 + ns) * 2 + nprim(ns) + ns + (5 + ns) + 12;
+  size_t bytes = sizeof(double) * (size_t)fs * nfields;
+  size_t itn = ((sizeof(uint32_t) * (size_t)fs + 255) / 256) * 256;
+  size_t facesz = sizeof(double) * 2 * (size_t)nface(ns) * P.ny * P.nz;
+  size_t cfgsz = ((sizeof(sg_config) + 255) / 256) * 256;
+  h->bytes = bytes + itn + ((facesz + 255) / 256) * 256 + cfgsz + 256;
+  if (hipMalloc(&h->base, h->bytes) != hipSuccess) {
+    hipStreamDestroy(h->stream);
+    delete h;
+    return SG_ENOMEM;
+  }
+  char *b = (char *)h->base;
+  double *d = (double *)b;
+  Fields &F = h->F;
+  F.q = d; d += (5 + ns) * fs;
+  F.run = d; d += (5 + ns) * fs;
+  F.prim = d; d += nprim(ns) * fs;
+  F.rate = d; d += ns * fs;
+  F.acc = d; d += (5 + ns) * fs;
+  F.grad = d; d += 12 * fs;
+  b = (char *)d;
+  F.itndex = (uint32_t *)b; b += itn;
+  F.face = (double *)b; b += ((facesz + 255) / 256) * 256;
+  h->dcfg = (sg_config *)b; b += cfgsz;
+  F.err = (int *)b;
+  if (hipMemset(h->base, 0, h->bytes) != hipSuccess ||
+      hipMemcpy(h->dcfg, &h->cfg, sizeof(sg_config), hipMemcpyHostToDevice) != hipSuccess) {
+    hipFree(h->base);
+    hipStreamDestroy(h->stream);
+    delete h;
+    return SG_EDEVICE;
+  }
+  hipEventCreate(&h->ev[0]);
+  hipEventCreate(&h->ev[1]);
+  h->etime = 0.0;
+  sg_set_timestep(h, 0.0);
+  *out = h;
+  return 0;
+}
+
+int sg_destroy(sg_handle *h) {
+  if (!h) return SG_EINVAL;
+  hipStreamSynchronize(h->stream);
+  hipEventDestroy(h->ev[0]);
+  hipEventDestroy(h->ev[1]);
+  hipFree(h->base);
+  hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+static int check_device_error(sg_handle *h) {
+  int e[4];
+  HIPCHK(h, hipMemcpyAsync(e, h->F.err, sizeof e, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (e[0]) {
+    const char *msg = e[0] == SG_ETEMPER ? "TEMPER: T iteration failed to converge"
+                                         : "TEMPIN: temperature bisection failed";
+    char b[200];
+    snprintf(b, sizeof b, "%s at point %d %d %d", msg, e[1], e[2], e[3]);
+    int z[4] = {0, 0, 0, 0};
+    hipMemcpy(h->F.err, z, sizeof z, hipMemcpyHostToDevice);
+    return set_err(h, e[0], b, e[1], e[2], e[3]);
+  }
+  return 0;
+}
+
+int sg_upload_state(sg_handle *h, const double *run, double etime) {
+  if (!h || !run) return set_err(h, SG_EINVAL, "null argument");
+  const Params &P = h->P;
+  size_t fb = sizeof(double) * P.npts;
+  for (int v = 0; v < 5 + P.ns; v++) {
+    HIPCHK(h, hipMemcpyAsync(h->F.run + v * P.fstride, run + v * P.npts, fb, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->F.q + v * P.fstride, run + v * P.npts, fb, hipMemcpyHostToDevice, h->stream));
+  }
+  HIPCHK(h, hipMemsetAsync(h->F.rate, 0, sizeof(double) * P.fstride * P.ns, h->stream));
+  int rc = timed(h, K_TEMPIN, [&] { launch_tempin(P, h->dcfg, h->F, h->stream); });
+  if (rc) return rc;
+  rc = check_device_error(h);
+  if (rc) return rc;
+  h->etime = etime;
+  h->have_state = 1;
+  return 0;
+}
+
+int sg_download_state(sg_handle *h, double *run, double *temp, double *pres, double *rate) {
+  if (!h) return SG_EINVAL;
+  if (!h->have_state) return set_err(h, SG_ESTATE, "no state uploaded");
+  const Params &P = h->P;
+  size_t fb = sizeof(double) * P.npts;
+  if (run)
+    for (int v = 0; v < 5 + P.ns; v++)
+      HIPCHK(h, hipMemcpyAsync(run + v * P.npts, h->F.run + v * P.fstride, fb, hipMemcpyDeviceToHost, h->stream));
+  if (temp) HIPCHK(h, hipMemcpyAsync(temp, h->F.prim + P_T * P.fstride, fb, hipMemcpyDeviceToHost, h->stream));
+  if (pres) HIPCHK(h, hipMemcpyAsync(pres, h->F.prim + P_P * P.fstride, fb, hipMemcpyDeviceToHost, h->stream));
+  if (rate)
+    for (int s = 0; s < P.ns; s++)
+      HIPCHK(h, hipMemcpyAsync(rate + s * P.npts, h->F.rate + s * P.fstride, fb, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return check_device_error(h);
+}
+
+// src/dtinit.f:76-94 (RK4(3)5[2R+]C, Kennedy-Carpenter) and 196-223
+int sg_set_timestep(sg_handle *h, double dt) {
+  if (!h) return SG_EINVAL;
+  const double acofrk[5] = {970286171893.0 / 4311952581923.0, 6584761158862.0 / 12103376702013.0,
+                            2251764453980.0 / 15575788980749.0, 26877169314380.0 / 34165994151039.0, 0.0};
+  const double bcofrk[5] = {1153189308089.0 / 22510343858157.0, 1772645290293.0 / 4653164025191.0,
+                            -1672844663538.0 / 4480602732383.0, 2114624349019.0 / 3568978502595.0,
+                            5198255086312.0 / 14908931495163.0};
+  const double bhatrk[5] = {1016888040809.0 / 7410784769900.0, 11231460423587.0 / 58533540763752.0,
+                            -1563879915014.0 / 6823010717585.0, 606302364029.0 / 971179775848.0,
+                            1097981568119.0 / 3980877426909.0};
+  h->tstep = dt;
+  for (int r = 0; r < 5; r++) {
+    h->rklhs[r] = bcofrk[r] * dt;
+    h->rkrhs[r] = acofrk[r] * dt;
+    h->rkerr[r] = (bcofrk[r] - bhatrk[r]) * dt;
+  }
+  h->rktim[0] = 0.0;
+  for (int r = 1; r < 5; r++) {
+    double t = acofrk[r - 1];
+    for (int q = 0; q < r - 1; q++) t = t + bcofrk[q];
+    h->rktim[r] = t * dt;
+  }
+  return 0;
+}
+
+// BOUNDT .. BOUNDS (src/senga2.f:118-131): everything up to the register update.
+static int eval_rhs(sg_handle *h) {
+  const Params &P = h->P;
+  int rc = 0;
+  if (P.inflow_xl) rc = timed(h, K_BOUNDT, [&] { launch_boundt(P, h->dcfg, h->F, h->stream); });
+  if (!rc) rc = timed(h, K_PRIM, [&] { launch_prim(P, h->dcfg, h->F, h->stream); });
+  if (!rc) rc = timed(h, K_PASSX, [&] { launch_pass(0, P, h->F, h->stream); });
+  if (!rc) rc = timed(h, K_PASSY, [&] { launch_pass(1, P, h->F, h->stream); });
+  if (!rc) rc = timed(h, K_PASSZ, [&] { launch_pass(2, P, h->F, h->stream); });
+  return rc;
+}
+
+int sg_substage(sg_handle *h, int irk) {
+  if (!h) return SG_EINVAL;
+  if (!h->have_state) return set_err(h, SG_ESTATE, "no state uploaded");
+  if (irk < 1 || irk > 5) return set_err(h, SG_EINVAL, "irk must be 1..5");
+  int rc = eval_rhs(h);
+  if (rc) return rc;
+  int mode = irk < 5 ? 0 : 1;
+  double lhs = h->rklhs[irk - 1], rhs = h->rkrhs[irk - 1];
+  return timed(h, K_FINAL, [&] { launch_final(h->P, h->dcfg, h->F, mode, lhs, rhs, nullptr, h->stream); });
+}
+
+int sg_end_step(sg_handle *h) {
+  if (!h) return SG_EINVAL;
+  h->etime = h->etime + h->tstep;
+  if (h->P.inflow_xl) return timed(h, K_BOUNTT, [&] { launch_bountt(h->P, h->dcfg, h->F, h->stream); });
+  return 0;
+}
+
+int sg_step(sg_handle *h, int nsteps) {
+  if (!h) return SG_EINVAL;
+  for (int n = 0; n < nsteps; n++) {
+    for (int r = 1; r <= 5; r++) {
+      int rc = sg_substage(h, r);
+      if (rc) return rc;
+    }
+    int rc = sg_end_step(h);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int sg_rhs(sg_handle *h, int irk, double *rhs) {
+  if (!h || !rhs) return SG_EINVAL;
+  if (!h->have_state) return set_err(h, SG_ESTATE, "no state uploaded");
+  (void)irk;
+  const Params &P = h->P;
+  size_t qbytes = sizeof(double) * P.fstride * (5 + P.ns);
+  double *save = nullptr, *out = nullptr;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMalloc((void **)&save, qbytes));
+  HIPCHK(h, hipMalloc((void **)&out, qbytes));
+  HIPCHK(h, hipMemcpyAsync(save, h->F.q, qbytes, hipMemcpyDeviceToDevice, h->stream));
+  int rc = eval_rhs(h);
+  if (!rc) rc = timed(h, K_FINAL, [&] { launch_final(P, h->dcfg, h->F, 2, 0.0, 0.0, out, h->stream); });
+  if (!rc) {
+    size_t fb = sizeof(double) * P.npts;
+    for (int v = 0; v < 5 + P.ns; v++)
+      HIPCHK(h, hipMemcpyAsync(rhs + v * P.npts, out + v * P.fstride, fb, hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHK(h, hipMemcpyAsync(h->F.q, save, qbytes, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipFree(save));
+  HIPCHK(h, hipFree(out));
+  if (rc) return rc;
+  return check_device_error(h);
+}
+
+int sg_derivatives(sg_handle *h, const double *f, double *out) {
+  if (!h || !f || !out) return SG_EINVAL;
+  const Params &P = h->P;
+  size_t fb = sizeof(double) * P.npts;
+  double *df = nullptr, *dout = nullptr;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMalloc((void **)&df, fb));
+  HIPCHK(h, hipMalloc((void **)&dout, 9 * fb));
+  HIPCHK(h, hipMemcpyAsync(df, f, fb, hipMemcpyHostToDevice, h->stream));
+  int rc = timed(h, K_DERIVS, [&] { launch_derivs(P, df, dout, h->stream); });
+  if (!rc) HIPCHK(h, hipMemcpyAsync(out, dout, 9 * fb, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipFree(df));
+  HIPCHK(h, hipFree(dout));
+  return rc;
+}
+
+int sg_sync(sg_handle *h) {
+  if (!h) return SG_EINVAL;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return check_device_error(h);
+}
+
+int sg_last_error(sg_handle *h, int *code, int ijk[3], char *msg, size_t n) {
+  if (!h) return SG_EINVAL;
+  if (code) *code = h->err_code;
+  if (ijk) for (int q = 0; q < 3; q++) ijk[q] = h->err_ijk[q];
+  if (msg && n) {
+    strncpy(msg, h->err_msg.c_str(), n - 1);
+    msg[n - 1] = 0;
+  }
+  return 0;
+}
+
+int sg_get_stream(sg_handle *h, void **stream) {
+  if (!h || !stream) return SG_EINVAL;
+  *stream = (void *)h->stream;
+  return 0;
+}
+
+int sg_profile(sg_handle *h, int on) {
+  if (!h) return SG_EINVAL;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (on) {
+    for (int q = 0; q < K_COUNT; q++) { h->prof_ms[q] = 0.0; h->prof_n[q] = 0; }
+  }
+  h->prof_on = on ? 1 : 0;
+  return 0;
+}
+
+int sg_profile_read(sg_handle *h, int idx, char *name, size_t n, double *total_ms, int64_t *count) {
+  if (!h || idx < 0 || idx >= K_COUNT) return SG_EINVAL;
+  if (name && n) {
+    strncpy(name, kKernelNames[idx], n - 1);
+    name[n - 1] = 0;
+  }
+  if (total_ms) *total_ms = h->prof_ms[idx];
+  if (count) *count = h->prof_n[idx];
+  return 0;
+}
+
+int sg_memory(sg_handle *h, size_t *bytes) {
+  if (!h || !bytes) return SG_EINVAL;
+  *bytes = h->bytes;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// sg_dispatch.cpp -- species-count dispatch of the kernel launchers.
+//
+// Each supported species count is compiled into its own object
+// (sg_kernels.hip with -DSG_NS=N); this file selects the instantiation.
+#include "sg_internal.h"
+
+namespace sg {
+
+#define SG_NS_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+
+template <int NS> void launch_tempin_t(const Params &, const sg_config *, const Fields &, hipStream_t);
+template <int NS> void launch_boundt_t(const Params &, const sg_config *, const Fields &, hipStream_t);
+template <int NS> void launch_bountt_t(const Params &, const sg_config *, const Fields &, hipStream_t);
+template <int NS> void launch_prim_t(const Params &, const sg_config *, const Fields &, hipStream_t);
+template <int NS> void launch_pass_t(int, const Params &, const Fields &, hipStream_t);
+template <int NS>
+void launch_final_t(const Params &, const sg_config *, const Fields &, int, double, double, double *, hipStream_t);
+
+bool species_supported(int ns) {
+#define CASE(N) if (ns == N) return true;
+  SG_NS_LIST(CASE)
+#undef CASE
+  return false;
+}
+
+void launch_tempin(const Params &P, const sg_config *c, const Fields &F, hipStream_t st) {
+  switch (P.ns) {
+#define CASE(N) case N: launch_tempin_t<N>(P, c, F, st); break;
+    SG_NS_LIST(CASE)
+#undef CASE
+    default: break;
+  }
+}
+void launch_boundt(const Params &P, const sg_config *c, const Fields &F, hipStream_t st) {
+  switch (P.ns) {
+#define CASE(N) case N: launch_boundt_t<N>(P, c, F, st); break;
+    SG_NS_LIST(CASE)
+#undef CASE
+    default: break;
+  }
+}
+void launch_bountt(const Params &P, const sg_config *c, const Fields &F, hipStream_t st) {
+  switch (P.ns) {
+#define CASE(N) case N: launch_bountt_t<N>(P, c, F, st); break;
+    SG_NS_LIST(CASE)
+#undef CASE
+    default: break;
+  }
+}
+void launch_prim(const Params &P, const sg_config *c, const Fields &F, hipStream_t st) {
+  switch (P.ns) {
+#define CASE(N) case N: launch_prim_t<N>(P, c, F, st); break;
+    SG_NS_LIST(CASE)
+#undef CASE
+    default: break;
+  }
+}
+void launch_pass(int dir, const Params &P, const Fields &F, hipStream_t st) {
+  switch (P.ns) {
+#define CASE(N) case N: launch_pass_t<N>(dir, P, F, st); break;
+    SG_NS_LIST(CASE)
+#undef CASE
+    default: break;
+  }
+}
+void launch_final(const Params &P, const sg_config *c, const Fields &F, int mode, double lhs, double rhs,
+                  double *out, hipStream_t st) {
+  switch (P.ns) {
+#define CASE(N) case N: launch_final_t<N>(P, c, F, mode, lhs, rhs, out, st); break;
+    SG_NS_LIST(CASE)
+#undef CASE
+    default: break;
+  }
+}
+
+}  // namespace sg

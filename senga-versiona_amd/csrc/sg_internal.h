@@ -1,0 +1,71 @@
+// sg_internal.h -- shared definitions of the HIP right-hand-side library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "senga_gpu.h"
+
+namespace sg {
+
+constexpr int HALO = 5;
+
+// Grid, stencil coefficients and boundary flags handed to every kernel by
+// value.  Coefficients are the values src/dfinit.f:61-477 computes.
+struct Params {
+  int nx, ny, nz, ns;
+  long sy, sz;                 // element strides of y and z
+  long npts;                   // nx*ny*nz
+  long fstride;                // elements between consecutive fields
+  int bnd[3];                  // 1: direction bounded (closures), 0: periodic
+  double ovdel[3], ovdl2[3];
+  double ac[5], c1[4], c2[4], c3[2], c4[3], c5[4];       // first derivative
+  double sc[5], s1[5], s2[5], s3[2], s4[3], s5[4];       // second derivative
+  double mc[5], my1[2], m1[4], m2[4], m3[2], m4[3], m5[4];  // mixed
+  double thrd, tthd, fthd;
+  // boundary flags per face xl,xr,yl,yr,zl,zr (src/bcinit.f)
+  int fcnv[6], fvsn[6], fvst[6], fcon[6], fadb[6], fdif[6];
+  int inflow_xl, outflow_xr;   // NSBCI3 at x-left, NSBCO1 at x-right
+  int rklo[6];                 // first x index of the RK update for D,U,V,W,E,Y
+  double drin, uin, cobcxr, pinfxr;
+  int mixavg;
+  double alamda, rlamda, prantl, tdifgb, pdifgb;
+};
+
+// Device field set: all arrays are [field][z][y][x] with field stride fstride.
+struct Fields {
+  double *q;      // stage register (5+ns): rho, rho u, rho v, rho w, rho E, rho Y
+  double *run;    // solution register (5+ns)
+  double *prim;   // primitives: see PrimSlot, then Y[ns], rhoD[ns], h[ns]
+  double *rate;   // chemical source (ns), kg/m^3/s
+  double *acc;    // RHS accumulators (5+ns) + VT (divergence of the correction velocity)
+  double *grad;   // du/dx,dv/dx,dw/dx,dmu/dx, du/dy,..,dmu/dy, du/dz,..,dmu/dz (12)
+  uint32_t *itndex;  // packed temperature interval indices (2 bits per species)
+  double *face;   // NSCBC face data, see FaceSlot, [face 0/1][slot][ny*nz]
+  int *err;       // [0]=error code, [1..3] = failing point (1-based)
+};
+
+enum PrimSlot { P_U = 0, P_V, P_W, P_E, P_T, P_P, P_LAM, P_MU, P_HMIX, P_NSCAL };
+// face slots: scalars then per-species
+enum FaceSlot { F_CP = 0, F_RR, F_DPDX, F_DPDY, F_DPDZ, F_DRDX, F_NSCAL };
+// per species: F_DYDX at F_NSCAL + s
+
+inline __host__ __device__ long prim_y(const Params &P, int s) { return (long)(P_NSCAL + s) * P.fstride; }
+inline __host__ __device__ long prim_rd(const Params &P, int s) { return (long)(P_NSCAL + P.ns + s) * P.fstride; }
+inline __host__ __device__ long prim_h(const Params &P, int s) { return (long)(P_NSCAL + 2 * P.ns + s) * P.fstride; }
+inline __host__ __device__ int nprim(int ns) { return P_NSCAL + 3 * ns; }
+inline __host__ __device__ int nface(int ns) { return F_NSCAL + ns; }
+
+// launchers (sg_kernels.hip)
+void launch_tempin(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st);
+void launch_boundt(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st);
+void launch_prim(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st);
+void launch_pass(int dir, const Params &P, const Fields &F, hipStream_t st);
+// mode: 0 = LINCOM (coefficients lhs, rhs), 1 = FINCOM, 2 = write RHS to out
+void launch_final(const Params &P, const sg_config *dcfg, const Fields &F, int mode, double lhs,
+                  double rhs, double *out, hipStream_t st);
+void launch_bountt(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st);
+void launch_derivs(const Params &P, const double *f, double *out, hipStream_t st);
+bool species_supported(int ns);
+
+}  // namespace sg

@@ -1,0 +1,954 @@
+// sg_kernels.hip -- CDNA4 kernels of one SENGA+ Runge-Kutta substage (FP64).
+//
+// Per substage (src/senga2.f:113-159):
+//   k_boundt   BOUNDT inflow values at x-left            (src/boundt.f:177-218)
+//   k_prim     TEMPER + transport + CHRATE + enthalpies   (src/temper.f, src/chrate.f,
+//                                                          src/rhscal.f:515-573,1292-1470,3854-3891)
+//   k_pass<d>  every derivative along direction d and the RHS terms that
+//              involve derivatives along d only           (src/rhscal.f, src/rhsvel.f)
+//   k_final    mixed derivatives, cross-direction viscous terms, NSCBC face
+//              corrections (src/bounds.f) and the RK register update
+//              (src/lincom.f, src/fincom.f)
+// Periodic directions use wrapped indices: no halo copies are stored.
+#include <math.h>
+
+#include "sg_internal.h"
+#include "sg_stencil.h"
+
+namespace sg {
+
+#define BLK 256
+
+__device__ __forceinline__ void coords(const Params &P, long idx, int &i, int &j, int &k) {
+  int nxy = P.nx * P.ny;
+  k = (int)(idx / nxy);
+  int r = (int)(idx - (long)k * nxy);
+  j = r / P.nx;
+  i = r - j * P.nx;
+}
+
+__device__ __forceinline__ int itint_of(uint32_t idx, int s) { return (int)((idx >> (2 * s)) & 3u); }
+
+__device__ __forceinline__ int locate(const sg_config *c, int s, double t) {
+  int it = 0;
+  while (t > c->tinthi[s][it] && it < c->ntint[s] - 1) it++;
+  return it;
+}
+
+__device__ void set_error(int *err, int code, int i, int j, int k) {
+  if (atomicCAS(err, 0, code) == 0) {
+    err[1] = i + 1;
+    err[2] = j + 1;
+    err[3] = k + 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// TEMPIN: bisection from TRIN in 50 K steps, src/tempin.f:55-512
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_tempin(Params P, const sg_config *__restrict__ c, Fields F) {
+  long idx = (long)blockIdx.x * BLK + threadIdx.x;
+  if (idx >= P.npts) return;
+  const long fs = P.fstride;
+  const double *q = F.q;
+  double d = q[idx], ru = q[fs + idx], rv = q[2 * fs + idx], rw = q[3 * fs + idx];
+  double ry[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) ry[s] = q[(5 + s) * fs + idx];
+  double ukuk = (ru * ru + rv * rv + rw * rw) / d;
+  double tuk2me = 0.5 * ukuk - q[4 * fs + idx];
+  const double toltmp = 1.0e-4, tininc = 5.0e1, tlimlo = 2.0e2, tlimhi = 3.0e3;
+  auto resid = [&](double t) {
+    double tc[6];
+    tc[0] = tuk2me;
+#pragma unroll
+    for (int m = 1; m <= 5; m++) tc[m] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      int it = locate(c, s, t);
+      tc[0] = tc[0] + ry[s] * c->amascp[s][it][c->ncenth[s][it] - 1];
+      tc[1] = tc[1] + ry[s] * c->amasct[s][it][0];
+      for (int m = 2; m <= c->ncpoly[s][it]; m++) tc[m] = tc[m] + ry[s] * c->amasct[s][it][m - 1];
+    }
+    double r = tc[5];
+    for (int m = 4; m >= 1; m--) r = tc[m] + r * t;
+    return tc[0] + r * t;
+  };
+  double tlower = tlimlo, tupper = tlimhi, t = c->trin;
+  bool fnconv = true, fail = false;
+  double res = resid(t);
+  if (fabs(res) < toltmp) {
+    fnconv = false;
+  } else if (res < 0.0) {
+    for (;;) {
+      tlower = t;
+      t = t + tininc;
+      res = resid(t);
+      if (fabs(res) < toltmp) { fnconv = false; break; }
+      if (res < 0.0) {
+        if (t < tlimhi) continue;
+        fail = true;
+        break;
+      }
+      tupper = t;
+      break;
+    }
+  } else {
+    for (;;) {
+      tupper = t;
+      t = t - tininc;
+      res = resid(t);
+      if (fabs(res) < toltmp) { fnconv = false; break; }
+      if (res > 0.0) {
+        if (t > tlimlo) continue;
+        fail = true;
+        break;
+      }
+      tlower = t;
+      break;
+    }
+  }
+  if (fail) {
+    int i, j, k;
+    coords(P, idx, i, j, k);
+    set_error(F.err, SG_ETEMPIN, i, j, k);
+    return;
+  }
+  if (fnconv) {
+    // bisection: the bracket halves each pass, bounded by 64 passes in double
+    for (int it = 0; it < 200; it++) {
+      t = 0.5 * (tlower + tupper);
+      res = resid(t);
+      if (fabs(res) < toltmp) break;
+      if (res < 0.0) tlower = t;
+      else tupper = t;
+    }
+  }
+  uint32_t nidx = 0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) nidx += (uint32_t)locate(c, s, t) << (2 * s);
+  F.itndex[idx] = nidx;
+  F.prim[P_T * fs + idx] = t;
+  double s7 = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) s7 = s7 + ry[s] * c->rgspec[s];
+  F.prim[P_P * fs + idx] = t * s7;
+}
+
+// ---------------------------------------------------------------------------
+// BOUNDT for NSBCI3 at x-left (src/boundt.f:177-218; BCDTXL, BCUTXL type 1, BCYTXL)
+template <int NS>
+__global__ void k_boundt(Params P, const sg_config *__restrict__ c, Fields F) {
+  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  if (t >= (long)P.ny * P.nz) return;
+  long p = t * P.nx;  // i = 0
+  const long fs = P.fstride;
+  double strd = P.drin, stru = P.uin;
+  F.q[p] = strd;
+  F.q[fs + p] = strd * stru;
+  F.q[2 * fs + p] = strd * 0.0;
+  F.q[3 * fs + p] = strd * 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) F.q[(5 + s) * fs + p] = strd * c->yrin[s];
+}
+
+// BOUNTT for NSBCI3 at x-left (src/bountt.f:195-262)
+template <int NS>
+__global__ void k_bountt(Params P, const sg_config *__restrict__ c, Fields F) {
+  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  if (t >= (long)P.ny * P.nz) return;
+  long p = t * P.nx;
+  const long fs = P.fstride;
+  double strd = P.drin, stru = P.uin;
+  double v[4] = {strd, strd * stru, strd * 0.0, strd * 0.0};
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    F.q[m * fs + p] = v[m];
+    F.run[m * fs + p] = v[m];
+  }
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    double y = strd * c->yrin[s];
+    F.q[(5 + s) * fs + p] = y;
+    F.run[(5 + s) * fs + p] = y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Point physics: TEMPER (src/temper.f:58-225), conductivity and viscosity
+// (src/rhscal.f:515-573, 3854-3891, src/rhsvel.f:926-980), CHRATE
+// (src/chrate.f:146-760), primitive variables, diffusivities
+// (src/rhscal.f:1292-1346) and species enthalpies (src/rhscal.f:1444-1470).
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_prim(Params P, const sg_config *__restrict__ c, Fields F) {
+  long idx = (long)blockIdx.x * BLK + threadIdx.x;
+  if (idx >= P.npts) return;
+  const long fs = P.fstride;
+  const double *q = F.q;
+  double *pr = F.prim;
+  double d = q[idx], ru = q[fs + idx], rv = q[2 * fs + idx], rw = q[3 * fs + idx];
+  double re = q[4 * fs + idx];
+  double ry[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) ry[s] = q[(5 + s) * fs + idx];
+  // --- TEMPER ---
+  double ukuk = (ru * ru + rv * rv + rw * rw) / d;
+  double tc[6], td[6];
+  tc[0] = 0.5 * ukuk - re;
+#pragma unroll
+  for (int m = 1; m <= 5; m++) { tc[m] = 0.0; td[m] = 0.0; }
+  double s7 = 0.0;
+  uint32_t idxo = F.itndex[idx];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    int it = itint_of(idxo, s);
+    tc[0] = tc[0] + ry[s] * c->amascp[s][it][c->ncenth[s][it] - 1];
+    tc[1] = tc[1] + ry[s] * c->amasct[s][it][0];
+    td[1] = tc[1];
+    for (int m = 2; m <= c->ncpoly[s][it]; m++) {
+      tc[m] = tc[m] + ry[s] * c->amasct[s][it][m - 1];
+      td[m] = td[m] + ry[s] * c->amascp[s][it][m - 1];
+    }
+    s7 = s7 + ry[s] * c->rgspec[s];
+  }
+  const double toltmp = (double)1.0e-10f;  // REAL literal, src/temper.f:33-34
+  double t = pr[P_T * fs + idx];
+  int iter = 1;
+  bool ok = true;
+  for (;;) {
+    double fp = tc[5], dp = td[5];
+#pragma unroll
+    for (int m = 4; m >= 1; m--) {
+      fp = tc[m] + fp * t;
+      dp = td[m] + dp * t;
+    }
+    fp = tc[0] + fp * t;
+    double dt = -fp / dp;
+    if (fabs(dt) > toltmp) {
+      if (iter < 100) {
+        t = t + dt;
+        iter++;
+        continue;
+      }
+      ok = false;
+    }
+    break;
+  }
+  if (!ok) {
+    int i, j, k;
+    coords(P, idx, i, j, k);
+    set_error(F.err, SG_ETEMPER, i, j, k);
+  }
+  uint32_t nidx = 0;
+  double cp = 0.0;
+  int itn[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    int it = locate(c, s, t);
+    itn[s] = it;
+    nidx += (uint32_t)it << (2 * s);
+    double f = c->amascp[s][it][c->ncpoly[s][it] - 1];
+    for (int m = c->ncpoly[s][it] - 2; m >= 0; m--) f = f * t + c->amascp[s][it][m];
+    cp = cp + ry[s] * f;
+  }
+  cp = cp / d;
+  F.itndex[idx] = nidx;
+  double pres = t * s7;
+  pr[P_T * fs + idx] = t;
+  pr[P_P * fs + idx] = pres;
+  // primitives
+  pr[P_U * fs + idx] = ru / d;
+  pr[P_V * fs + idx] = rv / d;
+  pr[P_W * fs + idx] = rw / d;
+  pr[P_E * fs + idx] = re / d;
+  // --- conductivity (src/rhscal.f:515-573) ---
+  double lam, transp, wmomix = 0.0;
+  {
+    double fornow = P.alamda * exp(P.rlamda * log(t));
+    lam = fornow * cp;
+    transp = fornow;
+  }
+  double ctr[NS];
+  if (P.mixavg) {
+    transp = log(t / P.tdifgb);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      double f = c->condco[s][c->ncocon - 1];
+      for (int m = c->ncocon - 2; m >= 0; m--) f = f * transp + c->condco[s][m];
+      ctr[s] = exp(f);
+    }
+    double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      double fornow = ry[s] * c->ovwmol[s];
+      c1 = c1 + fornow * ctr[s];
+      c2 = c2 + fornow / ctr[s];
+      c3 = c3 + fornow;
+    }
+    c3 = 1.0 / c3;
+    c1 = c1 * c3;
+    c2 = c2 * c3;
+    lam = 0.5 * (c1 + 1.0 / c2);
+    wmomix = d * c3;
+  }
+  pr[P_LAM * fs + idx] = lam;
+  // --- chemistry (src/chrate.f:146-760) on rho Y ---
+  {
+    const double ysmall = 1.0e-30, ydenom = 1.0e-15;
+    double rate[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) rate[s] = 0.0;
+    double lt = log(t), ot = 1.0 / t;
+    (void)ot;
+    for (int st = 0; st < c->nstep; st++) {
+      bool flthrd = true;
+      double s3 = 0.0;
+      int mb = c->mblist[st];
+      if (mb != 0) {
+#pragma unroll
+        for (int s = 0; s < NS; s++) s3 = s3 + ry[s] * (c->ovwmol[s] * c->effy3b[mb - 1][s]);
+      }
+      double s2 = c->rparam[st][0] + c->rparam[st][1] * lt - c->rparam[st][2] / t;
+      double s1 = exp(s2);
+      int ml = c->mllist[st];
+      if (ml != 0) {
+        const double *rl = c->rclind[ml - 1];
+        double s4 = rl[0] + rl[1] * lt - rl[2] / t;
+        s4 = exp(s4);
+        double preduc = s3 * s4 / s1;
+        double fornow = rl[3] * preduc / (1.0 + preduc);  // rl[3] pre-rounded to REAL
+        s1 = s1 * fornow;
+        s2 = log(s1);
+        flthrd = false;
+      }
+      int mt = c->mtlist[st];
+      if (mt != 0) {
+        const double *rt = c->rctroe[mt - 1];
+        double omalph = 1.0 - rt[3];
+        double s4 = rt[0] + rt[1] * lt - rt[2] / t;
+        s4 = exp(s4);
+        double preduc = s3 * s4 / s1;
+        double trats1 = t * rt[4], trats2 = -rt[5] / t, trats3 = t * rt[6];
+        double ftcent = omalph * exp(trats3) + rt[3] * exp(trats1) + exp(trats2);
+        ftcent = log10(ftcent);
+        double cfactr = rt[7] + rt[8] * ftcent;
+        double enfact = rt[9] + rt[10] * ftcent;
+        double fornow = log10(preduc) + cfactr;
+        fornow = fornow / (enfact - rt[11] * fornow);
+        fornow = 1.0 + fornow * fornow;
+        double fbroad = ftcent / fornow;
+        fbroad = exp(fbroad * c->clnten);
+        fornow = fbroad * preduc / (1.0 + preduc);
+        s1 = s1 * fornow;
+        s2 = log(s1);
+        flthrd = false;
+      }
+      int ms = c->mslist[st];
+      if (ms != 0) {
+        const double *rs = c->rcsrif[ms - 1];
+        double s4 = rs[0] + rs[1] * lt - rs[2] / t;
+        s4 = exp(s4);
+        double preduc = s3 * s4 / s1;
+        double fornow = rs[4] / t;
+        double fbroad = rs[3] * exp(fornow);
+        fornow = t * rs[5];
+        fbroad = fbroad + exp(fornow);
+        fornow = log10(preduc);
+        fornow = 1.0 / (1.0 + log10(fornow));
+        fbroad = rs[6] * exp(fornow * log(fbroad));
+        fornow = fbroad * preduc / (1.0 + preduc);
+        s1 = s1 * fornow;
+        s2 = log(s1);
+        flthrd = false;
+      }
+      for (int m = 0; m < c->nrslen[st]; m++) {
+        int s = c->nrspec[st][m];
+        double fornow = 0.0;
+#pragma unroll
+        for (int u = 0; u < NS; u++)
+          if (u == s) fornow = ry[u] * c->ovwmol[u];
+        fornow = fornow > 0.0 ? fornow : 0.0;
+        s1 = s1 * fornow;
+      }
+      for (int m = 0; m < c->nrclen[st]; m++) {
+        int s = c->nrcpec[st][m];
+        double fornow = 0.0;
+#pragma unroll
+        for (int u = 0; u < NS; u++)
+          if (u == s) fornow = ry[u] * c->ovwmol[u];
+        fornow = fornow > ysmall ? fornow : ysmall;
+        fornow = exp(c->crspec[st][m] * log(fornow));
+        fornow = fornow / (1.0 + ydenom * fornow);
+        s1 = s1 * fornow;
+      }
+      if (c->mglist[st] != 0) {
+        for (int m = 0; m < c->nsslen[st]; m++) {
+          int s = c->nsspec[st][m];
+          int it = itint_of(nidx, s);
+          const double *gb = c->amolgb[s][it];
+          int np = c->ncpoly[s][it];
+          double g = gb[np - 1];
+          for (int cc = np - 2; cc >= 0; cc--) g = gb[cc] + g * t;
+          g = gb[c->ncenth[s][it] - 1] / t - gb[c->ncenpy[s][it] - 1] * lt - g;
+          s2 = s2 + c->diffmu[st][m] * g;
+        }
+        s2 = exp(s2);
+        for (int m = 0; m < c->npslen[st]; m++) {
+          int s = c->npspec[st][m];
+          double fornow = 0.0;
+#pragma unroll
+          for (int u = 0; u < NS; u++)
+            if (u == s) fornow = ry[u] * c->ovwmol[u];
+          fornow = fornow > 0.0 ? fornow : 0.0;
+          s2 = s2 * fornow;
+        }
+        for (int m = 0; m < c->npclen[st]; m++) {
+          int s = c->npcpec[st][m];
+          double fornow = 0.0;
+#pragma unroll
+          for (int u = 0; u < NS; u++)
+            if (u == s) fornow = ry[u] * c->ovwmol[u];
+          fornow = fornow > ysmall ? fornow : ysmall;
+          fornow = exp(c->cpspec[st][m] * log(fornow));
+          fornow = fornow / (1.0 + ydenom * fornow);
+          s2 = s2 * fornow;
+        }
+        s1 = s1 - s2;
+      }
+      if (mb != 0 && flthrd) s1 = s1 * s3;
+      for (int m = 0; m < c->nsslen[st]; m++) {
+        int s = c->nsspec[st][m];
+        double add = s1 * c->diffmw[st][m];
+#pragma unroll
+        for (int u = 0; u < NS; u++)
+          if (u == s) rate[u] = rate[u] + add;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) F.rate[s * fs + idx] = rate[s];
+  }
+  // --- mass fractions, diffusivities, enthalpies ---
+  double y[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) y[s] = ry[s] / d;
+  double hmix = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    pr[prim_y(P, s) + idx] = y[s];
+    int it = itn[s];
+    double f = c->amasch[s][it][c->ncpoly[s][it] - 1];
+    for (int m = c->ncpoly[s][it] - 2; m >= 0; m--) f = f * t + c->amasch[s][it][m];
+    double h = c->amasch[s][it][c->ncenth[s][it] - 1] + f * t;
+    pr[prim_h(P, s) + idx] = h;
+    hmix = hmix + h * y[s];
+  }
+  pr[P_HMIX * fs + idx] = hmix;
+  if (!P.mixavg) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) pr[prim_rd(P, s) + idx] = transp * c->olewis[s];
+    pr[P_MU * fs + idx] = transp * P.prantl;
+  } else {
+    // binary diffusion (src/rhscal.f:1311-1346)
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      double ctd[NS];
+#pragma unroll
+      for (int js = 0; js < NS; js++) {
+        const double *dc = c->diffco[s][js];
+        double f = dc[c->ncodif - 1];
+        for (int m = c->ncodif - 2; m >= 0; m--) f = f * transp + dc[m];
+        ctd[js] = exp(f) * P.pdifgb / pres;
+      }
+      double c1 = 0.0, c2 = 0.0;
+#pragma unroll
+      for (int js = 0; js < NS; js++) {
+        double fornow = y[js] + 1.0e-12;
+        c1 = c1 + fornow;
+        c2 = c2 + fornow * c->ovwmol[js] / ctd[js];
+      }
+      double fornow = y[s] + 1.0e-12;
+      c1 = c1 - fornow;
+      c2 = c2 - fornow * c->ovwmol[s] / ctd[s];
+      c2 = c2 * wmomix;
+      pr[prim_rd(P, s) + idx] = d * c1 / c2;
+    }
+    // Wilke viscosity (src/rhscal.f:3854-3891)
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      double f = c->viscco[s][c->ncovis - 1];
+      for (int m = c->ncovis - 2; m >= 0; m--) f = f * transp + c->viscco[s][m];
+      ctr[s] = exp(f);
+    }
+    double c1 = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      double c2 = 0.0;
+#pragma unroll
+      for (int js = 0; js < NS; js++) {
+        double fornow = sqrt(ctr[s] / ctr[js]);
+        fornow = 1.0 + fornow * c->wilko2[s][js];
+        fornow = c->wilko1[s][js] * fornow * fornow;
+        c2 = c2 + y[js] * c->ovwmol[js] * fornow;
+      }
+      double fornow = ctr[s] / c2;
+      c1 = c1 + y[s] * c->ovwmol[s] * fornow;
+    }
+    pr[P_MU * fs + idx] = c1;
+  }
+  // face data for NSCBC: cp and mixture gas constant (src/rhscal.f:72-131)
+  if (P.inflow_xl || P.outflow_xr) {
+    int i, j, k;
+    coords(P, idx, i, j, k);
+    if (i == 0 || i == P.nx - 1) {
+      long nf = (long)P.ny * P.nz;
+      double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
+      long fq = (long)k * P.ny + j;
+      fc[F_CP * nf + fq] = cp;
+      fc[F_RR * nf + fq] = s7 / d;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Direction pass: all derivatives along DIR and the RHS terms separable in
+// DIR.  Fields read with the stencil: rho u_d (momenta), u, v, w, e, T, p,
+// lambda, mu, h_mix and per species Y, rho D, h.
+template <int DIR, bool BND, int NS>
+__global__ void __launch_bounds__(BLK) k_pass(Params P, Fields F) {
+  long idx = (long)blockIdx.x * BLK + threadIdx.x;
+  if (idx >= P.npts) return;
+  int i, j, k;
+  coords(P, idx, i, j, k);
+  const int m = DIR == 0 ? i : (DIR == 1 ? j : k);
+  const int n = DIR == 0 ? P.nx : (DIR == 1 ? P.ny : P.nz);
+  const int s = DIR == 0 ? 1 : (DIR == 1 ? (int)P.sy : (int)P.sz);
+  const Sten S = make_sten<BND>(m, n, s);
+  const long fs = P.fstride;
+  const double *__restrict__ Q = F.q;
+  const double *__restrict__ pr = F.prim;
+  const double *rho = Q, *mx = Q + fs, *my = Q + 2 * fs, *mz = Q + 3 * fs;
+  const double *md = DIR == 0 ? mx : (DIR == 1 ? my : mz);
+  const double *uu = pr + P_U * fs, *vv = pr + P_V * fs, *ww = pr + P_W * fs;
+  const double *ee = pr + P_E * fs, *TT = pr + P_T * fs, *pp = pr + P_P * fs;
+  const double *la = pr + P_LAM * fs, *mu = pr + P_MU * fs, *hm = pr + P_HMIX * fs;
+  const double *ud = DIR == 0 ? uu : (DIR == 1 ? vv : ww);
+  const double *ptr = pr + P_P * fs;
+  const double pref = ptr[0];   // PRUN(IPREF,JPREF,KPREF), src/rhsvel.f:741
+  auto V1 = [&](const double *a) { return d1<BND>(P, S, DIR, [&](int kk) { return a[idx + S.o[kk]]; }); };
+  auto V2 = [&](const double *a) { return d2<BND>(P, S, DIR, [&](int kk) { return a[idx + S.o[kk]]; }); };
+  auto VP = [&](const double *a, const double *b) {
+    return d1<BND>(P, S, DIR, [&](int kk) { long o = idx + S.o[kk]; return a[o] * b[o]; });
+  };
+  // boundary-plane flags of this direction (src/bcinit.f)
+  const bool atL = BND && m == 0, atR = BND && m == n - 1;
+  const int fl = 2 * DIR, fr = 2 * DIR + 1;
+  auto zf = [&](const int *flag) { return (atL && flag[fl]) || (atR && flag[fr]); };
+  const bool zcon = zf(P.fcon), zdif = zf(P.fdif), zadb = zf(P.fadb), zvsn = zf(P.fvsn),
+             zvst = zf(P.fvst);
+  const bool zdifadb = zdif || zadb;
+  // ADB zeroing of the correction-velocity divergence acts on any face plane
+  const bool zadb_any = (i == 0 && P.fadb[0]) || (i == P.nx - 1 && P.fadb[1]) ||
+                        (j == 0 && P.fadb[2]) || (j == P.ny - 1 && P.fadb[3]) ||
+                        (k == 0 && P.fadb[4]) || (k == P.nz - 1 && P.fadb[5]);
+  // centre values
+  const double r0 = rho[idx], md0 = md[idx], e0 = ee[idx], p0 = pp[idx], lam0 = la[idx],
+               mu0 = mu[idx], hm0 = hm[idx];
+  const double u0[3] = {uu[idx], vv[idx], ww[idx]};
+  // ---------------- continuity and energy convection (src/rhscal.f:141-415)
+  const double dm = V1(md);
+  double aD = -dm;
+  const double dem = VP(ee, md);
+  const double de = V1(ee);
+  double aE = -0.5 * (e0 * dm + dem + md0 * de);
+  // ---------------- heat conduction (src/rhscal.f:427-797)
+  {
+    const double dT = V1(TT);
+    const double dl = V1(la);
+    const double d2T = V2(TT);
+    aE = aE + (zcon ? 0.0 : dl) * dT;
+    aE = aE + lam0 * (zcon ? 0.0 : d2T);
+  }
+  // ---------------- momentum convection (src/rhsvel.f:348-729, 1312-2110)
+  double du[3], d2u[3];
+  du[0] = V1(uu); du[1] = V1(vv); du[2] = V1(ww);
+  double aM[3];
+  {
+    // fluxes as the reference forms them: rho u*u, rho u*v, rho u*w, rho v*v, rho v*w, rho w*w
+    double fl0, fl1, fl2;
+    if (DIR == 0) { fl0 = VP(mx, uu); fl1 = VP(mx, vv); fl2 = VP(mx, ww); }
+    else if (DIR == 1) { fl0 = VP(mx, vv); fl1 = VP(my, vv); fl2 = VP(my, ww); }
+    else { fl0 = VP(mx, ww); fl1 = VP(my, ww); fl2 = VP(mz, ww); }
+    const double flx[3] = {fl0, fl1, fl2};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      if (a == DIR) aM[a] = -0.5 * (md0 * du[a] + flx[a] + u0[a] * dm);
+      else aM[a] = -0.5 * (flx[a] + u0[a] * dm) - 0.5 * r0 * u0[DIR] * du[a];
+    }
+  }
+  // ---------------- pressure gradient and pressure work (src/rhsvel.f:741-917)
+  {
+    const double dp = d1<BND>(P, S, DIR, [&](int kk) { return ptr[idx + S.o[kk]] - pref; });
+    aM[DIR] = aM[DIR] - dp;
+    aE = aE - u0[DIR] * dp - p0 * du[DIR];
+    if (DIR == 0 && BND && (i == 0 || i == P.nx - 1)) {
+      long nf = (long)P.ny * P.nz, fq = (long)k * P.ny + j;
+      double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
+      fc[F_DPDX * nf + fq] = dp;
+      fc[F_DRDX * nf + fq] = V1(rho);
+    } else if (DIR != 0 && (P.inflow_xl || P.outflow_xr) && (i == 0 || i == P.nx - 1)) {
+      long nf = (long)P.ny * P.nz, fq = (long)k * P.ny + j;
+      double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
+      fc[(DIR == 1 ? F_DPDY : F_DPDZ) * nf + fq] = dp;
+    }
+  }
+  // ---------------- separable viscous terms (src/rhsvel.f:1462-2305)
+  d2u[0] = V2(uu); d2u[1] = V2(vv); d2u[2] = V2(ww);
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    double tv;
+    if (a == DIR) tv = zvsn ? 0.0 : mu0 * P.fthd * d2u[a];
+    else tv = zvst ? 0.0 : mu0 * d2u[a];
+    aM[a] = aM[a] + tv;
+    aE = aE + tv * u0[a];
+  }
+  const double dmu = V1(mu);
+  // ---------------- species (src/rhscal.f:1030-3944)
+  double vt = 0.0, ucor = 0.0;
+  double aY[NS], dYz[NS], y0[NS];
+#pragma unroll
+  for (int sp = 0; sp < NS; sp++) {
+    const double *Ys = pr + prim_y(P, sp), *Rs = pr + prim_rd(P, sp), *Hs = pr + prim_h(P, sp);
+    const double yc = Ys[idx], rdc = Rs[idx], hc = Hs[idx];
+    y0[sp] = yc;
+    const double dY = V1(Ys);
+    const double dYm = VP(Ys, md);
+    const double drD = V1(Rs);
+    const double dh = V1(Hs);
+    const double d2Y = V2(Ys);
+    double ys = -0.5 * yc * dm - 0.5 * dYm - 0.5 * md0 * dY;
+    const double g1 = (zdif ? 0.0 : drD) * dY;
+    const double g2 = rdc * (zdif ? 0.0 : d2Y);
+    ys = ys + g1 + g2;
+    vt = vt + g1 + g2;
+    aE = aE + hc * ((zdifadb ? 0.0 : drD) * dY);
+    aE = aE + ((zdifadb ? 0.0 : dh) * dY) * rdc;
+    aE = aE + (rdc * (zdifadb ? 0.0 : d2Y)) * hc;
+    ucor = ucor + rdc * dY;
+    aY[sp] = ys;
+    dYz[sp] = zdif ? 0.0 : dY;
+    if (DIR == 0 && BND && (i == 0 || i == P.nx - 1)) {
+      long nf = (long)P.ny * P.nz, fq = (long)k * P.ny + j;
+      double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
+      fc[(F_NSCAL + sp) * nf + fq] = dY;
+    }
+  }
+  {
+    const double dhm = V1(hm);
+    aE = aE - hm0 * (zadb_any ? 0.0 : vt) - (zdifadb ? 0.0 : dhm) * ucor;
+  }
+#pragma unroll
+  for (int sp = 0; sp < NS; sp++) aY[sp] = aY[sp] - y0[sp] * vt - dYz[sp] * ucor;
+  // ---------------- store
+  double *A = F.acc;
+  if (DIR == 0) {
+    A[idx] = aD;
+    A[fs + idx] = aM[0];
+    A[2 * fs + idx] = aM[1];
+    A[3 * fs + idx] = aM[2];
+    A[4 * fs + idx] = aE;
+#pragma unroll
+    for (int sp = 0; sp < NS; sp++) A[(5 + sp) * fs + idx] = F.rate[sp * fs + idx] + aY[sp];
+  } else {
+    A[idx] += aD;
+    A[fs + idx] += aM[0];
+    A[2 * fs + idx] += aM[1];
+    A[3 * fs + idx] += aM[2];
+    A[4 * fs + idx] += aE;
+#pragma unroll
+    for (int sp = 0; sp < NS; sp++) A[(5 + sp) * fs + idx] += aY[sp];
+  }
+  double *G = F.grad + (long)DIR * 4 * fs;
+  G[idx] = du[0];
+  G[fs + idx] = du[1];
+  G[2 * fs + idx] = du[2];
+  G[3 * fs + idx] = dmu;
+}
+
+// ---------------------------------------------------------------------------
+// Final kernel: cross-direction viscous terms (src/rhsvel.f:992-2185), NSCBC
+// face corrections (src/bounds.f:115-206, 610-682, 1196-1458) and the RK
+// register update (src/lincom.f:53-169, src/fincom.f:54-168).
+template <bool BND, int NS>
+__global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__restrict__ c, Fields F, int mode,
+                                               double lhs, double rhsc, double *__restrict__ out) {
+  long idx = (long)blockIdx.x * BLK + threadIdx.x;
+  if (idx >= P.npts) return;
+  int i, j, k;
+  coords(P, idx, i, j, k);
+  const long fs = P.fstride;
+  const double *pr = F.prim;
+  const double *uu = pr + P_U * fs, *vv = pr + P_V * fs, *ww = pr + P_W * fs;
+  const double u = uu[idx], v = vv[idx], w = ww[idx], mu = pr[P_MU * fs + idx];
+  const double *G = F.grad;
+  const double dux = G[idx], dvx = G[fs + idx], dwx = G[2 * fs + idx], dmux = G[3 * fs + idx];
+  const double duy = G[4 * fs + idx], dvy = G[5 * fs + idx], dwy = G[6 * fs + idx],
+               dmuy = G[7 * fs + idx];
+  const double duz = G[8 * fs + idx], dvz = G[9 * fs + idx], dwz = G[10 * fs + idx],
+               dmuz = G[11 * fs + idx];
+  const double *A = F.acc;
+  double rD = A[idx], rU = A[fs + idx], rV = A[2 * fs + idx], rW = A[3 * fs + idx],
+         rE = A[4 * fs + idx];
+  double rY[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) rY[s] = A[(5 + s) * fs + idx];
+  auto fz = [&](const int *fl, int dir) {
+    int m = dir == 0 ? i : (dir == 1 ? j : k);
+    int n = dir == 0 ? P.nx : (dir == 1 ? P.ny : P.nz);
+    return (m == 0 && fl[2 * dir]) || (m == n - 1 && fl[2 * dir + 1]);
+  };
+  const double fthd = P.fthd, tthd = P.tthd;
+  // normal stresses with viscosity gradients
+  {
+    double sxx = fthd * dux - tthd * (dvy + dwz);
+    rE = rE + mu * sxx * dux;
+    double t = fz(P.fvsn, 0) ? 0.0 : sxx * dmux;
+    rU = rU + t;
+    rE = rE + u * t;
+    double syy = fthd * dvy - tthd * (dux + dwz);
+    rE = rE + mu * syy * dvy;
+    t = fz(P.fvsn, 1) ? 0.0 : syy * dmuy;
+    rV = rV + t;
+    rE = rE + v * t;
+    double szz = fthd * dwz - tthd * (dux + dvy);
+    rE = rE + mu * szz * dwz;
+    t = fz(P.fvsn, 2) ? 0.0 : szz * dmuz;
+    rW = rW + t;
+    rE = rE + w * t;
+  }
+  // cross derivatives
+  const Sten Sx = make_sten<BND>(i, P.nx, 1);
+  const Sten Sy = make_sten<false>(j, P.ny, (int)P.sy);
+  const Sten Sz = make_sten<false>(k, P.nz, (int)P.sz);
+  auto mxy = [&](const double *a) {
+    return dmix<BND>(P, Sx, 0, 1, [&](int k1, int k2) { return a[idx + Sx.o[k1] + Sy.o[k2]]; });
+  };
+  auto mxz = [&](const double *a) {
+    return dmix<BND>(P, Sx, 0, 2, [&](int k1, int k2) { return a[idx + Sx.o[k1] + Sz.o[k2]]; });
+  };
+  auto myz = [&](const double *a) {
+    return dmix<false>(P, Sy, 1, 2, [&](int k1, int k2) { return a[idx + Sy.o[k1] + Sz.o[k2]]; });
+  };
+  {  // xy pair
+    double S7 = duy + dvx;
+    double ma = mxy(uu), mb = mxy(vv);
+    double t = fz(P.fvsn, 0) ? 0.0 : tthd * mu * mb;
+    rU = rU - t; rE = rE - u * t;
+    t = fz(P.fvsn, 1) ? 0.0 : tthd * mu * ma;
+    rV = rV - t; rE = rE - v * t;
+    t = fz(P.fvst, 1) ? 0.0 : mu * mb + S7 * dmuy;
+    rU = rU + t; rE = rE + u * t;
+    t = fz(P.fvst, 0) ? 0.0 : mu * ma + S7 * dmux;
+    rV = rV + t; rE = rE + v * t;
+    rE = rE + mu * S7 * S7;
+  }
+  {  // xz pair
+    double S7 = duz + dwx;
+    double ma = mxz(uu), mb = mxz(ww);
+    double t = fz(P.fvsn, 0) ? 0.0 : tthd * mu * mb;
+    rU = rU - t; rE = rE - u * t;
+    t = fz(P.fvsn, 2) ? 0.0 : tthd * mu * ma;
+    rW = rW - t; rE = rE - w * t;
+    t = fz(P.fvst, 2) ? 0.0 : mu * mb + S7 * dmuz;
+    rU = rU + t; rE = rE + u * t;
+    t = fz(P.fvst, 0) ? 0.0 : mu * ma + S7 * dmux;
+    rW = rW + t; rE = rE + w * t;
+    rE = rE + mu * S7 * S7;
+  }
+  {  // yz pair
+    double S7 = dvz + dwy;
+    double ma = myz(vv), mb = myz(ww);
+    double t = fz(P.fvsn, 1) ? 0.0 : tthd * mu * mb;
+    rV = rV - t; rE = rE - v * t;
+    t = fz(P.fvsn, 2) ? 0.0 : tthd * mu * ma;
+    rW = rW - t; rE = rE - w * t;
+    t = fz(P.fvst, 2) ? 0.0 : mu * mb + S7 * dmuz;
+    rV = rV + t; rE = rE + v * t;
+    t = fz(P.fvst, 1) ? 0.0 : mu * ma + S7 * dmuy;
+    rW = rW + t; rE = rE + w * t;
+    rE = rE + mu * S7 * S7;
+  }
+  // ---------------- NSCBC faces
+  if (BND && (i == 0 || i == P.nx - 1)) {
+    const long nf = (long)P.ny * P.nz, fq = (long)k * P.ny + j;
+    const double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
+    const double strg = fc[F_CP * nf + fq], strr = fc[F_RR * nf + fq];
+    const double strt = pr[P_T * fs + idx];
+    const double strd = F.q[idx];
+    double gam1 = strg - strr;
+    const double stre = pr[P_E * fs + idx] - gam1 * strt;
+    gam1 = strr / gam1;
+    const double ovgm = 1.0 / gam1;
+    double fornow = strg * gam1 * strt;
+    const double acou = sqrt(fornow), ova2 = 1.0 / fornow;
+    double strh[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) strh[s] = pr[prim_h(P, s) + idx] - strg * strt * c->rgspec[s] / strr;
+    if (i == 0 && P.inflow_xl) {
+      // NSBCI3, src/bounds.f:610-682 (imposed values have zero time derivative)
+      double bcl1 = dux, bcl2 = fc[F_DRDX * nf + fq], bcl3 = dvx, bcl4 = dwx,
+             bcl5 = fc[F_DPDX * nf + fq];
+      fornow = strd * acou * bcl1;
+      bcl1 = 0.5 * (u - acou) * (bcl5 - fornow);
+      bcl2 = u * (bcl2 - bcl5 * ova2);
+      bcl3 = u * bcl3;
+      bcl4 = u * bcl4;
+      bcl5 = 0.5 * (u + acou) * (bcl5 + fornow);
+      fornow = bcl1 - strd * acou * 0.0;
+      bcl2 = -0.0 - ova2 * (bcl1 + fornow) - bcl2;
+      bcl3 = -0.0 - bcl3;
+      bcl4 = -0.0 - bcl4;
+      bcl5 = fornow - bcl5;
+      rE = rE - bcl2 * stre - bcl3 * strd * v - bcl4 * strd * w - bcl5 * (ova2 * stre + u / acou + ovgm);
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        double bcly = F.rate[s * fs + idx] / strd - 0.0 - u * fc[(F_NSCAL + s) * nf + fq];
+        rE = rE - bcly * strd * strh[s];
+      }
+    }
+    if (i == P.nx - 1 && P.outflow_xr) {
+      // NSBCO1, src/bounds.f:1296-1458 (pointwise in/outflow switch and local-Mach beta off)
+      const double bet = 0.050;
+      const double strp = pr[P_P * fs + idx];
+      const double t51 = -v * fc[F_DPDY * nf + fq] - w * fc[F_DPDZ * nf + fq];
+      const double t52 = -strp * (dvy + dwz);
+      const double t2b = -v * duy - w * duz;
+      const double tt1 = t51 + t52 * (gam1 + 1.0) - strd * acou * t2b;
+      double sorp = 0.0;
+#pragma unroll
+      for (int s = 0; s < NS; s++) sorp = sorp + strh[s] * F.rate[s * fs + idx];
+      sorp = -sorp * gam1;
+      double bcl1 = 0.5 * (u - acou) * (fc[F_DPDX * nf + fq] - strd * acou * dux);
+      bcl1 = 0.5 * sorp + P.cobcxr * acou * (strp - P.pinfxr) + 0.5 * (1.0 - bet) * tt1 - bcl1;
+      rD = rD - bcl1 * ova2;
+      rU = rU - bcl1 * ova2 * (u - acou);
+      rV = rV - bcl1 * ova2 * v;
+      rW = rW - bcl1 * ova2 * w;
+      rE = rE - bcl1 * (ova2 * stre - u / acou + ovgm);
+#pragma unroll
+      for (int s = 0; s < NS; s++) rY[s] = rY[s] - bcl1 * ova2 * pr[prim_y(P, s) + idx];
+    }
+  }
+  // ---------------- register update
+  double R[5 + NS];
+  R[0] = rD; R[1] = rU; R[2] = rV; R[3] = rW; R[4] = rE;
+#pragma unroll
+  for (int s = 0; s < NS; s++) R[5 + s] = rY[s];
+  if (mode == 2) {
+#pragma unroll
+    for (int vv2 = 0; vv2 < 5 + NS; vv2++) out[vv2 * fs + idx] = R[vv2];
+    return;
+  }
+#pragma unroll
+  for (int vv2 = 0; vv2 < 5 + NS; vv2++) {
+    const int slot = vv2 < 5 ? vv2 : 5;
+    const long o = vv2 * fs + idx;
+    if (i >= P.rklo[slot]) {
+      double old = F.run[o];
+      if (mode == 0) {
+        F.run[o] = old + lhs * R[vv2];
+        F.q[o] = old + rhsc * R[vv2];
+      } else {
+        double nw = old + lhs * R[vv2];
+        F.run[o] = nw;
+        F.q[o] = nw;
+      }
+    } else {
+      F.q[o] = R[vv2];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The nine operators on one field (parity of the stencil device functions).
+template <bool BX, bool BY, bool BZ>
+__global__ void __launch_bounds__(BLK) k_derivs(Params P, const double *__restrict__ f, double *__restrict__ out) {
+  long idx = (long)blockIdx.x * BLK + threadIdx.x;
+  if (idx >= P.npts) return;
+  int i, j, k;
+  coords(P, idx, i, j, k);
+  const Sten Sx = make_sten<BX>(i, P.nx, 1);
+  const Sten Sy = make_sten<BY>(j, P.ny, (int)P.sy);
+  const Sten Sz = make_sten<BZ>(k, P.nz, (int)P.sz);
+  const long n = P.npts;
+  out[idx] = d1<BX>(P, Sx, 0, [&](int q) { return f[idx + Sx.o[q]]; });
+  out[n + idx] = d1<BY>(P, Sy, 1, [&](int q) { return f[idx + Sy.o[q]]; });
+  out[2 * n + idx] = d1<BZ>(P, Sz, 2, [&](int q) { return f[idx + Sz.o[q]]; });
+  out[3 * n + idx] = d2<BX>(P, Sx, 0, [&](int q) { return f[idx + Sx.o[q]]; });
+  out[4 * n + idx] = d2<BY>(P, Sy, 1, [&](int q) { return f[idx + Sy.o[q]]; });
+  out[5 * n + idx] = d2<BZ>(P, Sz, 2, [&](int q) { return f[idx + Sz.o[q]]; });
+  out[6 * n + idx] = dmix<BX>(P, Sx, 0, 1, [&](int a, int b) { return f[idx + Sx.o[a] + Sy.o[b]]; });
+  out[7 * n + idx] = dmix<BX>(P, Sx, 0, 2, [&](int a, int b) { return f[idx + Sx.o[a] + Sz.o[b]]; });
+  out[8 * n + idx] = dmix<BY>(P, Sy, 1, 2, [&](int a, int b) { return f[idx + Sy.o[a] + Sz.o[b]]; });
+}
+
+// ---------------------------------------------------------------------------
+// Launchers, instantiated for the species count SG_NS of this translation
+// unit (the build compiles one object per supported count, in parallel).
+
+static inline unsigned grid_for(long n) { return (unsigned)((n + BLK - 1) / BLK); }
+
+template <int NS>
+void launch_tempin_t(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st) {
+  hipLaunchKernelGGL(k_tempin<NS>, dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F);
+}
+template <int NS>
+void launch_boundt_t(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st) {
+  hipLaunchKernelGGL(k_boundt<NS>, dim3(grid_for((long)P.ny * P.nz)), dim3(BLK), 0, st, P, dcfg, F);
+}
+template <int NS>
+void launch_bountt_t(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st) {
+  hipLaunchKernelGGL(k_bountt<NS>, dim3(grid_for((long)P.ny * P.nz)), dim3(BLK), 0, st, P, dcfg, F);
+}
+template <int NS>
+void launch_prim_t(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st) {
+  hipLaunchKernelGGL(k_prim<NS>, dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F);
+}
+template <int NS>
+void launch_pass_t(int dir, const Params &P, const Fields &F, hipStream_t st) {
+  dim3 g(grid_for(P.npts)), b(BLK);
+  if (dir == 0) {
+    if (P.bnd[0]) hipLaunchKernelGGL((k_pass<0, true, NS>), g, b, 0, st, P, F);
+    else hipLaunchKernelGGL((k_pass<0, false, NS>), g, b, 0, st, P, F);
+  } else if (dir == 1) {
+    hipLaunchKernelGGL((k_pass<1, false, NS>), g, b, 0, st, P, F);
+  } else {
+    hipLaunchKernelGGL((k_pass<2, false, NS>), g, b, 0, st, P, F);
+  }
+}
+template <int NS>
+void launch_final_t(const Params &P, const sg_config *dcfg, const Fields &F, int mode, double lhs,
+                    double rhs, double *out, hipStream_t st) {
+  dim3 g(grid_for(P.npts)), b(BLK);
+  if (P.bnd[0]) hipLaunchKernelGGL((k_final<true, NS>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, out);
+  else hipLaunchKernelGGL((k_final<false, NS>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, out);
+}
+
+#ifdef SG_NS
+template void launch_tempin_t<SG_NS>(const Params &, const sg_config *, const Fields &, hipStream_t);
+template void launch_boundt_t<SG_NS>(const Params &, const sg_config *, const Fields &, hipStream_t);
+template void launch_bountt_t<SG_NS>(const Params &, const sg_config *, const Fields &, hipStream_t);
+template void launch_prim_t<SG_NS>(const Params &, const sg_config *, const Fields &, hipStream_t);
+template void launch_pass_t<SG_NS>(int, const Params &, const Fields &, hipStream_t);
+template void launch_final_t<SG_NS>(const Params &, const sg_config *, const Fields &, int, double, double,
+                                    double *, hipStream_t);
+#endif
+
+#ifdef SG_DERIVS
+void launch_derivs(const Params &P, const double *f, double *out, hipStream_t st) {
+  dim3 g(grid_for(P.npts)), b(BLK);
+  if (P.bnd[0]) hipLaunchKernelGGL((k_derivs<true, false, false>), g, b, 0, st, P, f, out);
+  else hipLaunchKernelGGL((k_derivs<false, false, false>), g, b, 0, st, P, f, out);
+}
+#endif
+
+}  // namespace sg

@@ -163,83 +163,141 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-class Oracle:
-    """The C restatement (oracle/lib/liboracle.so) -- test infrastructure."""
+ABI_FUNCS = ["sg_abi_version", "sg_config_size", "sg_create", "sg_destroy", "sg_upload_state",
+             "sg_download_state", "sg_set_timestep", "sg_substage", "sg_end_step", "sg_step",
+             "sg_rhs", "sg_derivatives", "sg_sync", "sg_last_error", "sg_get_stream",
+             "sg_profile", "sg_profile_read", "sg_memory"]
 
-    def __init__(self, case, libpath=None):
-        root = os.path.dirname(PKG)
-        libpath = libpath or os.path.join(root, "oracle", "lib", "liboracle.so")
-        lib = C.CDLL(libpath)
-        lib.orc_create.restype = C.c_void_p
-        lib.orc_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_int)]
-        for n in ("orc_set_state",):
-            getattr(lib, n).argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_double]
-        lib.orc_set_timestep.argtypes = [C.c_void_p, C.c_double]
-        lib.orc_rhs.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
-        lib.orc_substage.argtypes = [C.c_void_p, C.c_int]
-        lib.orc_end_step.argtypes = [C.c_void_p]
-        lib.orc_step.argtypes = [C.c_void_p, C.c_int]
-        lib.orc_get.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
-        lib.orc_get_stage.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
-        lib.orc_get_itndex.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
-        lib.orc_derivatives.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        lib.orc_destroy.argtypes = [C.c_void_p]
-        self.lib = lib
+
+def load_library(path=None):
+    """Load libsenga_gpu.so; raises OSError if it is not built (no fallback)."""
+    path = path or LIBPATH
+    if not os.path.exists(path):
+        raise OSError(f"{path} is not built; run senga-versiona_amd/build.py")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    dp = P(C.c_double)
+    vp = C.c_void_p
+    lib.sg_abi_version.restype = C.c_int
+    lib.sg_config_size.restype = C.c_size_t
+    lib.sg_create.argtypes = [P(Config), P(vp)]
+    lib.sg_destroy.argtypes = [vp]
+    lib.sg_upload_state.argtypes = [vp, dp, C.c_double]
+    lib.sg_download_state.argtypes = [vp, dp, dp, dp, dp]
+    lib.sg_set_timestep.argtypes = [vp, C.c_double]
+    lib.sg_substage.argtypes = [vp, C.c_int]
+    lib.sg_end_step.argtypes = [vp]
+    lib.sg_step.argtypes = [vp, C.c_int]
+    lib.sg_rhs.argtypes = [vp, C.c_int, dp]
+    lib.sg_derivatives.argtypes = [vp, dp, dp]
+    lib.sg_sync.argtypes = [vp]
+    lib.sg_last_error.argtypes = [vp, P(C.c_int), P(C.c_int), C.c_char_p, C.c_size_t]
+    lib.sg_get_stream.argtypes = [vp, P(vp)]
+    lib.sg_profile.argtypes = [vp, C.c_int]
+    lib.sg_profile_read.argtypes = [vp, C.c_int, C.c_char_p, C.c_size_t, dp, P(C.c_int64)]
+    lib.sg_memory.argtypes = [vp, P(C.c_size_t)]
+    for f in ABI_FUNCS:
+        if f not in ("sg_config_size",):
+            getattr(lib, f).restype = C.c_int
+    return lib
+
+
+class SgError(RuntimeError):
+    def __init__(self, code, msg, ijk=(0, 0, 0)):
+        super().__init__(f"{msg} (code {code}, point {tuple(ijk)})")
+        self.code, self.ijk = code, tuple(ijk)
+
+
+class Solver:
+    """One case on the current GPU through the C ABI."""
+
+    def __init__(self, case, lib=None):
+        self.lib = lib or load_library()
+        if self.lib.sg_config_size() != C.sizeof(Config):
+            raise RuntimeError("sg_config layout mismatch between library and binding")
         self.case = case
         self.cfg = config_from_case(case)
-        err = C.c_int(0)
-        self.h = lib.orc_create(C.byref(self.cfg), C.byref(err))
-        if not self.h:
-            raise RuntimeError(f"orc_create failed: {err.value}")
+        h = C.c_void_p()
+        rc = self.lib.sg_create(C.byref(self.cfg), C.byref(h))
+        if rc != 0:
+            raise SgError(rc, "sg_create failed")
+        self.h = h
         self.nx, self.ny, self.nz = case.shape
         self.ns = case.nspec
 
-    def __del__(self):
+    def close(self):
         if getattr(self, "h", None):
-            self.lib.orc_destroy(self.h)
+            self.lib.sg_destroy(self.h)
             self.h = None
 
-    def _chk(self, rc, what):
-        if rc != 0:
-            raise RuntimeError(f"{what} failed: {rc}")
+    def __del__(self):
+        self.close()
 
-    def set_state(self, run, etime=0.0):
+    def _chk(self, rc):
+        if rc != 0:
+            code = C.c_int(0)
+            ijk = (C.c_int * 3)()
+            msg = C.create_string_buffer(256)
+            self.lib.sg_last_error(self.h, C.byref(code), ijk, msg, 256)
+            raise SgError(rc, msg.value.decode(), list(ijk))
+
+    def upload(self, run, etime=0.0):
         run = np.ascontiguousarray(run, dtype=np.float64)
-        self._chk(self.lib.orc_set_state(self.h, _dp(run), etime), "orc_set_state")
+        self._chk(self.lib.sg_upload_state(self.h, _dp(run), etime))
 
     def set_timestep(self, dt):
-        self.lib.orc_set_timestep(self.h, dt)
+        self._chk(self.lib.sg_set_timestep(self.h, dt))
+
+    def substage(self, irk):
+        self._chk(self.lib.sg_substage(self.h, irk))
+
+    def end_step(self):
+        self._chk(self.lib.sg_end_step(self.h))
+
+    def step(self, n=1):
+        self._chk(self.lib.sg_step(self.h, n))
+
+    def sync(self):
+        self._chk(self.lib.sg_sync(self.h))
 
     def rhs(self, irk=1):
         out = np.zeros((5 + self.ns, self.nz, self.ny, self.nx))
-        self._chk(self.lib.orc_rhs(self.h, irk, _dp(out)), "orc_rhs")
+        self._chk(self.lib.sg_rhs(self.h, irk, _dp(out)))
         return out
 
-    def substage(self, irk):
-        self._chk(self.lib.orc_substage(self.h, irk), "orc_substage")
-
-    def step(self, n=1):
-        self._chk(self.lib.orc_step(self.h, n), "orc_step")
-
-    def get(self):
+    def download(self):
         run = np.zeros((5 + self.ns, self.nz, self.ny, self.nx))
-        T = np.zeros((self.nz, self.ny, self.nx)); P = np.zeros_like(T)
+        T = np.zeros((self.nz, self.ny, self.nx)); Pr = np.zeros_like(T)
         rate = np.zeros((self.ns, self.nz, self.ny, self.nx))
-        self.lib.orc_get(self.h, _dp(run), _dp(T), _dp(P), _dp(rate))
-        return run, T, P, rate
-
-    def stage(self):
-        st = np.zeros((5 + self.ns, self.nz, self.ny, self.nx))
-        self.lib.orc_get_stage(self.h, _dp(st))
-        return st
-
-    def itndex(self):
-        a = np.zeros((self.nz, self.ny, self.nx), dtype=np.int32)
-        self.lib.orc_get_itndex(self.h, a.ctypes.data_as(C.POINTER(C.c_int)))
-        return a
+        self._chk(self.lib.sg_download_state(self.h, _dp(run), _dp(T), _dp(Pr), _dp(rate)))
+        return run, T, Pr, rate
 
     def derivatives(self, f):
         f = np.ascontiguousarray(f, dtype=np.float64)
         out = np.zeros((9,) + f.shape)
-        self.lib.orc_derivatives(self.h, _dp(f), _dp(out))
+        self._chk(self.lib.sg_derivatives(self.h, _dp(f), _dp(out)))
         return out
+
+    def stream(self):
+        s = C.c_void_p()
+        self._chk(self.lib.sg_get_stream(self.h, C.byref(s)))
+        return s.value
+
+    def profile(self, on):
+        self._chk(self.lib.sg_profile(self.h, 1 if on else 0))
+
+    def profile_read(self):
+        out = {}
+        name = C.create_string_buffer(64)
+        ms = C.c_double(0)
+        cnt = C.c_int64(0)
+        i = 0
+        while self.lib.sg_profile_read(self.h, i, name, 64, C.byref(ms), C.byref(cnt)) == 0:
+            out[name.value.decode()] = (ms.value, cnt.value)
+            i += 1
+        return out
+
+    def memory(self):
+        b = C.c_size_t(0)
+        self._chk(self.lib.sg_memory(self.h, C.byref(b)))
+        return b.value

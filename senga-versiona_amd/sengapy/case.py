@@ -67,12 +67,14 @@ class Case:
         write_dump(os.path.join(rundir, "output", "dmpi0000000.dat"), self.dump)
 
 
-def sound_speed_max(tables, state):
-    """max(|u|+a) over the field, a^2 = gamma R T with gamma from the mixture cp."""
+def sound_speed_max(tables, state, T=None):
+    """max(|u|+a) over the field, a^2 = gamma R T with gamma from the mixture cp.
+    T is recovered from the state by bisection when not given."""
     rho = state[0]
     u = state[1] / rho; v = state[2] / rho; w = state[3] / rho
     Y = [state[5 + i] / rho for i in range(tables.nspec)]
-    T = _case_temperature(tables, state)
+    if T is None:
+        T = _case_temperature(tables, state)
     rmix = sum(tables.rgspec[i] * Y[i] for i in range(tables.nspec))
     cp = np.zeros_like(T)
     for i in range(tables.nspec):
@@ -163,7 +165,7 @@ def periodic_1step(n=(32, 32, 32), length=2.0e-3 * math.pi, seed=1, uprime=2.0,
     u, v, w = _ic.solenoidal_field(shape, (d[2], d[1], d[0]), uprime, 2.0 * 2.0 * delta, seed,
                                    spectrum="batchelor-townsend")
     st = _ic.conservative_state(tab, T, np.full(shape, 1.0e5), Y, u, v, w)
-    dt = _f32(cfl * min(d) / sound_speed_max(tab, st))
+    dt = _f32(cfl * min(d) / sound_speed_max(tab, st, T))
     ctl = Control(L[0], L[1], L[2], nx, ny, nz, dt, 1, nsteps, 0, prin=1.0e5, trin=Tu,
                   yrin=[1.0, 0.0])
     ctl.bc = [(1, [0, 0, 0, 0], [0.0] * 4)] * 6
@@ -228,7 +230,7 @@ def h2_nscbc(n=(128, 128, 128), length=(5.0e-3, 5.0e-3, 5.0e-3), seed=7, phi=0.7
         u = u * taper; v = v * taper; w = w * taper
     u = u + uin
     st = _ic.conservative_state(tab, T, np.full(shape, 1.0e5), Y, u, v, w)
-    dt = _f32(cfl * min(d) / sound_speed_max(tab, st))
+    dt = _f32(cfl * min(d) / sound_speed_max(tab, st, T))
     ctl = Control(length[0], length[1], length[2], nx, ny, nz, dt, 1, nsteps, 0, prin=1.0e5,
                   trin=Tu, urin=uin, yrin=[float(y) for y in Yu])
     per = (1, [0, 0, 0, 0], [0.0] * 4)

@@ -10,7 +10,7 @@ from dataclasses import dataclass, field
 
 
 def _d(x: float) -> str:
-    s = f"{x:.15E}"
+    s = f"{x:.16E}"
     m, e = s.split("E")
     return f"{m}D{int(e):+03d}"
 

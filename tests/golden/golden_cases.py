@@ -1,0 +1,37 @@
+"""Seeded cases behind the golden fixtures (shared by make_golden.py and tests)."""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "senga-versiona_amd"))
+
+from sengapy import case as C  # noqa: E402
+
+
+def build(name):
+    if name == "periodic_1step_16x12x11":
+        return C.periodic_1step(n=(16, 12, 11))
+    if name == "h2_nscbc_mix_12x11x11":
+        return C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=True)
+    if name == "h2_nscbc_cle_12x11x11":
+        return C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=False)
+    if name == "h2_per_mix_11x12x11":
+        return C.h2_nscbc(n=(11, 12, 11), length=(1.4e-3, 1.5e-3, 1.4e-3), mixavg=True, periodic_x=True)
+    raise KeyError(name)
+
+
+NAMES = ["periodic_1step_16x12x11", "h2_nscbc_mix_12x11x11", "h2_nscbc_cle_12x11x11", "h2_per_mix_11x12x11"]
+OPS = {"ops_periodic_16x12x11": "periodic_1step_16x12x11", "ops_boundedx_12x11x11": "h2_nscbc_cle_12x11x11"}
+
+
+def load(name):
+    """Case with the fixture's exact initial state and time step, plus the fixture."""
+    fx = dict(np.load(os.path.join(HERE, name + ".npz")))
+    c = build(name)
+    c.dump.state = fx["state0"]
+    c.dump.tstep = float(fx["tstep"])
+    c.control.tstep = c.dump.tstep
+    return c, fx

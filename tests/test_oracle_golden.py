@@ -1,0 +1,43 @@
+"""The C oracle against the reference's own outputs (golden fixtures).
+
+The fixtures were written by the compiled reference (tests/golden/make_golden.py);
+the oracle restates the reference in the same evaluation order, so agreement is
+expected to the last bit.
+"""
+import numpy as np
+import pytest
+
+import golden_cases
+from pyoracle import Oracle
+
+
+@pytest.mark.parametrize("name", golden_cases.NAMES)
+def test_tempin_rhs_and_steps_bitexact(name, oracle_built):
+    c, fx = golden_cases.load(name)
+    o = Oracle(c)
+    o.set_timestep(c.dump.tstep)
+    o.set_state(c.dump.state, 0.0)
+    _, T, P, _ = o.get()
+    assert np.array_equal(T, fx["T0"])
+    assert np.array_equal(o.itndex(), fx["itndex0"])
+    rhs = o.rhs(1)
+    assert np.array_equal(rhs, fx["rhs"])
+    _, T1, P1, rate1 = o.get()
+    assert np.array_equal(T1, fx["T1"])
+    assert np.array_equal(P1, fx["P1"])
+    assert np.array_equal(rate1, fx["rate1"])
+    o.set_state(c.dump.state, 0.0)
+    o.step(int(fx["nsteps"]))
+    run, T2, P2, rate2 = o.get()
+    assert np.array_equal(run, fx["run2"])
+    assert np.array_equal(T2, fx["T2"])
+    assert np.array_equal(rate2, fx["rate2"])
+
+
+@pytest.mark.parametrize("name", sorted(golden_cases.OPS))
+def test_operators_bitexact(name, oracle_built):
+    fx = dict(np.load(golden_cases.os.path.join(golden_cases.HERE, name + ".npz")))
+    c = golden_cases.build(golden_cases.OPS[name])
+    o = Oracle(c)
+    out = o.derivatives(fx["field"])
+    assert np.array_equal(out, fx["ops"])
