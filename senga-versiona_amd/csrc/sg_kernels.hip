@@ -19,6 +19,13 @@ namespace sg {
 
 #define BLK 256
 
+// Forces x to be materialised here.  Without it the compiler sinks stencil
+// sums toward their last use and keeps every neighbour value live across the
+// following stages (hundreds of registers in the direction passes).  The empty
+// volatile asm is ordered against the LDS barriers, so each derivative is
+// formed within its own stage.
+__device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
+
 __device__ __forceinline__ void coords(const Params &P, long idx, int &i, int &j, int &k) {
   int nxy = P.nx * P.ny;
   k = (int)(idx / nxy);
@@ -511,34 +518,97 @@ __global__ void __launch_bounds__(BLK) k_prim(Params P, const sg_config *__restr
 
 // ---------------------------------------------------------------------------
 // Direction pass: all derivatives along DIR and the RHS terms separable in
-// DIR.  Fields read with the stencil: rho u_d (momenta), u, v, w, e, T, p,
-// lambda, mu, h_mix and per species Y, rho D, h.
+// DIR.  The block owns a tile of points; every field (or product of fields)
+// is staged once into LDS with a 5-deep halo along DIR, then each thread
+// evaluates its stencils from LDS at compile-time offsets.  Fields staged:
+// rho u_d, e*rho u_d, e, T, lambda, u, v, w, the three momentum fluxes,
+// p - p_ref, mu, h_mix and per species Y, Y*rho u_d, rho D, h.
+template <int DIR>
+struct PassTile {
+  static constexpr int TX = DIR == 0 ? 64 : 32;
+  static constexpr int TY = DIR == 0 ? 4 : (DIR == 1 ? 8 : 1);
+  static constexpr int TZ = DIR == 2 ? 8 : 1;
+  static constexpr int LX = TX + (DIR == 0 ? 10 : 0);
+  static constexpr int LY = TY + (DIR == 1 ? 10 : 0);
+  static constexpr int LZ = TZ + (DIR == 2 ? 10 : 0);
+  static constexpr int LN = LX * LY * LZ;
+  static constexpr int LSTR = DIR == 0 ? 1 : (DIR == 1 ? LX : LX * LY);   // LDS stride along DIR
+};
+
 template <int DIR, bool BND, int NS>
 __global__ void __launch_bounds__(BLK) k_pass(Params P, Fields F) {
-  long idx = (long)blockIdx.x * BLK + threadIdx.x;
-  if (idx >= P.npts) return;
-  int i, j, k;
-  coords(P, idx, i, j, k);
+  using T_ = PassTile<DIR>;
+  constexpr int TX = T_::TX, TY = T_::TY, LX = T_::LX, LY = T_::LY, LN = T_::LN, LSTR = T_::LSTR;
+  static_assert(T_::TX * T_::TY * T_::TZ == BLK, "tile must match the block");
+  // LN staged values per field, padded to whole block-sized rows so that
+  // every thread loads and stores unconditionally
+  constexpr int NIT = (LN + BLK - 1) / BLK;
+  __shared__ double tile[2][NIT * BLK];
+  const int t = threadIdx.x;
+  const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * T_::TZ;
+  const int i = x0 + lx, j = y0 + ly, k = z0 + lz;
+  const bool active = i < P.nx && j < P.ny && k < P.nz;
+  const long idx = ((long)k * P.ny + j) * P.nx + i;
   const int m = DIR == 0 ? i : (DIR == 1 ? j : k);
   const int n = DIR == 0 ? P.nx : (DIR == 1 ? P.ny : P.nz);
-  const int s = DIR == 0 ? 1 : (DIR == 1 ? (int)P.sy : (int)P.sz);
-  const Sten S = make_sten<BND>(m, n, s);
+  // closure row of this point (bounded direction only)
+  Sten S;
+  S.mode = 0;
+  if (BND) {
+    if (m < 5) S.mode = 1 + m;
+    else if (m > n - 6) S.mode = 11 + (n - 1 - m);
+  }
+  // LDS position of this point
+  const int lbase = (lz + (DIR == 2 ? 5 : 0)) * LX * LY + (ly + (DIR == 1 ? 5 : 0)) * LX + lx + (DIR == 0 ? 5 : 0);
+  // staging: global source index of LDS slot l (wrap when periodic, clamp when bounded)
+  auto gsrc = [&](int l) -> long {
+    int tx = l % LX, ty = (l / LX) % LY, tz = l / (LX * LY);
+    int gx = x0 + tx - (DIR == 0 ? 5 : 0);
+    int gy = y0 + ty - (DIR == 1 ? 5 : 0);
+    int gz = z0 + tz - (DIR == 2 ? 5 : 0);
+    int g = DIR == 0 ? gx : (DIR == 1 ? gy : gz);
+    if (BND) {
+      g = g < 0 ? 0 : (g >= n ? n - 1 : g);
+    } else {
+      g = g < 0 ? g + n : (g >= n ? g - n : g);
+    }
+    if (DIR == 0) gx = g; else if (DIR == 1) gy = g; else gz = g;
+    gx = gx < P.nx ? gx : P.nx - 1;
+    gy = gy < P.ny ? gy : P.ny - 1;
+    gz = gz < P.nz ? gz : P.nz - 1;
+    return ((long)gz * P.ny + gy) * P.nx + gx;
+  };
+  int buf = 0;
+  // Stage one field (or product) into the current buffer.  One barrier per
+  // stage: the two buffers alternate, and the barrier of stage s orders the
+  // reads of stage s-1 before the writes of stage s+1 into the same buffer.
+  auto stage = [&](auto fn) {
+    double v[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      const int l = t + it * BLK;
+      v[it] = fn(gsrc(l < LN ? l : LN - 1));
+    }
+    double *tb = tile[buf];
+#pragma unroll
+    for (int it = 0; it < NIT; it++) tb[t + it * BLK] = v[it];
+    __syncthreads();
+  };
+  auto at = [&](int kk) { return tile[buf][lbase + (kk - 8) * LSTR]; };
+  auto ctr = [&]() { return tile[buf][lbase]; };
+  auto D1 = [&]() { return d1<BND>(P, S, DIR, at); };
+  auto D2 = [&]() { return d2<BND>(P, S, DIR, at); };
+  auto next = [&]() { buf ^= 1; };
   const long fs = P.fstride;
-  const double *__restrict__ Q = F.q;
-  const double *__restrict__ pr = F.prim;
-  const double *rho = Q, *mx = Q + fs, *my = Q + 2 * fs, *mz = Q + 3 * fs;
+  const double *Q = F.q;
+  const double *pr = F.prim;
+  const double *mx = Q + fs, *my = Q + 2 * fs, *mz = Q + 3 * fs;
   const double *md = DIR == 0 ? mx : (DIR == 1 ? my : mz);
   const double *uu = pr + P_U * fs, *vv = pr + P_V * fs, *ww = pr + P_W * fs;
   const double *ee = pr + P_E * fs, *TT = pr + P_T * fs, *pp = pr + P_P * fs;
   const double *la = pr + P_LAM * fs, *mu = pr + P_MU * fs, *hm = pr + P_HMIX * fs;
-  const double *ud = DIR == 0 ? uu : (DIR == 1 ? vv : ww);
-  const double *ptr = pr + P_P * fs;
-  const double pref = ptr[0];   // PRUN(IPREF,JPREF,KPREF), src/rhsvel.f:741
-  auto V1 = [&](const double *a) { return d1<BND>(P, S, DIR, [&](int kk) { return a[idx + S.o[kk]]; }); };
-  auto V2 = [&](const double *a) { return d2<BND>(P, S, DIR, [&](int kk) { return a[idx + S.o[kk]]; }); };
-  auto VP = [&](const double *a, const double *b) {
-    return d1<BND>(P, S, DIR, [&](int kk) { long o = idx + S.o[kk]; return a[o] * b[o]; });
-  };
+  const double pref = pp[0];   // PRUN(IPREF,JPREF,KPREF), src/rhsvel.f:741
   // boundary-plane flags of this direction (src/bcinit.f)
   const bool atL = BND && m == 0, atR = BND && m == n - 1;
   const int fl = 2 * DIR, fr = 2 * DIR + 1;
@@ -550,59 +620,90 @@ __global__ void __launch_bounds__(BLK) k_pass(Params P, Fields F) {
   const bool zadb_any = (i == 0 && P.fadb[0]) || (i == P.nx - 1 && P.fadb[1]) ||
                         (j == 0 && P.fadb[2]) || (j == P.ny - 1 && P.fadb[3]) ||
                         (k == 0 && P.fadb[4]) || (k == P.nz - 1 && P.fadb[5]);
-  // centre values
-  const double r0 = rho[idx], md0 = md[idx], e0 = ee[idx], p0 = pp[idx], lam0 = la[idx],
-               mu0 = mu[idx], hm0 = hm[idx];
-  const double u0[3] = {uu[idx], vv[idx], ww[idx]};
   // ---------------- continuity and energy convection (src/rhscal.f:141-415)
-  const double dm = V1(md);
+  stage([&](long g) { return md[g]; });
+  double dm = D1(), md0 = ctr();
+  pin(dm); pin(md0);
+  next();
   double aD = -dm;
-  const double dem = VP(ee, md);
-  const double de = V1(ee);
+  stage([&](long g) { return ee[g] * md[g]; });
+  double dem = D1();
+  pin(dem);
+  next();
+  stage([&](long g) { return ee[g]; });
+  const double de = D1();
+  const double e0 = ctr();
+  next();
   double aE = -0.5 * (e0 * dm + dem + md0 * de);
+  pin(aE);
   // ---------------- heat conduction (src/rhscal.f:427-797)
   {
-    const double dT = V1(TT);
-    const double dl = V1(la);
-    const double d2T = V2(TT);
+    stage([&](long g) { return TT[g]; });
+    double dT = D1(), d2T = D2();
+    pin(dT); pin(d2T);
+    next();
+    stage([&](long g) { return la[g]; });
+    const double dl = D1();
+    const double lam0 = ctr();
+    next();
     aE = aE + (zcon ? 0.0 : dl) * dT;
     aE = aE + lam0 * (zcon ? 0.0 : d2T);
+    pin(aE);
   }
+  // ---------------- velocity derivatives
+  double du[3], d2u[3], u0[3];
+  stage([&](long g) { return uu[g]; });
+  du[0] = D1(); d2u[0] = D2(); u0[0] = ctr();
+  pin(du[0]); pin(d2u[0]); pin(u0[0]);
+  next();
+  stage([&](long g) { return vv[g]; });
+  du[1] = D1(); d2u[1] = D2(); u0[1] = ctr();
+  pin(du[1]); pin(d2u[1]); pin(u0[1]);
+  next();
+  stage([&](long g) { return ww[g]; });
+  du[2] = D1(); d2u[2] = D2(); u0[2] = ctr();
+  pin(du[2]); pin(d2u[2]); pin(u0[2]);
+  next();
   // ---------------- momentum convection (src/rhsvel.f:348-729, 1312-2110)
-  double du[3], d2u[3];
-  du[0] = V1(uu); du[1] = V1(vv); du[2] = V1(ww);
   double aM[3];
   {
+    const double r0 = active ? Q[idx] : 0.0;
     // fluxes as the reference forms them: rho u*u, rho u*v, rho u*w, rho v*v, rho v*w, rho w*w
-    double fl0, fl1, fl2;
-    if (DIR == 0) { fl0 = VP(mx, uu); fl1 = VP(mx, vv); fl2 = VP(mx, ww); }
-    else if (DIR == 1) { fl0 = VP(mx, vv); fl1 = VP(my, vv); fl2 = VP(my, ww); }
-    else { fl0 = VP(mx, ww); fl1 = VP(my, ww); fl2 = VP(mz, ww); }
-    const double flx[3] = {fl0, fl1, fl2};
+    const double *fa[3], *fb[3];
+    if (DIR == 0) { fa[0] = mx; fb[0] = uu; fa[1] = mx; fb[1] = vv; fa[2] = mx; fb[2] = ww; }
+    else if (DIR == 1) { fa[0] = mx; fb[0] = vv; fa[1] = my; fb[1] = vv; fa[2] = my; fb[2] = ww; }
+    else { fa[0] = mx; fb[0] = ww; fa[1] = my; fb[1] = ww; fa[2] = mz; fb[2] = ww; }
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      if (a == DIR) aM[a] = -0.5 * (md0 * du[a] + flx[a] + u0[a] * dm);
-      else aM[a] = -0.5 * (flx[a] + u0[a] * dm) - 0.5 * r0 * u0[DIR] * du[a];
+      const double *A_ = fa[a], *B_ = fb[a];
+      stage([&](long g) { return A_[g] * B_[g]; });
+      const double flx = D1();
+      next();
+      if (a == DIR) aM[a] = -0.5 * (md0 * du[a] + flx + u0[a] * dm);
+      else aM[a] = -0.5 * (flx + u0[a] * dm) - 0.5 * r0 * u0[DIR] * du[a];
+      pin(aM[a]);
     }
   }
   // ---------------- pressure gradient and pressure work (src/rhsvel.f:741-917)
   {
-    const double dp = d1<BND>(P, S, DIR, [&](int kk) { return ptr[idx + S.o[kk]] - pref; });
+    stage([&](long g) { return pp[g] - pref; });
+    const double dp = D1();
+    const double p0 = ctr() + pref;
+    next();
     aM[DIR] = aM[DIR] - dp;
     aE = aE - u0[DIR] * dp - p0 * du[DIR];
-    if (DIR == 0 && BND && (i == 0 || i == P.nx - 1)) {
+    pin(aM[DIR]); pin(aE);
+    if (active && (P.inflow_xl || P.outflow_xr) && (i == 0 || i == P.nx - 1)) {
       long nf = (long)P.ny * P.nz, fq = (long)k * P.ny + j;
       double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
-      fc[F_DPDX * nf + fq] = dp;
-      fc[F_DRDX * nf + fq] = V1(rho);
-    } else if (DIR != 0 && (P.inflow_xl || P.outflow_xr) && (i == 0 || i == P.nx - 1)) {
-      long nf = (long)P.ny * P.nz, fq = (long)k * P.ny + j;
-      double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
-      fc[(DIR == 1 ? F_DPDY : F_DPDZ) * nf + fq] = dp;
+      fc[(DIR == 0 ? F_DPDX : (DIR == 1 ? F_DPDY : F_DPDZ)) * nf + fq] = dp;
     }
   }
   // ---------------- separable viscous terms (src/rhsvel.f:1462-2305)
-  d2u[0] = V2(uu); d2u[1] = V2(vv); d2u[2] = V2(ww);
+  stage([&](long g) { return mu[g]; });
+  double dmu = D1(), mu0 = ctr();
+  pin(dmu); pin(mu0);
+  next();
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     double tv;
@@ -610,21 +711,38 @@ __global__ void __launch_bounds__(BLK) k_pass(Params P, Fields F) {
     else tv = zvst ? 0.0 : mu0 * d2u[a];
     aM[a] = aM[a] + tv;
     aE = aE + tv * u0[a];
+    pin(aM[a]);
   }
-  const double dmu = V1(mu);
+  pin(aE);
+  if (active) {
+    double *G = F.grad + (long)DIR * 4 * fs;
+    G[idx] = du[0];
+    G[fs + idx] = du[1];
+    G[2 * fs + idx] = du[2];
+    G[3 * fs + idx] = dmu;
+  }
   // ---------------- species (src/rhscal.f:1030-3944)
   double vt = 0.0, ucor = 0.0;
-  double aY[NS], dYz[NS], y0[NS];
+  double aY[NS], dYz[NS], yv[NS];
 #pragma unroll
   for (int sp = 0; sp < NS; sp++) {
     const double *Ys = pr + prim_y(P, sp), *Rs = pr + prim_rd(P, sp), *Hs = pr + prim_h(P, sp);
-    const double yc = Ys[idx], rdc = Rs[idx], hc = Hs[idx];
-    y0[sp] = yc;
-    const double dY = V1(Ys);
-    const double dYm = VP(Ys, md);
-    const double drD = V1(Rs);
-    const double dh = V1(Hs);
-    const double d2Y = V2(Ys);
+    stage([&](long g) { return Ys[g]; });
+    double dY = D1(), d2Y = D2(), yc = ctr();
+    pin(dY); pin(d2Y); pin(yc);
+    next();
+    stage([&](long g) { return Ys[g] * md[g]; });
+    double dYm = D1();
+    pin(dYm);
+    next();
+    stage([&](long g) { return Rs[g]; });
+    double drD = D1(), rdc = ctr();
+    pin(drD); pin(rdc);
+    next();
+    stage([&](long g) { return Hs[g]; });
+    const double dh = D1();
+    const double hc = ctr();
+    next();
     double ys = -0.5 * yc * dm - 0.5 * dYm - 0.5 * md0 * dY;
     const double g1 = (zdif ? 0.0 : drD) * dY;
     const double g2 = rdc * (zdif ? 0.0 : d2Y);
@@ -636,18 +754,21 @@ __global__ void __launch_bounds__(BLK) k_pass(Params P, Fields F) {
     ucor = ucor + rdc * dY;
     aY[sp] = ys;
     dYz[sp] = zdif ? 0.0 : dY;
-    if (DIR == 0 && BND && (i == 0 || i == P.nx - 1)) {
+    yv[sp] = yc;
+    pin(aY[sp]); pin(dYz[sp]); pin(yv[sp]); pin(vt); pin(ucor); pin(aE);
+    if (DIR == 0 && BND && active && (i == 0 || i == P.nx - 1)) {
       long nf = (long)P.ny * P.nz, fq = (long)k * P.ny + j;
       double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
       fc[(F_NSCAL + sp) * nf + fq] = dY;
     }
   }
   {
-    const double dhm = V1(hm);
+    stage([&](long g) { return hm[g]; });
+    const double dhm = D1();
+    const double hm0 = ctr();
     aE = aE - hm0 * (zadb_any ? 0.0 : vt) - (zdifadb ? 0.0 : dhm) * ucor;
   }
-#pragma unroll
-  for (int sp = 0; sp < NS; sp++) aY[sp] = aY[sp] - y0[sp] * vt - dYz[sp] * ucor;
+  if (!active) return;
   // ---------------- store
   double *A = F.acc;
   if (DIR == 0) {
@@ -656,22 +777,19 @@ __global__ void __launch_bounds__(BLK) k_pass(Params P, Fields F) {
     A[2 * fs + idx] = aM[1];
     A[3 * fs + idx] = aM[2];
     A[4 * fs + idx] = aE;
-#pragma unroll
-    for (int sp = 0; sp < NS; sp++) A[(5 + sp) * fs + idx] = F.rate[sp * fs + idx] + aY[sp];
   } else {
     A[idx] += aD;
     A[fs + idx] += aM[0];
     A[2 * fs + idx] += aM[1];
     A[3 * fs + idx] += aM[2];
     A[4 * fs + idx] += aE;
-#pragma unroll
-    for (int sp = 0; sp < NS; sp++) A[(5 + sp) * fs + idx] += aY[sp];
   }
-  double *G = F.grad + (long)DIR * 4 * fs;
-  G[idx] = du[0];
-  G[fs + idx] = du[1];
-  G[2 * fs + idx] = du[2];
-  G[3 * fs + idx] = dmu;
+#pragma unroll
+  for (int sp = 0; sp < NS; sp++) {
+    const double ay = aY[sp] - yv[sp] * vt - dYz[sp] * ucor;
+    if (DIR == 0) A[(5 + sp) * fs + idx] = F.rate[sp * fs + idx] + ay;
+    else A[(5 + sp) * fs + idx] += ay;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -795,8 +913,9 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
     for (int s = 0; s < NS; s++) strh[s] = pr[prim_h(P, s) + idx] - strg * strt * c->rgspec[s] / strr;
     if (i == 0 && P.inflow_xl) {
       // NSBCI3, src/bounds.f:610-682 (imposed values have zero time derivative)
-      double bcl1 = dux, bcl2 = fc[F_DRDX * nf + fq], bcl3 = dvx, bcl4 = dwx,
-             bcl5 = fc[F_DPDX * nf + fq];
+      // drho/dx at the face (src/rhscal.f:3966-3990), one-sided closure row
+      const double drdx = d1<BND>(P, Sx, 0, [&](int kk) { return F.q[idx + Sx.o[kk]]; });
+      double bcl1 = dux, bcl2 = drdx, bcl3 = dvx, bcl4 = dwx, bcl5 = fc[F_DPDX * nf + fq];
       fornow = strd * acou * bcl1;
       bcl1 = 0.5 * (u - acou) * (bcl5 - fornow);
       bcl2 = u * (bcl2 - bcl5 * ova2);
@@ -913,16 +1032,22 @@ template <int NS>
 void launch_prim_t(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st) {
   hipLaunchKernelGGL(k_prim<NS>, dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F);
 }
+template <int DIR>
+static dim3 pass_grid(const Params &P) {
+  using T_ = PassTile<DIR>;
+  return dim3((P.nx + T_::TX - 1) / T_::TX, (P.ny + T_::TY - 1) / T_::TY, (P.nz + T_::TZ - 1) / T_::TZ);
+}
+
 template <int NS>
 void launch_pass_t(int dir, const Params &P, const Fields &F, hipStream_t st) {
-  dim3 g(grid_for(P.npts)), b(BLK);
+  dim3 b(BLK);
   if (dir == 0) {
-    if (P.bnd[0]) hipLaunchKernelGGL((k_pass<0, true, NS>), g, b, 0, st, P, F);
-    else hipLaunchKernelGGL((k_pass<0, false, NS>), g, b, 0, st, P, F);
+    if (P.bnd[0]) hipLaunchKernelGGL((k_pass<0, true, NS>), pass_grid<0>(P), b, 0, st, P, F);
+    else hipLaunchKernelGGL((k_pass<0, false, NS>), pass_grid<0>(P), b, 0, st, P, F);
   } else if (dir == 1) {
-    hipLaunchKernelGGL((k_pass<1, false, NS>), g, b, 0, st, P, F);
+    hipLaunchKernelGGL((k_pass<1, false, NS>), pass_grid<1>(P), b, 0, st, P, F);
   } else {
-    hipLaunchKernelGGL((k_pass<2, false, NS>), g, b, 0, st, P, F);
+    hipLaunchKernelGGL((k_pass<2, false, NS>), pass_grid<2>(P), b, 0, st, P, F);
   }
 }
 template <int NS>
