@@ -143,7 +143,10 @@ size_t sg_config_size(void);
 /* Create a handle on the current HIP device: validates cfg, derives the
  * stencil coefficients (src/dfinit.f), the BC flags and loop ranges
  * (src/bcinit.f), allocates the device registers.  Replaces the table set-up
- * part of INDATA/DFINIT/BCINIT. */
+ * part of INDATA/DFINIT/BCINIT.  Returns SG_EINVAL for a species count that is
+ * not compiled in, nx/ny/nz < 11, more than 2^29 points per field, cp fits of
+ * more than 5 coefficients (src/temper.f NCTMAX) or transport fits of more
+ * than SG_NTRC coefficients. */
 int sg_create(const sg_config *cfg, sg_handle **out);
 int sg_destroy(sg_handle *h);
 

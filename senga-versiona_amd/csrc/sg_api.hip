@@ -25,6 +25,7 @@ const char *kKernelNames[K_COUNT] = {"tempin", "boundt", "prim", "pass_x", "pass
 struct sg_handle {
   sg_config cfg;       // host copy (species lists 0-based, REAL-rounded falloff constants)
   sg_config *dcfg;     // device copy
+  Tables tab;          // derived tables (host copy)
   Params P;
   Fields F;
   hipStream_t stream;
@@ -43,6 +44,54 @@ struct sg_handle {
   double prof_ms[K_COUNT];
   int64_t prof_n[K_COUNT];
 };
+
+// Derived tables (see Tables).  Transport fits are zero-padded above their
+// order in the device copy of the configuration for the same reason.
+static void build_tables(sg_config &c, Tables &t) {
+  memset(&t, 0, sizeof t);
+  for (int s = 0; s < c.nspec; s++) {
+    for (int it = 0; it < c.ntint[s]; it++) {
+      int np = c.ncpoly[s][it];
+      for (int m = 0; m < np; m++) {
+        t.cp[s][it][m] = c.amascp[s][it][m];
+        t.ct[s][it][m] = c.amasct[s][it][m];
+        t.ch[s][it][m] = c.amasch[s][it][m];
+        t.gb[s][it][m] = c.amolgb[s][it][m];
+      }
+      t.hcp[s][it] = c.amascp[s][it][c.ncenth[s][it] - 1];
+      t.hch[s][it] = c.amasch[s][it][c.ncenth[s][it] - 1];
+      t.gbh[s][it] = c.amolgb[s][it][c.ncenth[s][it] - 1];
+      t.gbs[s][it] = c.amolgb[s][it][c.ncenpy[s][it] - 1];
+    }
+    // an unused second interval repeats the first, so a per-point choice
+    // between the two is always well defined
+    for (int it = c.ntint[s]; it < SG_NTINT; it++) {
+      for (int m = 0; m < SG_NCPMAX; m++) {
+        t.cp[s][it][m] = t.cp[s][0][m];
+        t.ct[s][it][m] = t.ct[s][0][m];
+        t.ch[s][it][m] = t.ch[s][0][m];
+        t.gb[s][it][m] = t.gb[s][0][m];
+      }
+      t.hcp[s][it] = t.hcp[s][0];
+      t.hch[s][it] = t.hch[s][0];
+      t.gbh[s][it] = t.gbh[s][0];
+      t.gbs[s][it] = t.gbs[s][0];
+    }
+  }
+  if (c.mixavg) {
+    for (int s = 0; s < SG_MAX_SPEC; s++) {
+      for (int m = c.ncovis; m < SG_NTRC; m++) c.viscco[s][m] = 0.0;
+      for (int m = c.ncocon; m < SG_NTRC; m++) c.condco[s][m] = 0.0;
+      for (int j = 0; j < SG_MAX_SPEC; j++)
+        for (int m = c.ncodif; m < SG_NTRC; m++) c.diffco[s][j][m] = 0.0;
+    }
+  }
+  t.diffsym = 1;
+  for (int i = 0; i < c.nspec; i++)
+    for (int j = i + 1; j < c.nspec; j++)
+      for (int m = 0; m < SG_NTRC; m++)
+        if (c.diffco[i][j][m] != c.diffco[j][i][m]) t.diffsym = 0;
+}
 
 static int set_err(sg_handle *h, int code, const char *msg, int i = 0, int j = 0, int k = 0) {
   if (h) {
@@ -182,8 +231,16 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   if (cfg->mixavg && (cfg->ncovis < 1 || cfg->ncocon < 1 || cfg->ncodif < 1 || cfg->ncovis > SG_NTRC ||
                       cfg->ncocon > SG_NTRC || cfg->ncodif > SG_NTRC))
     return SG_EINVAL;
+  for (int s = 0; s < cfg->nspec; s++) {
+    if (cfg->ntint[s] < 1 || cfg->ntint[s] > SG_NTINT) return SG_EINVAL;
+    for (int it = 0; it < cfg->ntint[s]; it++)
+      if (cfg->ncpoly[s][it] < 1 || cfg->ncpoly[s][it] > SG_NCPMAX || cfg->ncenth[s][it] < 1 ||
+          cfg->ncenth[s][it] > SG_NCOF || cfg->ncenpy[s][it] < 1 || cfg->ncenpy[s][it] > SG_NCOF)
+        return SG_EINVAL;
+  }
   sg_handle *h = new sg_handle();
   h->cfg = *cfg;
+  build_tables(h->cfg, h->tab);
   // The reference reads these falloff constants through implicitly typed
   // REAL locals (FLCNST, TALPHA, CFCST1/2, ENCST1/2, DTCNST in
   // src/chrate.f:249,319-330); round them once here.
@@ -193,6 +250,7 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
     for (int q = 0; q < 6; q++) h->cfg.rctroe[j][tr[q]] = (double)(float)h->cfg.rctroe[j][tr[q]];
   }
   int rc = setup_params(h);
+  if (!rc && h->P.fstride * (long)sizeof(double) >= (1L << 32)) rc = SG_EINVAL;  // 32-bit field offsets
   if (rc) { delete h; return rc; }
   if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
@@ -206,7 +264,8 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   size_t itn = ((sizeof(uint32_t) * (size_t)fs + 255) / 256) * 256;
   size_t facesz = sizeof(double) * 2 * (size_t)nface(ns) * P.ny * P.nz;
   size_t cfgsz = ((sizeof(sg_config) + 255) / 256) * 256;
-  h->bytes = bytes + itn + ((facesz + 255) / 256) * 256 + cfgsz + 256;
+  size_t tabsz = ((sizeof(Tables) + 255) / 256) * 256;
+  h->bytes = bytes + itn + ((facesz + 255) / 256) * 256 + cfgsz + tabsz + 256;
   if (hipMalloc(&h->base, h->bytes) != hipSuccess) {
     hipStreamDestroy(h->stream);
     delete h;
@@ -225,9 +284,11 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   F.itndex = (uint32_t *)b; b += itn;
   F.face = (double *)b; b += ((facesz + 255) / 256) * 256;
   h->dcfg = (sg_config *)b; b += cfgsz;
+  F.tab = (const Tables *)b; b += tabsz;
   F.err = (int *)b;
   if (hipMemset(h->base, 0, h->bytes) != hipSuccess ||
-      hipMemcpy(h->dcfg, &h->cfg, sizeof(sg_config), hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(h->dcfg, &h->cfg, sizeof(sg_config), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy((void *)F.tab, &h->tab, sizeof(Tables), hipMemcpyHostToDevice) != hipSuccess) {
     hipFree(h->base);
     hipStreamDestroy(h->stream);
     delete h;

@@ -32,8 +32,27 @@ struct Params {
   double alamda, rlamda, prantl, tdifgb, pdifgb;
 };
 
+// Thermodynamic tables derived from sg_config when a handle is created.  The
+// NASA polynomials are held as fixed SG_NCPMAX-term arrays, zero-padded above
+// each interval's own order (a zero leading coefficient leaves Horner's result
+// bit-identical), with the enthalpy and entropy constants pulled out, so that
+// kernels evaluate them with fixed trip counts and uniform loads.
+#define SG_NCPMAX 5   // highest cp order the temperature solve accepts (src/temper.f NCTMAX)
+struct Tables {
+  double cp[SG_MAX_SPEC][SG_NTINT][SG_NCPMAX];   // amascp polynomial part
+  double ct[SG_MAX_SPEC][SG_NTINT][SG_NCPMAX];   // amasct
+  double ch[SG_MAX_SPEC][SG_NTINT][SG_NCPMAX];   // amasch
+  double gb[SG_MAX_SPEC][SG_NTINT][SG_NCPMAX];   // amolgb
+  double hcp[SG_MAX_SPEC][SG_NTINT];             // amascp(ncenth)
+  double hch[SG_MAX_SPEC][SG_NTINT];             // amasch(ncenth)
+  double gbh[SG_MAX_SPEC][SG_NTINT];             // amolgb(ncenth)
+  double gbs[SG_MAX_SPEC][SG_NTINT];             // amolgb(ncenpy)
+  int diffsym;                                   // binary diffusion fits symmetric in (i, j)
+};
+
 // Device field set: all arrays are [field][z][y][x] with field stride fstride.
 struct Fields {
+  const Tables *tab;   // derived thermodynamic tables (device)
   double *q;      // stage register (5+ns): rho, rho u, rho v, rho w, rho E, rho Y
   double *run;    // solution register (5+ns)
   double *prim;   // primitives: see PrimSlot, then Y[ns], rhoD[ns], h[ns]

@@ -26,6 +26,19 @@ namespace sg {
 // formed within its own stage.
 __device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
 
+// Scheduling boundary between the phases of a long straight-line kernel, so
+// that independent phases are not interleaved (which multiplies live values).
+#define SG_PHASE() __builtin_amdgcn_sched_barrier(0)
+
+// Element at byte offset bo of a field.  A 32-bit offset from a uniform field
+// base lets loads and stores use the scalar-base addressing form, instead of a
+// live 64-bit vector address per field (field byte offsets stay below 2^32,
+// which handle creation checks).
+template <class T>
+__device__ __forceinline__ T &fld(T *base, uint32_t bo) {
+  return *(T *)((char *)base + bo);
+}
+
 __device__ __forceinline__ void coords(const Params &P, long idx, int &i, int &j, int &k) {
   int nxy = P.nx * P.ny;
   k = (int)(idx / nxy);
@@ -186,40 +199,80 @@ __global__ void k_bountt(Params P, const sg_config *__restrict__ c, Fields F) {
 // (src/rhscal.f:515-573, 3854-3891, src/rhsvel.f:926-980), CHRATE
 // (src/chrate.f:146-760), primitive variables, diffusivities
 // (src/rhscal.f:1292-1346) and species enthalpies (src/rhscal.f:1444-1470).
+// Coefficient m of species s's interval it (0 or 1) from a two-interval table:
+// both candidates are uniform loads, the choice is per point.
+// The empty asm pins each candidate in scalar registers; without it the
+// compiler turns the choice of two loaded values into a per-point load from a
+// chosen address, i.e. a vector load and live registers for every coefficient.
+__device__ __forceinline__ double uni(double x) {
+  asm("" : "+s"(x));
+  return x;
+}
+template <int N>
+__device__ __forceinline__ double pick(const double (&a)[SG_NTINT][N], int it, int m) {
+  static_assert(SG_NTINT == 2, "two temperature intervals");
+  const double a0 = uni(a[0][m]), a1 = uni(a[1][m]);
+  return it ? a1 : a0;
+}
+__device__ __forceinline__ double pick1(const double (&a)[SG_NTINT], int it) {
+  const double a0 = uni(a[0]), a1 = uni(a[1]);
+  return it ? a1 : a0;
+}
+
+// Horner over the zero-padded SG_NCPMAX-term polynomial of interval it.
+__device__ __forceinline__ double horner(const double (&a)[SG_NTINT][SG_NCPMAX], int it, double x) {
+  double f = pick(a, it, SG_NCPMAX - 1);
+#pragma unroll
+  for (int m = SG_NCPMAX - 2; m >= 0; m--) f = f * x + pick(a, it, m);
+  return f;
+}
+
+// Transport fits, zero-padded to SG_NTRC terms.
+__device__ __forceinline__ double trfit(const double *a, double x) {
+  double f = a[SG_NTRC - 1];
+#pragma unroll
+  for (int m = SG_NTRC - 2; m >= 0; m--) f = f * x + a[m];
+  return f;
+}
+
 template <int NS>
-__global__ void __launch_bounds__(BLK) k_prim(Params P, const sg_config *__restrict__ c, Fields F) {
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k_prim(Params P, const sg_config *__restrict__ c,
+                                              const Tables *__restrict__ tb, Fields F) {
   long idx = (long)blockIdx.x * BLK + threadIdx.x;
   if (idx >= P.npts) return;
+  const uint32_t bo = (uint32_t)idx * 8u;
   const long fs = P.fstride;
   const double *q = F.q;
   double *pr = F.prim;
-  double d = q[idx], ru = q[fs + idx], rv = q[2 * fs + idx], rw = q[3 * fs + idx];
-  double re = q[4 * fs + idx];
+  double d = fld(q, bo), ru = fld(q + fs, bo), rv = fld(q + 2 * fs, bo), rw = fld(q + 3 * fs, bo);
+  double re = fld(q + 4 * fs, bo);
   double ry[NS];
 #pragma unroll
-  for (int s = 0; s < NS; s++) ry[s] = q[(5 + s) * fs + idx];
-  // --- TEMPER ---
+  for (int s = 0; s < NS; s++) ry[s] = fld(q + (5 + s) * fs, bo);
+  // --- TEMPER (src/temper.f:58-160) ---
   double ukuk = (ru * ru + rv * rv + rw * rw) / d;
   double tc[6], td[6];
   tc[0] = 0.5 * ukuk - re;
 #pragma unroll
   for (int m = 1; m <= 5; m++) { tc[m] = 0.0; td[m] = 0.0; }
   double s7 = 0.0;
-  uint32_t idxo = F.itndex[idx];
+  uint32_t idxo = fld(F.itndex, bo / 2u);
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     int it = itint_of(idxo, s);
-    tc[0] = tc[0] + ry[s] * c->amascp[s][it][c->ncenth[s][it] - 1];
-    tc[1] = tc[1] + ry[s] * c->amasct[s][it][0];
+    tc[0] = tc[0] + ry[s] * pick1(tb->hcp[s], it);
+    tc[1] = tc[1] + ry[s] * pick(tb->ct[s], it, 0);
     td[1] = tc[1];
-    for (int m = 2; m <= c->ncpoly[s][it]; m++) {
-      tc[m] = tc[m] + ry[s] * c->amasct[s][it][m - 1];
-      td[m] = td[m] + ry[s] * c->amascp[s][it][m - 1];
+    // terms above the interval's order add ry*0, which leaves the sums unchanged
+#pragma unroll
+    for (int m = 2; m <= SG_NCPMAX; m++) {
+      tc[m] = tc[m] + ry[s] * pick(tb->ct[s], it, m - 1);
+      td[m] = td[m] + ry[s] * pick(tb->cp[s], it, m - 1);
     }
     s7 = s7 + ry[s] * c->rgspec[s];
   }
   const double toltmp = (double)1.0e-10f;  // REAL literal, src/temper.f:33-34
-  double t = pr[P_T * fs + idx];
+  double t = fld(pr + P_T * fs, bo);
   int iter = 1;
   bool ok = true;
   for (;;) {
@@ -254,20 +307,19 @@ __global__ void __launch_bounds__(BLK) k_prim(Params P, const sg_config *__restr
     int it = locate(c, s, t);
     itn[s] = it;
     nidx += (uint32_t)it << (2 * s);
-    double f = c->amascp[s][it][c->ncpoly[s][it] - 1];
-    for (int m = c->ncpoly[s][it] - 2; m >= 0; m--) f = f * t + c->amascp[s][it][m];
-    cp = cp + ry[s] * f;
+    cp = cp + ry[s] * horner(tb->cp[s], it, t);
   }
   cp = cp / d;
-  F.itndex[idx] = nidx;
+  fld(F.itndex, bo / 2u) = nidx;
+  SG_PHASE();
   double pres = t * s7;
-  pr[P_T * fs + idx] = t;
-  pr[P_P * fs + idx] = pres;
+  fld(pr + P_T * fs, bo) = t;
+  fld(pr + P_P * fs, bo) = pres;
   // primitives
-  pr[P_U * fs + idx] = ru / d;
-  pr[P_V * fs + idx] = rv / d;
-  pr[P_W * fs + idx] = rw / d;
-  pr[P_E * fs + idx] = re / d;
+  fld(pr + P_U * fs, bo) = ru / d;
+  fld(pr + P_V * fs, bo) = rv / d;
+  fld(pr + P_W * fs, bo) = rw / d;
+  fld(pr + P_E * fs, bo) = re / d;
   // --- conductivity (src/rhscal.f:515-573) ---
   double lam, transp, wmomix = 0.0;
   {
@@ -275,21 +327,15 @@ __global__ void __launch_bounds__(BLK) k_prim(Params P, const sg_config *__restr
     lam = fornow * cp;
     transp = fornow;
   }
-  double ctr[NS];
   if (P.mixavg) {
     transp = log(t / P.tdifgb);
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-      double f = c->condco[s][c->ncocon - 1];
-      for (int m = c->ncocon - 2; m >= 0; m--) f = f * transp + c->condco[s][m];
-      ctr[s] = exp(f);
-    }
     double c1 = 0.0, c2 = 0.0, c3 = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
+      double ctr = exp(trfit(c->condco[s], transp));
       double fornow = ry[s] * c->ovwmol[s];
-      c1 = c1 + fornow * ctr[s];
-      c2 = c2 + fornow / ctr[s];
+      c1 = c1 + fornow * ctr;
+      c2 = c2 + fornow / ctr;
       c3 = c3 + fornow;
     }
     c3 = 1.0 / c3;
@@ -298,15 +344,31 @@ __global__ void __launch_bounds__(BLK) k_prim(Params P, const sg_config *__restr
     lam = 0.5 * (c1 + 1.0 / c2);
     wmomix = d * c3;
   }
-  pr[P_LAM * fs + idx] = lam;
+  fld(pr + P_LAM * fs, bo) = lam;
+  SG_PHASE();
   // --- chemistry (src/chrate.f:146-760) on rho Y ---
   {
     const double ysmall = 1.0e-30, ydenom = 1.0e-15;
     double rate[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) rate[s] = 0.0;
-    double lt = log(t), ot = 1.0 / t;
-    (void)ot;
+    double lt = log(t);
+    // Gibbs functions of all species at this temperature, each formed as
+    // src/chrate.f:612-632 forms it for every step it enters
+    double gibbs[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      int it = itn[s];
+      double g = horner(tb->gb[s], it, t);
+      gibbs[s] = pick1(tb->gbh[s], it) / t - pick1(tb->gbs[s], it) * lt - g;
+    }
+    auto sel = [&](const double *v, int s) {
+      double r = 0.0;
+#pragma unroll
+      for (int u = 0; u < NS; u++)
+        if (u == s) r = v[u];
+      return r;
+    };
     for (int st = 0; st < c->nstep; st++) {
       bool flthrd = true;
       double s3 = 0.0;
@@ -369,52 +431,27 @@ __global__ void __launch_bounds__(BLK) k_prim(Params P, const sg_config *__restr
         flthrd = false;
       }
       for (int m = 0; m < c->nrslen[st]; m++) {
-        int s = c->nrspec[st][m];
-        double fornow = 0.0;
-#pragma unroll
-        for (int u = 0; u < NS; u++)
-          if (u == s) fornow = ry[u] * c->ovwmol[u];
+        double fornow = sel(ry, c->nrspec[st][m]) * c->ovwmol[c->nrspec[st][m]];
         fornow = fornow > 0.0 ? fornow : 0.0;
         s1 = s1 * fornow;
       }
       for (int m = 0; m < c->nrclen[st]; m++) {
-        int s = c->nrcpec[st][m];
-        double fornow = 0.0;
-#pragma unroll
-        for (int u = 0; u < NS; u++)
-          if (u == s) fornow = ry[u] * c->ovwmol[u];
+        double fornow = sel(ry, c->nrcpec[st][m]) * c->ovwmol[c->nrcpec[st][m]];
         fornow = fornow > ysmall ? fornow : ysmall;
         fornow = exp(c->crspec[st][m] * log(fornow));
         fornow = fornow / (1.0 + ydenom * fornow);
         s1 = s1 * fornow;
       }
       if (c->mglist[st] != 0) {
-        for (int m = 0; m < c->nsslen[st]; m++) {
-          int s = c->nsspec[st][m];
-          int it = itint_of(nidx, s);
-          const double *gb = c->amolgb[s][it];
-          int np = c->ncpoly[s][it];
-          double g = gb[np - 1];
-          for (int cc = np - 2; cc >= 0; cc--) g = gb[cc] + g * t;
-          g = gb[c->ncenth[s][it] - 1] / t - gb[c->ncenpy[s][it] - 1] * lt - g;
-          s2 = s2 + c->diffmu[st][m] * g;
-        }
+        for (int m = 0; m < c->nsslen[st]; m++) s2 = s2 + c->diffmu[st][m] * sel(gibbs, c->nsspec[st][m]);
         s2 = exp(s2);
         for (int m = 0; m < c->npslen[st]; m++) {
-          int s = c->npspec[st][m];
-          double fornow = 0.0;
-#pragma unroll
-          for (int u = 0; u < NS; u++)
-            if (u == s) fornow = ry[u] * c->ovwmol[u];
+          double fornow = sel(ry, c->npspec[st][m]) * c->ovwmol[c->npspec[st][m]];
           fornow = fornow > 0.0 ? fornow : 0.0;
           s2 = s2 * fornow;
         }
         for (int m = 0; m < c->npclen[st]; m++) {
-          int s = c->npcpec[st][m];
-          double fornow = 0.0;
-#pragma unroll
-          for (int u = 0; u < NS; u++)
-            if (u == s) fornow = ry[u] * c->ovwmol[u];
+          double fornow = sel(ry, c->npcpec[st][m]) * c->ovwmol[c->npcpec[st][m]];
           fornow = fornow > ysmall ? fornow : ysmall;
           fornow = exp(c->cpspec[st][m] * log(fornow));
           fornow = fornow / (1.0 + ydenom * fornow);
@@ -432,59 +469,91 @@ __global__ void __launch_bounds__(BLK) k_prim(Params P, const sg_config *__restr
       }
     }
 #pragma unroll
-    for (int s = 0; s < NS; s++) F.rate[s * fs + idx] = rate[s];
+    for (int s = 0; s < NS; s++) fld(F.rate + s * fs, bo) = rate[s];
   }
-  // --- mass fractions, diffusivities, enthalpies ---
+  // --- mass fractions and enthalpies (src/rhscal.f:1444-1470) ---
   double y[NS];
 #pragma unroll
   for (int s = 0; s < NS; s++) y[s] = ry[s] / d;
   double hmix = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; s++) {
-    pr[prim_y(P, s) + idx] = y[s];
+    fld(pr + prim_y(P, s), bo) = y[s];
     int it = itn[s];
-    double f = c->amasch[s][it][c->ncpoly[s][it] - 1];
-    for (int m = c->ncpoly[s][it] - 2; m >= 0; m--) f = f * t + c->amasch[s][it][m];
-    double h = c->amasch[s][it][c->ncenth[s][it] - 1] + f * t;
-    pr[prim_h(P, s) + idx] = h;
+    double h = pick1(tb->hch[s], it) + horner(tb->ch[s], it, t) * t;
+    fld(pr + prim_h(P, s), bo) = h;
     hmix = hmix + h * y[s];
   }
-  pr[P_HMIX * fs + idx] = hmix;
+  fld(pr + P_HMIX * fs, bo) = hmix;
+  SG_PHASE();
   if (!P.mixavg) {
 #pragma unroll
-    for (int s = 0; s < NS; s++) pr[prim_rd(P, s) + idx] = transp * c->olewis[s];
-    pr[P_MU * fs + idx] = transp * P.prantl;
+    for (int s = 0; s < NS; s++) fld(pr + prim_rd(P, s), bo) = transp * c->olewis[s];
+    fld(pr + P_MU * fs, bo) = transp * P.prantl;
   } else {
     // binary diffusion (src/rhscal.f:1311-1346)
+    double yt[NS];
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-      double ctd[NS];
+    for (int s = 0; s < NS; s++) yt[s] = y[s] + 1.0e-12;
+    double c1t = 0.0;
 #pragma unroll
-      for (int js = 0; js < NS; js++) {
-        const double *dc = c->diffco[s][js];
-        double f = dc[c->ncodif - 1];
-        for (int m = c->ncodif - 2; m >= 0; m--) f = f * transp + dc[m];
-        ctd[js] = exp(f) * P.pdifgb / pres;
+    for (int js = 0; js < NS; js++) c1t = c1t + yt[js];
+    if (tb->diffsym) {
+      // D_ij = D_ji: one fit per unordered pair, its term added to both
+      // sums.  Visiting pairs (a, b >= a) row by row delivers the terms of
+      // every sum in increasing species order, the reference's order.
+      double cs[NS], self[NS];
+#pragma unroll
+      for (int s = 0; s < NS; s++) cs[s] = 0.0;
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+#pragma unroll
+        for (int b = a; b < NS; b++) {
+          const double ctd = exp(trfit(c->diffco[a][b], transp)) * P.pdifgb / pres;
+          const double tab = yt[b] * c->ovwmol[b] / ctd;
+          cs[a] = cs[a] + tab;
+          if (b == a) {
+            self[a] = tab;
+          } else {
+            const double tba = yt[a] * c->ovwmol[a] / ctd;
+            cs[b] = cs[b] + tba;
+          }
+        }
       }
-      double c1 = 0.0, c2 = 0.0;
 #pragma unroll
-      for (int js = 0; js < NS; js++) {
-        double fornow = y[js] + 1.0e-12;
-        c1 = c1 + fornow;
-        c2 = c2 + fornow * c->ovwmol[js] / ctd[js];
+      for (int s = 0; s < NS; s++) {
+        double c1 = c1t - yt[s];
+        double c2 = cs[s] - self[s];
+        c2 = c2 * wmomix;
+        fld(pr + prim_rd(P, s), bo) = d * c1 / c2;
       }
-      double fornow = y[s] + 1.0e-12;
-      c1 = c1 - fornow;
-      c2 = c2 - fornow * c->ovwmol[s] / ctd[s];
-      c2 = c2 * wmomix;
-      pr[prim_rd(P, s) + idx] = d * c1 / c2;
+    } else {
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        double c2 = 0.0, self = 0.0;
+#pragma unroll
+        for (int js = 0; js < NS; js++) {
+          const double ctd = exp(trfit(c->diffco[s][js], transp)) * P.pdifgb / pres;
+          const double term = yt[js] * c->ovwmol[js] / ctd;
+          c2 = c2 + term;
+          if (js == s) self = term;
+        }
+        double c1 = c1t - yt[s];
+        c2 = c2 - self;
+        c2 = c2 * wmomix;
+        fld(pr + prim_rd(P, s), bo) = d * c1 / c2;
+      }
     }
-    // Wilke viscosity (src/rhscal.f:3854-3891)
+    SG_PHASE();
+    // Wilke viscosity (src/rhscal.f:3854-3891).  sqrt(mu_s / mu_j) is formed
+    // as sqrt(mu_s) / sqrt(mu_j), one square root and reciprocal per species.
+    double mus[NS], sq[NS], rsq[NS], yw[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      double f = c->viscco[s][c->ncovis - 1];
-      for (int m = c->ncovis - 2; m >= 0; m--) f = f * transp + c->viscco[s][m];
-      ctr[s] = exp(f);
+      mus[s] = exp(trfit(c->viscco[s], transp));
+      sq[s] = sqrt(mus[s]);
+      rsq[s] = 1.0 / sq[s];
+      yw[s] = y[s] * c->ovwmol[s];
     }
     double c1 = 0.0;
 #pragma unroll
@@ -492,15 +561,15 @@ __global__ void __launch_bounds__(BLK) k_prim(Params P, const sg_config *__restr
       double c2 = 0.0;
 #pragma unroll
       for (int js = 0; js < NS; js++) {
-        double fornow = sqrt(ctr[s] / ctr[js]);
+        double fornow = js == s ? 1.0 : sq[s] * rsq[js];
         fornow = 1.0 + fornow * c->wilko2[s][js];
         fornow = c->wilko1[s][js] * fornow * fornow;
-        c2 = c2 + y[js] * c->ovwmol[js] * fornow;
+        c2 = c2 + yw[js] * fornow;
       }
-      double fornow = ctr[s] / c2;
-      c1 = c1 + y[s] * c->ovwmol[s] * fornow;
+      double fornow = mus[s] / c2;
+      c1 = c1 + yw[s] * fornow;
     }
-    pr[P_MU * fs + idx] = c1;
+    fld(pr + P_MU * fs, bo) = c1;
   }
   // face data for NSCBC: cp and mixture gas constant (src/rhscal.f:72-131)
   if (P.inflow_xl || P.outflow_xr) {
@@ -1030,7 +1099,7 @@ void launch_bountt_t(const Params &P, const sg_config *dcfg, const Fields &F, hi
 }
 template <int NS>
 void launch_prim_t(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st) {
-  hipLaunchKernelGGL(k_prim<NS>, dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F);
+  hipLaunchKernelGGL(k_prim<NS>, dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F.tab, F);
 }
 template <int DIR>
 static dim3 pass_grid(const Params &P) {
