@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "sg_internal.h"
+#include "sg_stencil.h"
 
 using namespace sg;
 
@@ -252,6 +253,7 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   int rc = setup_params(h);
   if (!rc && h->P.fstride * (long)sizeof(double) >= (1L << 32)) rc = SG_EINVAL;  // 32-bit field offsets
   if (rc) { delete h; return rc; }
+  for (int r = 0; r < SG_NROW; r++) row_weights(h->P, r, h->tab.roww[r]);
   if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return SG_EDEVICE;

@@ -38,7 +38,9 @@ struct Params {
 // bit-identical), with the enthalpy and entropy constants pulled out, so that
 // kernels evaluate them with fixed trip counts and uniform loads.
 #define SG_NCPMAX 5   // highest cp order the temperature solve accepts (src/temper.f NCTMAX)
+#define SG_NROW 11    // derivative row kinds: interior, 5 left and 5 right closure rows
 struct Tables {
+  double roww[SG_NROW][10];                      // 5-term row weights W1, W2 per row kind
   double cp[SG_MAX_SPEC][SG_NTINT][SG_NCPMAX];   // amascp polynomial part
   double ct[SG_MAX_SPEC][SG_NTINT][SG_NCPMAX];   // amasct
   double ch[SG_MAX_SPEC][SG_NTINT][SG_NCPMAX];   // amasch

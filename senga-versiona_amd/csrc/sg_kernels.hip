@@ -594,8 +594,8 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
 // p - p_ref, mu, h_mix and per species Y, Y*rho u_d, rho D, h.
 template <int DIR>
 struct PassTile {
-  static constexpr int TX = DIR == 0 ? 64 : 32;
-  static constexpr int TY = DIR == 0 ? 4 : (DIR == 1 ? 8 : 1);
+  static constexpr int TX = 32;
+  static constexpr int TY = DIR == 2 ? 1 : 8;
   static constexpr int TZ = DIR == 2 ? 8 : 1;
   static constexpr int LX = TX + (DIR == 0 ? 10 : 0);
   static constexpr int LY = TY + (DIR == 1 ? 10 : 0);
@@ -605,7 +605,7 @@ struct PassTile {
 };
 
 template <int DIR, bool BND, int NS>
-__global__ void __launch_bounds__(BLK) k_pass(Params P, Fields F) {
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_pass(Params P, Fields F) {
   using T_ = PassTile<DIR>;
   constexpr int TX = T_::TX, TY = T_::TY, LX = T_::LX, LY = T_::LY, LN = T_::LN, LSTR = T_::LSTR;
   static_assert(T_::TX * T_::TY * T_::TZ == BLK, "tile must match the block");
@@ -626,7 +626,21 @@ __global__ void __launch_bounds__(BLK) k_pass(Params P, Fields F) {
   S.mode = 0;
   if (BND) {
     if (m < 5) S.mode = 1 + m;
-    else if (m > n - 6) S.mode = 11 + (n - 1 - m);
+    else if (m > n - 6 && m < n) S.mode = 11 + (n - 1 - m);
+  }
+  // Bounded direction: waves holding any closure row evaluate all their
+  // stencils in the uniform 5-term row form (weights from an LDS table,
+  // offsets packed per lane); other waves take the interior stencil.
+  __shared__ double wtab[BND ? SG_NROW : 1][10];
+  int pk[5] = {0, 0, 0, 0, 0};
+  int row = 0;
+  bool edge = false;
+  if (BND) {
+    if (t < SG_NROW * 10) (&wtab[0][0])[t] = (&F.tab->roww[0][0])[t];
+    row = row_of_mode(S.mode);
+#pragma unroll
+    for (int q = 0; q < 5; q++) pk[q] = c_row_offsets.w[row][q];
+    edge = __ballot(S.mode != 0) != 0;
   }
   // LDS position of this point
   const int lbase = (lz + (DIR == 2 ? 5 : 0)) * LX * LY + (ly + (DIR == 1 ? 5 : 0)) * LX + lx + (DIR == 0 ? 5 : 0);
@@ -666,8 +680,16 @@ __global__ void __launch_bounds__(BLK) k_pass(Params P, Fields F) {
   };
   auto at = [&](int kk) { return tile[buf][lbase + (kk - 8) * LSTR]; };
   auto ctr = [&]() { return tile[buf][lbase]; };
-  auto D1 = [&]() { return d1<BND>(P, S, DIR, at); };
-  auto D2 = [&]() { return d2<BND>(P, S, DIR, at); };
+  auto fq = [&](int q) { return tile[buf][lbase + q * LSTR]; };
+  auto wq = [&](int q) { return wtab[BND ? row : 0][q]; };
+  auto D1 = [&]() {
+    if (BND && edge) return d1_rows(P, pk, DIR, wq, fq);
+    return d1<false>(P, S, DIR, at);
+  };
+  auto D2 = [&]() {
+    if (BND && edge) return d2_rows(P, pk, DIR, wq, fq);
+    return d2<false>(P, S, DIR, at);
+  };
   auto next = [&]() { buf ^= 1; };
   const long fs = P.fstride;
   const double *Q = F.q;

@@ -142,6 +142,88 @@ __device__ __forceinline__ double d2(const Params &P, const Sten &S, int dir, Fn
 
 #undef FQ
 
+// ---------------------------------------------------------------------------
+// Every first/second-derivative row above written as one 5-term form,
+//   d1 = sum_k W1[k] * (f(P1[k]) - f(M1[k]))
+//   d2 = sum_k W2[k] * ((f(P2[k]) - f(0)) - (f(0) - f(M2[k])))
+// summed left to right.  Rows with fewer terms carry W = 0 and P = M = 0,
+// which add an exact zero; one-sided second-derivative rows use M2 = 0, whose
+// (f(0) - f(0)) is an exact zero.  The result is bit-identical to d1/d2, and
+// lanes on different rows run the same instructions, so a wave that mixes
+// closure rows with interior points does not serialise over row kinds.
+// Row index: 0 interior, 1..5 left rows m = 0..4, 6..10 right rows n-1-m = 0..4
+// (mode 1..5 -> row 1..5, mode 11..15 -> row 6..10).
+
+__device__ __forceinline__ int row_of_mode(int mode) { return mode == 0 ? 0 : (mode < 10 ? mode : mode - 5); }
+
+// Offsets (P1, M1, P2, M2: 4 x 5 values in -5..5) of every row, packed as
+// signed bytes, four to an int: word w holds entries 4w..4w+3 of the 20.
+struct RowOffsets {
+  int w[SG_NROW][5];
+};
+constexpr RowOffsets make_row_offsets() {
+  RowOffsets r{};
+  for (int row = 0; row < SG_NROW; row++) {
+    int o[20] = {};
+    auto set = [&](int base, int a, int b, int c, int d, int e) {
+      o[base] = a; o[base + 1] = b; o[base + 2] = c; o[base + 3] = d; o[base + 4] = e;
+    };
+    switch (row) {
+      case 0: set(0, 1, 2, 3, 4, 5); set(5, -1, -2, -3, -4, -5); set(10, 1, 2, 3, 4, 5); set(15, -1, -2, -3, -4, -5); break;
+      case 1: set(0, 1, 2, 3, 4, 0); set(10, 1, 2, 3, 4, 5); break;
+      case 2: set(0, -1, 1, 2, 3, 0); set(10, -1, 1, 2, 3, 4); break;
+      case 3: case 8: set(0, 1, 2, 0, 0, 0); set(5, -1, -2, 0, 0, 0); set(10, 1, 2, 0, 0, 0); set(15, -1, -2, 0, 0, 0); break;
+      case 4: case 9: set(0, 1, 2, 3, 0, 0); set(5, -1, -2, -3, 0, 0); set(10, 1, 2, 3, 0, 0); set(15, -1, -2, -3, 0, 0); break;
+      case 5: case 10: set(0, 1, 2, 3, 4, 0); set(5, -1, -2, -3, -4, 0); set(10, 1, 2, 3, 4, 0); set(15, -1, -2, -3, -4, 0); break;
+      case 7: set(5, 1, -1, -2, -3, 0); set(10, 1, -1, -2, -3, -4); break;
+      default: set(5, -1, -2, -3, -4, 0); set(10, -1, -2, -3, -4, -5); break;   // 6: last point
+    }
+    for (int w = 0; w < 5; w++)
+      r.w[row][w] = (int)((unsigned)(o[4 * w] & 255) | ((unsigned)(o[4 * w + 1] & 255) << 8) |
+                          ((unsigned)(o[4 * w + 2] & 255) << 16) | ((unsigned)(o[4 * w + 3] & 255) << 24));
+  }
+  return r;
+}
+__constant__ static const RowOffsets c_row_offsets = make_row_offsets();
+
+// Weights (W1 then W2) of a row, written to w[0..9].  Host side: the table
+// is built once per handle (a per-lane switch filling it on the device was
+// miscompiled for gfx950 at -O3, storing a wrong weight in rows 6 and 8).
+inline void row_weights(const Params &P, int row, double *w) {
+  for (int k = 0; k < 10; k++) w[k] = 0.0;
+  switch (row) {
+    case 0: for (int k = 0; k < 5; k++) { w[k] = P.ac[k]; w[5 + k] = P.sc[k]; } break;
+    case 1: case 6: for (int k = 0; k < 4; k++) w[k] = P.c1[k]; for (int k = 0; k < 5; k++) w[5 + k] = P.s1[k]; break;
+    case 2: case 7: for (int k = 0; k < 4; k++) w[k] = P.c2[k]; for (int k = 0; k < 5; k++) w[5 + k] = P.s2[k]; break;
+    case 3: case 8: for (int k = 0; k < 2; k++) { w[k] = P.c3[k]; w[5 + k] = P.s3[k]; } break;
+    case 4: case 9: for (int k = 0; k < 3; k++) { w[k] = P.c4[k]; w[5 + k] = P.s4[k]; } break;
+    default: for (int k = 0; k < 4; k++) { w[k] = P.c5[k]; w[5 + k] = P.s5[k]; } break;
+  }
+}
+
+// Offset entry e (0..19) from the packed words.
+__device__ __forceinline__ int row_off(const int packed[5], int e) {
+  return __builtin_amdgcn_sbfe(packed[e >> 2], 8 * (e & 3), 8);
+}
+
+// First/second derivative in the 5-term form; f(q) gives the value at offset
+// q, w the row's 10 weights.
+template <class Fn, class Wn>
+__device__ __forceinline__ double d1_rows(const Params &P, const int packed[5], int dir, Wn w, Fn f) {
+  double r = w(0) * (f(row_off(packed, 0)) - f(row_off(packed, 5)));
+#pragma unroll
+  for (int k = 1; k < 5; k++) r = r + w(k) * (f(row_off(packed, k)) - f(row_off(packed, 5 + k)));
+  return r * P.ovdel[dir];
+}
+template <class Fn, class Wn>
+__device__ __forceinline__ double d2_rows(const Params &P, const int packed[5], int dir, Wn w, Fn f) {
+  const double c = f(0);
+  double r = w(5) * ((f(row_off(packed, 10)) - c) - (c - f(row_off(packed, 15))));
+#pragma unroll
+  for (int k = 1; k < 5; k++) r = r + w(5 + k) * ((f(row_off(packed, 10 + k)) - c) - (c - f(row_off(packed, 15 + k))));
+  return r * P.ovdl2[dir];
+}
+
 // Mixed derivative in the (d1,d2) plane; d2 periodic, d1 bounded if BND.
 // g(k1, k2) returns the value at offsets q1 = k1 - 8 (along d1), q2 = k2 - 8.
 #define G(a, b) g((a) + 8, (b) + 8)
