@@ -87,6 +87,19 @@ static void build_tables(sg_config &c, Tables &t) {
         for (int m = c.ncodif; m < SG_NTRC; m++) c.diffco[s][j][m] = 0.0;
     }
   }
+  t.ratedense = 1;
+  for (int st = 0; st < c.nstep; st++) {
+    bool seen[SG_MAX_SPEC] = {};
+    for (int m = 0; m < c.nsslen[st]; m++) {
+      const int k = c.nsspec[st][m];
+      if (k < 0 || k >= c.nspec || seen[k]) {
+        t.ratedense = 0;
+      } else {
+        seen[k] = true;
+        t.nuw[st][k] = c.diffmw[st][m];
+      }
+    }
+  }
   t.diffsym = 1;
   for (int i = 0; i < c.nspec; i++)
     for (int j = i + 1; j < c.nspec; j++)

@@ -49,7 +49,11 @@ struct Tables {
   double hch[SG_MAX_SPEC][SG_NTINT];             // amasch(ncenth)
   double gbh[SG_MAX_SPEC][SG_NTINT];             // amolgb(ncenth)
   double gbs[SG_MAX_SPEC][SG_NTINT];             // amolgb(ncenpy)
+  // dense per-step rate coefficients: nuw[st][k] = diffmw of species k in
+  // step st's list (0 if absent); valid when no species repeats in a list
+  double nuw[SG_MAX_STEP][SG_MAX_SPEC];
   int diffsym;                                   // binary diffusion fits symmetric in (i, j)
+  int ratedense;                                 // nuw is usable
 };
 
 // Device field set: all arrays are [field][z][y][x] with field stride fstride.

@@ -362,13 +362,18 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
       double g = horner(tb->gb[s], it, t);
       gibbs[s] = pick1(tb->gbh[s], it) / t - pick1(tb->gbs[s], it) * lt - g;
     }
-    auto sel = [&](const double *v, int s) {
-      double r = 0.0;
+    // Molar concentrations and Gibbs functions of this point in LDS, so that
+    // a step's species lists (uniform, known only at run time) index them
+    // with one LDS read instead of a select over all species.
+    __shared__ double lcy[NS][BLK], lgb[NS][BLK];
+    const int tl = threadIdx.x;
 #pragma unroll
-      for (int u = 0; u < NS; u++)
-        if (u == s) r = v[u];
-      return r;
-    };
+    for (int s = 0; s < NS; s++) {
+      lcy[s][tl] = ry[s] * c->ovwmol[s];
+      lgb[s][tl] = gibbs[s];
+    }
+    auto cyv = [&](int s) { return lcy[s][tl]; };
+    auto gbv = [&](int s) { return lgb[s][tl]; };
     for (int st = 0; st < c->nstep; st++) {
       bool flthrd = true;
       double s3 = 0.0;
@@ -431,27 +436,27 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
         flthrd = false;
       }
       for (int m = 0; m < c->nrslen[st]; m++) {
-        double fornow = sel(ry, c->nrspec[st][m]) * c->ovwmol[c->nrspec[st][m]];
+        double fornow = cyv(c->nrspec[st][m]);
         fornow = fornow > 0.0 ? fornow : 0.0;
         s1 = s1 * fornow;
       }
       for (int m = 0; m < c->nrclen[st]; m++) {
-        double fornow = sel(ry, c->nrcpec[st][m]) * c->ovwmol[c->nrcpec[st][m]];
+        double fornow = cyv(c->nrcpec[st][m]);
         fornow = fornow > ysmall ? fornow : ysmall;
         fornow = exp(c->crspec[st][m] * log(fornow));
         fornow = fornow / (1.0 + ydenom * fornow);
         s1 = s1 * fornow;
       }
       if (c->mglist[st] != 0) {
-        for (int m = 0; m < c->nsslen[st]; m++) s2 = s2 + c->diffmu[st][m] * sel(gibbs, c->nsspec[st][m]);
+        for (int m = 0; m < c->nsslen[st]; m++) s2 = s2 + c->diffmu[st][m] * gbv(c->nsspec[st][m]);
         s2 = exp(s2);
         for (int m = 0; m < c->npslen[st]; m++) {
-          double fornow = sel(ry, c->npspec[st][m]) * c->ovwmol[c->npspec[st][m]];
+          double fornow = cyv(c->npspec[st][m]);
           fornow = fornow > 0.0 ? fornow : 0.0;
           s2 = s2 * fornow;
         }
         for (int m = 0; m < c->npclen[st]; m++) {
-          double fornow = sel(ry, c->npcpec[st][m]) * c->ovwmol[c->npcpec[st][m]];
+          double fornow = cyv(c->npcpec[st][m]);
           fornow = fornow > ysmall ? fornow : ysmall;
           fornow = exp(c->cpspec[st][m] * log(fornow));
           fornow = fornow / (1.0 + ydenom * fornow);
@@ -460,12 +465,18 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
         s1 = s1 - s2;
       }
       if (mb != 0 && flthrd) s1 = s1 * s3;
-      for (int m = 0; m < c->nsslen[st]; m++) {
-        int s = c->nsspec[st][m];
-        double add = s1 * c->diffmw[st][m];
+      if (tb->ratedense) {
+        // every species once per list: absent species add s1*0, an exact zero
 #pragma unroll
-        for (int u = 0; u < NS; u++)
-          if (u == s) rate[u] = rate[u] + add;
+        for (int u = 0; u < NS; u++) rate[u] = rate[u] + s1 * tb->nuw[st][u];
+      } else {
+        for (int m = 0; m < c->nsslen[st]; m++) {
+          int s = c->nsspec[st][m];
+          double add = s1 * c->diffmw[st][m];
+#pragma unroll
+          for (int u = 0; u < NS; u++)
+            if (u == s) rate[u] = rate[u] + add;
+        }
       }
     }
 #pragma unroll
@@ -498,6 +509,13 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
     double c1t = 0.0;
 #pragma unroll
     for (int js = 0; js < NS; js++) c1t = c1t + yt[js];
+    // The reference's term (Y_j + tol) W_j^-1 / CTRANS, CTRANS = exp(f) p_g / p,
+    // is formed as (Y_j + tol) W_j^-1 * exp(-f) * (p / p_g): no division per
+    // pair (a few ulp from the reference's operation order).
+    const double ppd = pres / P.pdifgb;
+    double yw2[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) yw2[s] = yt[s] * c->ovwmol[s];
     if (tb->diffsym) {
       // D_ij = D_ji: one fit per unordered pair, its term added to both
       // sums.  Visiting pairs (a, b >= a) row by row delivers the terms of
@@ -509,13 +527,13 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
       for (int a = 0; a < NS; a++) {
 #pragma unroll
         for (int b = a; b < NS; b++) {
-          const double ctd = exp(trfit(c->diffco[a][b], transp)) * P.pdifgb / pres;
-          const double tab = yt[b] * c->ovwmol[b] / ctd;
+          const double ictd = exp(-trfit(c->diffco[a][b], transp)) * ppd;
+          const double tab = yw2[b] * ictd;
           cs[a] = cs[a] + tab;
           if (b == a) {
             self[a] = tab;
           } else {
-            const double tba = yt[a] * c->ovwmol[a] / ctd;
+            const double tba = yw2[a] * ictd;
             cs[b] = cs[b] + tba;
           }
         }
@@ -533,8 +551,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
         double c2 = 0.0, self = 0.0;
 #pragma unroll
         for (int js = 0; js < NS; js++) {
-          const double ctd = exp(trfit(c->diffco[s][js], transp)) * P.pdifgb / pres;
-          const double term = yt[js] * c->ovwmol[js] / ctd;
+          const double term = yw2[js] * (exp(-trfit(c->diffco[s][js], transp)) * ppd);
           c2 = c2 + term;
           if (js == s) self = term;
         }
@@ -594,9 +611,12 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
 // p - p_ref, mu, h_mix and per species Y, Y*rho u_d, rho D, h.
 template <int DIR>
 struct PassTile {
+  // x: 32x8 points (halo 42/32 along x).  y, z: 32x16 points, 512 threads,
+  // so that the 10-point halo along the pass direction costs 26/16 re-reads
   static constexpr int TX = 32;
-  static constexpr int TY = DIR == 2 ? 1 : 8;
-  static constexpr int TZ = DIR == 2 ? 8 : 1;
+  static constexpr int TY = DIR == 0 ? 8 : (DIR == 1 ? 16 : 1);
+  static constexpr int TZ = DIR == 2 ? 16 : 1;
+  static constexpr int NT = TX * TY * TZ;   // threads per block
   static constexpr int LX = TX + (DIR == 0 ? 10 : 0);
   static constexpr int LY = TY + (DIR == 1 ? 10 : 0);
   static constexpr int LZ = TZ + (DIR == 2 ? 10 : 0);
@@ -605,14 +625,14 @@ struct PassTile {
 };
 
 template <int DIR, bool BND, int NS>
-__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_pass(Params P, Fields F) {
+__global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves_per_eu(4))) k_pass(Params P, Fields F) {
   using T_ = PassTile<DIR>;
   constexpr int TX = T_::TX, TY = T_::TY, LX = T_::LX, LY = T_::LY, LN = T_::LN, LSTR = T_::LSTR;
-  static_assert(T_::TX * T_::TY * T_::TZ == BLK, "tile must match the block");
+  constexpr int NTH = T_::NT;
   // LN staged values per field, padded to whole block-sized rows so that
   // every thread loads and stores unconditionally
-  constexpr int NIT = (LN + BLK - 1) / BLK;
-  __shared__ double tile[2][NIT * BLK];
+  constexpr int NIT = (LN + NTH - 1) / NTH;
+  __shared__ double tile[2][NIT * NTH];
   const int t = threadIdx.x;
   const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * T_::TZ;
@@ -670,12 +690,12 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
     double v[NIT];
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
-      const int l = t + it * BLK;
+      const int l = t + it * NTH;
       v[it] = fn(gsrc(l < LN ? l : LN - 1));
     }
     double *tb = tile[buf];
 #pragma unroll
-    for (int it = 0; it < NIT; it++) tb[t + it * BLK] = v[it];
+    for (int it = 0; it < NIT; it++) tb[t + it * NTH] = v[it];
     __syncthreads();
   };
   auto at = [&](int kk) { return tile[buf][lbase + (kk - 8) * LSTR]; };
@@ -1131,14 +1151,14 @@ static dim3 pass_grid(const Params &P) {
 
 template <int NS>
 void launch_pass_t(int dir, const Params &P, const Fields &F, hipStream_t st) {
-  dim3 b(BLK);
   if (dir == 0) {
+    dim3 b(PassTile<0>::NT);
     if (P.bnd[0]) hipLaunchKernelGGL((k_pass<0, true, NS>), pass_grid<0>(P), b, 0, st, P, F);
     else hipLaunchKernelGGL((k_pass<0, false, NS>), pass_grid<0>(P), b, 0, st, P, F);
   } else if (dir == 1) {
-    hipLaunchKernelGGL((k_pass<1, false, NS>), pass_grid<1>(P), b, 0, st, P, F);
+    hipLaunchKernelGGL((k_pass<1, false, NS>), pass_grid<1>(P), dim3(PassTile<1>::NT), 0, st, P, F);
   } else {
-    hipLaunchKernelGGL((k_pass<2, false, NS>), pass_grid<2>(P), b, 0, st, P, F);
+    hipLaunchKernelGGL((k_pass<2, false, NS>), pass_grid<2>(P), dim3(PassTile<2>::NT), 0, st, P, F);
   }
 }
 template <int NS>

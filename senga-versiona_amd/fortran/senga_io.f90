@@ -1,0 +1,500 @@
+! senga_io.f90 -- the SENGA+ input and dump formats for the GPU driver.
+!
+! Readers for input/cont.dat (layout of src/contin.f:83-214), input/chem.dat
+! (src/chemin.f:155-344) and input/diff.dat (src/diffin.f:95-212), the derived
+! thermochemical tables of src/chemin.f:822-983 and the transport tables of
+! src/diffin.f:280-405, written into the C-ABI configuration sg_config_t; and
+! the restart dump dmpi in its unformatted and formatted forms
+! (src/output.f:306-333, src/indata.f:1430-1475).  The edit descriptors are
+! those of the file formats, so files written for the reference read here
+! unchanged.
+module senga_io
+  use, intrinsic :: iso_c_binding
+  use senga_gpu
+  implicit none
+  private
+
+  ! physical constants of src/com_senga2.h:112-119
+  real(c_double), parameter :: RGUNIV = 8.3142d3
+  real(c_double), parameter :: ALAMDC = 2.58d-5, RLAMDA = 0.7d0, TLAMDA = 298.0d0
+  real(c_double), parameter :: PRANTL = 0.7d0
+
+  ! run control (cont.dat)
+  type, public :: control_t
+    real(c_double) :: xgdlen, ygdlen, zgdlen
+    integer :: nx, ny, nz
+    real(c_double) :: tstep
+    integer :: ntime1, ntime, nstpsw
+    integer :: ntdump, ntrept, ntstat, ndofmt
+    integer :: ncdmpi, ndifmt
+    real(c_double) :: prin, trin, urin, vrin, wrin
+    integer :: nyrin
+    real(c_double) :: yrin(SG_MAX_SPEC)
+    integer :: bctype(6), bcipar(4, 6)
+    real(c_double) :: bcrpar(4, 6)
+    integer :: nfmavt
+  end type control_t
+
+  public :: read_cont, read_chem, read_diff, fill_config, zero_config
+  public :: read_dump, write_dump
+
+contains
+
+  subroutine zero_config(cfg)
+    type(sg_config_t), target, intent(inout) :: cfg
+    integer(c_int8_t), pointer :: b(:)
+    call c_f_pointer(c_loc(cfg), b, [c_sizeof(cfg)])
+    b = 0_c_int8_t
+  end subroutine zero_config
+
+  ! ---------------------------------------------------------------- cont.dat
+  subroutine read_cont(path, ctl, ierr)
+    character(len=*), intent(in) :: path
+    type(control_t), intent(out) :: ctl
+    integer, intent(out) :: ierr
+    integer :: u, i, j, nprx, npry, nprz, inturb, inseed, inflam
+    real(c_double) :: sparam(4)
+    character(len=64) :: line
+    ctl%yrin = 0.0d0
+    ctl%nfmavt = 0
+    open(newunit=u, file=path, status='old', form='formatted', iostat=ierr)
+    if (ierr /= 0) return
+    do i = 1, 7
+      read(u, *)
+    end do
+    read(u, *) ctl%xgdlen, ctl%ygdlen, ctl%zgdlen
+    read(u, *)
+    read(u, *)
+    read(u, *) ctl%nx, ctl%ny, ctl%nz
+    read(u, *)
+    read(u, *)
+    read(u, *) nprx, npry, nprz
+    read(u, *)
+    read(u, *)
+    read(u, *) ctl%tstep, ctl%ntime1, ctl%ntime, ctl%nstpsw
+    read(u, *)
+    read(u, *)
+    read(u, *) ctl%ntdump, ctl%ntrept, ctl%ntstat, ctl%ndofmt
+    read(u, *)
+    read(u, *)
+    read(u, *) ctl%ncdmpi, ctl%ndifmt
+    read(u, *)
+    read(u, *)
+    read(u, *)
+    read(u, *) inturb, inseed, sparam
+    read(u, *)
+    read(u, *)
+    read(u, *) inflam
+    read(u, *)
+    read(u, *)
+    read(u, *)
+    read(u, *) ctl%prin, ctl%trin, ctl%urin, ctl%vrin, ctl%wrin
+    read(u, *)
+    read(u, *) ctl%nyrin
+    if (ctl%nyrin > SG_MAX_SPEC) then
+      ierr = 1
+      close(u)
+      return
+    end if
+    do i = 1, ctl%nyrin
+      read(u, *) j, ctl%yrin(i)
+    end do
+    do i = 1, 5
+      read(u, *)
+    end do
+    do i = 1, 6
+      read(u, *) ctl%bctype(i), ctl%bcipar(:, i), ctl%bcrpar(:, i)
+    end do
+    read(u, *)
+    read(u, '(a)', iostat=ierr) line
+    if (ierr == 0 .and. line(1:32) == 'Molecular transport control data') then
+      read(u, *)
+      read(u, *) ctl%nfmavt
+    end if
+    ierr = 0
+    close(u)
+  end subroutine read_cont
+
+  ! ---------------------------------------------------------------- chem.dat
+  ! Reads the mechanism and fills the thermochemical members of cfg with the
+  ! derived tables (src/chemin.f:822-983), species lists 0-based.
+  subroutine read_chem(path, cfg, ierr)
+    character(len=*), intent(in) :: path
+    type(sg_config_t), intent(inout) :: cfg
+    integer, intent(out) :: ierr
+    integer :: u, ns, nst, i, j, k, isp, ist, n, it, nc, icp, nbody, ngibb, nlind, ntroe, nsrif
+    character(len=32) :: sym
+    real(c_double) :: prefgb, giblet, clewis(SG_MAX_SPEC), tlo, thi
+    real(c_double) :: amolcp(SG_NCOF, SG_NTINT, SG_MAX_SPEC), diffmu_r
+    integer :: ncofcp(SG_NTINT, SG_MAX_SPEC)
+    open(newunit=u, file=path, status='old', form='formatted', iostat=ierr)
+    if (ierr /= 0) return
+    do i = 1, 6
+      read(u, *)
+    end do
+    read(u, '(I5)') ns
+    if (ns > SG_MAX_SPEC) then
+      ierr = 1
+      close(u)
+      return
+    end if
+    cfg%nspec = ns
+    do i = 1, ns
+      read(u, '(I5,3X,A)') isp, sym
+    end do
+    read(u, *)
+    read(u, *) prefgb
+    amolcp = 0.0d0
+    ncofcp = 0
+    do i = 1, ns
+      read(u, '(I5,1PE12.4)') isp, tlo
+      cfg%wmolar(isp) = tlo
+      read(u, '(1PE12.4)') clewis(isp)
+      read(u, '(I5)') cfg%ntint(isp)
+      do it = 1, cfg%ntint(isp)
+        read(u, '(2(1PE12.4),I5)') tlo, thi, ncofcp(it, isp)
+        cfg%tinthi(it, isp) = thi
+        do icp = 1, ncofcp(it, isp)
+          read(u, '(1PE15.7)') amolcp(icp, it, isp)
+        end do
+      end do
+    end do
+    read(u, *)
+    read(u, '(I5)') nst
+    if (nst > SG_MAX_STEP) then
+      ierr = 1
+      close(u)
+      return
+    end if
+    cfg%nstep = nst
+    do j = 1, nst
+      read(u, '(I5,3(1PE12.4))') ist, tlo, thi, diffmu_r
+      cfg%rparam(:, ist) = [tlo, thi, diffmu_r]
+    end do
+    ! species, reactant and product lists
+    read(u, *)
+    do j = 1, nst
+      read(u, '(2I5)') ist, cfg%nsslen(ist)
+      do k = 1, cfg%nsslen(ist)
+        read(u, '(2I5,I8)') ist, i, n
+        cfg%nsspec(i, ist) = n - 1
+      end do
+    end do
+    read(u, *)
+    do j = 1, nst
+      read(u, '(2I5)') ist, cfg%nrslen(ist)
+      do k = 1, cfg%nrslen(ist)
+        read(u, '(2I5,I8)') ist, i, n
+        cfg%nrspec(i, ist) = n - 1
+      end do
+    end do
+    read(u, *)
+    do j = 1, nst
+      read(u, '(2I5)') ist, cfg%npslen(ist)
+      do k = 1, cfg%npslen(ist)
+        read(u, '(2I5,I8)') ist, i, n
+        cfg%npspec(i, ist) = n - 1
+      end do
+    end do
+    read(u, *)
+    do j = 1, nst
+      read(u, '(2I5)') ist, cfg%nrclen(ist)
+      do k = 1, cfg%nrclen(ist)
+        read(u, '(2I5,I8,1PE12.4)') ist, i, n, cfg%crspec(i, ist)
+        cfg%nrcpec(i, ist) = n - 1
+      end do
+    end do
+    read(u, *)
+    do j = 1, nst
+      read(u, '(2I5)') ist, cfg%npclen(ist)
+      do k = 1, cfg%npclen(ist)
+        read(u, '(2I5,I8,1PE12.4)') ist, i, n, cfg%cpspec(i, ist)
+        cfg%npcpec(i, ist) = n - 1
+      end do
+    end do
+    read(u, *)
+    do j = 1, nst
+      do k = 1, cfg%nsslen(j)
+        read(u, '(2I5,F5.1)') ist, i, diffmu_r
+        cfg%diffmu(i, ist) = diffmu_r
+      end do
+    end do
+    ! third bodies
+    read(u, *)
+    read(u, '(I5)') nbody
+    do i = 1, nbody
+      read(u, '(I5,3X,A)') n, sym
+    end do
+    read(u, *)
+    if (nbody > 0) then
+      do j = 1, nst
+        read(u, '(I5,I8)') ist, cfg%mblist(ist)
+      end do
+    end if
+    read(u, *)
+    do k = 1, nbody
+      do i = 1, ns
+        read(u, '(2I5,1PE12.4)') n, isp, cfg%effy3b(isp, n)
+      end do
+    end do
+    ! Gibbs, Lindemann, Troe, SRI step lists and data
+    read(u, *)
+    read(u, '(I5)') ngibb
+    if (ngibb > 0) then
+      do j = 1, nst
+        read(u, '(I5,I8)') ist, cfg%mglist(ist)
+      end do
+    end if
+    read(u, *)
+    read(u, '(I5)') nlind
+    if (nlind > 0) then
+      do j = 1, nst
+        read(u, '(I5,I8)') ist, cfg%mllist(ist)
+      end do
+    end if
+    read(u, *)
+    do k = 1, nlind
+      read(u, '(I5,4(1PE12.4))') n, cfg%rclind(:, n)
+    end do
+    read(u, *)
+    read(u, '(I5)') ntroe
+    if (ntroe > 0) then
+      do j = 1, nst
+        read(u, '(I5,I8)') ist, cfg%mtlist(ist)
+      end do
+    end if
+    read(u, *)
+    do k = 1, ntroe
+      read(u, '(I5,6(1PE12.4))') n, cfg%rctroe(1:6, n)
+      read(u, '(I5,6(1PE12.4))') n, cfg%rctroe(7:12, n)
+    end do
+    read(u, *)
+    read(u, '(I5)') nsrif
+    if (nsrif > 0) then
+      do j = 1, nst
+        read(u, '(I5,I8)') ist, cfg%mslist(ist)
+      end do
+    end if
+    read(u, *)
+    do k = 1, nsrif
+      read(u, '(I5,3(1PE12.4))') n, cfg%rcsrif(1:3, n)
+      read(u, '(I5,5(1PE12.4))') n, cfg%rcsrif(4:8, n)
+    end do
+    close(u)
+
+    ! ---- derived tables, src/chemin.f:822-983
+    do j = 1, nst
+      cfg%rparam(1, j) = log(cfg%rparam(1, j))
+      cfg%rparam(3, j) = cfg%rparam(3, j) / RGUNIV
+      do k = 1, cfg%nsslen(j)
+        cfg%diffmw(k, j) = cfg%diffmu(k, j) * cfg%wmolar(cfg%nsspec(k, j) + 1)
+      end do
+    end do
+    do k = 1, nlind
+      cfg%rclind(1, k) = log(cfg%rclind(1, k))
+      cfg%rclind(3, k) = cfg%rclind(3, k) / RGUNIV
+    end do
+    do k = 1, ntroe
+      cfg%rctroe(1, k) = log(cfg%rctroe(1, k))
+      cfg%rctroe(3, k) = cfg%rctroe(3, k) / RGUNIV
+      cfg%rctroe(5, k) = -1.0d0 / cfg%rctroe(5, k)
+      cfg%rctroe(7, k) = -1.0d0 / cfg%rctroe(7, k)
+    end do
+    do k = 1, nsrif
+      cfg%rcsrif(1, k) = log(cfg%rcsrif(1, k))
+      cfg%rcsrif(3, k) = cfg%rcsrif(3, k) / RGUNIV
+      cfg%rcsrif(5, k) = -cfg%rcsrif(5, k)
+      cfg%rcsrif(6, k) = -1.0d0 / cfg%rcsrif(6, k)
+    end do
+    if (nbody == 0) cfg%mblist = 0
+    giblet = log(prefgb / RGUNIV)
+    do i = 1, ns
+      cfg%ovwmol(i) = 1.0d0 / cfg%wmolar(i)
+      cfg%rgspec(i) = RGUNIV * cfg%ovwmol(i)
+      cfg%olewis(i) = 1.0d0 / clewis(i)
+      do it = 1, cfg%ntint(i)
+        nc = ncofcp(it, i)
+        cfg%ncpoly(it, i) = nc - 2
+        cfg%ncenth(it, i) = nc - 1
+        cfg%ncenpy(it, i) = nc
+        do icp = 1, nc
+          cfg%amascp(icp, it, i) = amolcp(icp, it, i) * RGUNIV * cfg%ovwmol(i)
+        end do
+        cfg%amasct(1, it, i) = cfg%amascp(1, it, i) - cfg%rgspec(i)
+        cfg%amasch(1, it, i) = cfg%amascp(1, it, i)
+        cfg%amolgb(1, it, i) = amolcp(nc, it, i) - amolcp(1, it, i) + giblet
+        do icp = 2, nc - 2
+          cfg%amasct(icp, it, i) = cfg%amascp(icp, it, i) / real(icp, c_double)
+          cfg%amasch(icp, it, i) = cfg%amascp(icp, it, i) / real(icp, c_double)
+          cfg%amolgb(icp, it, i) = amolcp(icp, it, i) / real(icp * (icp - 1), c_double)
+        end do
+        cfg%amasct(nc - 1, it, i) = 0.0d0
+        cfg%amasct(nc, it, i) = 0.0d0
+        cfg%amasch(nc - 1, it, i) = cfg%amascp(nc - 1, it, i)
+        cfg%amasch(nc, it, i) = 0.0d0
+        cfg%amolgb(nc - 1, it, i) = amolcp(nc - 1, it, i)
+        cfg%amolgb(nc, it, i) = amolcp(1, it, i) - 1.0d0
+      end do
+    end do
+    cfg%alamda = ALAMDC * exp(-RLAMDA * log(TLAMDA))
+    cfg%rlamda = RLAMDA
+    cfg%prantl = PRANTL
+    cfg%clnten = log(1.0d1)                 ! src/indata.f:1071
+    ierr = 0
+  end subroutine read_chem
+
+  ! ---------------------------------------------------------------- diff.dat
+  ! Mixture-averaged transport fits (src/diffin.f:95-212) and the symmetric
+  ! binary-diffusion fill and Wilke factors (src/diffin.f:280-405).
+  subroutine read_diff(path, cfg, ierr)
+    character(len=*), intent(in) :: path
+    type(sg_config_t), intent(inout) :: cfg
+    integer, intent(out) :: ierr
+    integer :: u, ns, i, j, k, n, jsp, ncv, ncc, ncd
+    character(len=32) :: sym
+    real(c_double) :: fornow
+    open(newunit=u, file=path, status='old', form='formatted', iostat=ierr)
+    if (ierr /= 0) return
+    do i = 1, 6
+      read(u, *)
+    end do
+    read(u, '(I5)') ns
+    do i = 1, ns
+      read(u, '(I5,3X,A)') n, sym
+    end do
+    read(u, *)
+    read(u, *) cfg%pdifgb, cfg%tdifgb
+    read(u, *)
+    do i = 1, ns
+      read(u, *) jsp, ncv
+      if (ncv > SG_NTRC) then
+        ierr = 1
+        close(u)
+        return
+      end if
+      read(u, *) cfg%viscco(1:ncv, i)
+      if (i == 1) cfg%ncovis = ncv
+    end do
+    read(u, *)
+    do i = 1, ns
+      read(u, *) jsp, ncc
+      if (ncc > SG_NTRC) then
+        ierr = 1
+        close(u)
+        return
+      end if
+      read(u, *) cfg%condco(1:ncc, i)
+      if (i == 1) cfg%ncocon = ncc
+    end do
+    read(u, *)
+    do i = 1, ns
+      read(u, *) jsp, ncd
+      if (ncd > SG_NTRC) then
+        ierr = 1
+        close(u)
+        return
+      end if
+      do j = 1, i
+        read(u, *) k, cfg%diffco(1:ncd, j, i)
+      end do
+      if (i == 1) cfg%ncodif = ncd
+    end do
+    close(u)
+    do i = 1, ns
+      do j = i + 1, ns
+        cfg%diffco(:, j, i) = cfg%diffco(:, i, j)
+      end do
+    end do
+    do i = 1, ns
+      do j = 1, ns
+        fornow = cfg%wmolar(i) / cfg%wmolar(j)
+        cfg%wilko2(j, i) = 1.0d0 / sqrt(sqrt(fornow))
+        fornow = 8.0d0 * (1.0d0 + fornow)
+        cfg%wilko1(j, i) = 1.0d0 / sqrt(fornow)
+      end do
+    end do
+    ierr = 0
+  end subroutine read_diff
+
+  ! Grid, boundary conditions, default state and transport switch from cont.dat.
+  subroutine fill_config(ctl, cfg)
+    type(control_t), intent(in) :: ctl
+    type(sg_config_t), intent(inout) :: cfg
+    integer :: i
+    cfg%nx = ctl%nx
+    cfg%ny = ctl%ny
+    cfg%nz = ctl%nz
+    cfg%mixavg = merge(1, 0, ctl%nfmavt == 1)
+    cfg%precision = 0
+    cfg%xgdlen = ctl%xgdlen
+    cfg%ygdlen = ctl%ygdlen
+    cfg%zgdlen = ctl%zgdlen
+    do i = 1, 6
+      cfg%bc_type(i) = ctl%bctype(i)
+      cfg%bc_ipar(:, i) = ctl%bcipar(:, i)
+      cfg%bc_rpar(:, i) = ctl%bcrpar(:, i)
+    end do
+    cfg%prin = ctl%prin
+    cfg%trin = ctl%trin
+    cfg%urin = ctl%urin
+    cfg%vrin = ctl%vrin
+    cfg%wrin = ctl%wrin
+    do i = 1, cfg%nspec
+      cfg%yrin(i) = ctl%yrin(i)
+    end do
+  end subroutine fill_config
+
+  ! ---------------------------------------------------------------- dumps
+  ! run(nx*ny*nz, 5+N): DRUN, URUN, VRUN, WRUN, ERUN, YRUN(:,1..N).
+  subroutine read_dump(path, formatted, nx, ny, nz, ns, run, etime, tstep, errold, errldr, ierr)
+    character(len=*), intent(in) :: path
+    logical, intent(in) :: formatted
+    integer, intent(in) :: nx, ny, nz, ns
+    real(c_double), intent(out) :: run(nx * ny * nz, 5 + ns)
+    real(c_double), intent(out) :: etime, tstep, errold, errldr
+    integer, intent(out) :: ierr
+    integer :: u, n1, n2, n3, n4, p
+    if (formatted) then
+      open(newunit=u, file=path, status='old', form='formatted', iostat=ierr)
+      if (ierr /= 0) return
+      read(u, *) n1, n2, n3, n4
+      do p = 1, nx * ny * nz
+        read(u, *) run(p, :)
+      end do
+      read(u, *) etime, tstep, errold, errldr
+    else
+      open(newunit=u, file=path, status='old', form='unformatted', iostat=ierr)
+      if (ierr /= 0) return
+      read(u) n1, n2, n3, n4, run, etime, tstep, errold, errldr
+    end if
+    close(u)
+    ierr = 0
+    if (n1 /= nx .or. n2 /= ny .or. n3 /= nz .or. n4 /= ns) ierr = 2
+  end subroutine read_dump
+
+  subroutine write_dump(path, formatted, nx, ny, nz, ns, run, etime, tstep, errold, errldr, ierr)
+    character(len=*), intent(in) :: path
+    logical, intent(in) :: formatted
+    integer, intent(in) :: nx, ny, nz, ns
+    real(c_double), intent(in) :: run(nx * ny * nz, 5 + ns)
+    real(c_double), intent(in) :: etime, tstep, errold, errldr
+    integer, intent(out) :: ierr
+    integer :: u, p
+    if (formatted) then
+      open(newunit=u, file=path, status='replace', form='formatted', iostat=ierr)
+      if (ierr /= 0) return
+      write(u, *) nx, ny, nz, ns
+      do p = 1, nx * ny * nz
+        write(u, *) run(p, :)
+      end do
+      write(u, *) etime, tstep, errold, errldr
+    else
+      open(newunit=u, file=path, status='replace', form='unformatted', iostat=ierr)
+      if (ierr /= 0) return
+      write(u) nx, ny, nz, ns, run, etime, tstep, errold, errldr
+    end if
+    close(u)
+    ierr = 0
+  end subroutine write_dump
+
+end module senga_io
