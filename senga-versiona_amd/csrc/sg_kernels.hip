@@ -686,16 +686,22 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   // Stage one field (or product) into the current buffer.  One barrier per
   // stage: the two buffers alternate, and the barrier of stage s orders the
   // reads of stage s-1 before the writes of stage s+1 into the same buffer.
-  auto stage = [&](auto fn) {
-    double v[NIT];
+  // Software pipeline: prefetch(fn) issues the global loads of the next
+  // field into registers; stage() stores them into the current buffer and
+  // synchronises.  Each stage's loads are thus in flight while the previous
+  // stage's stencils are evaluated.
+  double pf[NIT];
+  auto prefetch = [&](auto fn) {
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
       const int l = t + it * NTH;
-      v[it] = fn(gsrc(l < LN ? l : LN - 1));
+      pf[it] = fn(gsrc(l < LN ? l : LN - 1));
     }
+  };
+  auto stage = [&]() {
     double *tb = tile[buf];
 #pragma unroll
-    for (int it = 0; it < NIT; it++) tb[t + it * NTH] = v[it];
+    for (int it = 0; it < NIT; it++) tb[t + it * NTH] = pf[it];
     __syncthreads();
   };
   auto at = [&](int kk) { return tile[buf][lbase + (kk - 8) * LSTR]; };
@@ -731,17 +737,31 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   const bool zadb_any = (i == 0 && P.fadb[0]) || (i == P.nx - 1 && P.fadb[1]) ||
                         (j == 0 && P.fadb[2]) || (j == P.ny - 1 && P.fadb[3]) ||
                         (k == 0 && P.fadb[4]) || (k == P.nz - 1 && P.fadb[5]);
+  // fluxes as the reference forms them: rho u*u, rho u*v, rho u*w, rho v*v, rho v*w, rho w*w
+  const double *fa[3], *fb[3];
+  if (DIR == 0) { fa[0] = mx; fb[0] = uu; fa[1] = mx; fb[1] = vv; fa[2] = mx; fb[2] = ww; }
+  else if (DIR == 1) { fa[0] = mx; fb[0] = vv; fa[1] = my; fb[1] = vv; fa[2] = my; fb[2] = ww; }
+  else { fa[0] = mx; fb[0] = ww; fa[1] = my; fb[1] = ww; fa[2] = mz; fb[2] = ww; }
+  auto f_flux = [&](int a) {
+    const double *A_ = fa[a], *B_ = fb[a];
+    return [A_, B_](long g) { return A_[g] * B_[g]; };
+  };
+  auto f_y = [&](int sp) { const double *Y_ = pr + prim_y(P, sp); return [Y_](long g) { return Y_[g]; }; };
   // ---------------- continuity and energy convection (src/rhscal.f:141-415)
-  stage([&](long g) { return md[g]; });
+  prefetch([&](long g) { return md[g]; });
+  stage();
+  prefetch([&](long g) { return ee[g] * md[g]; });
   double dm = D1(), md0 = ctr();
   pin(dm); pin(md0);
   next();
   double aD = -dm;
-  stage([&](long g) { return ee[g] * md[g]; });
+  stage();
+  prefetch([&](long g) { return ee[g]; });
   double dem = D1();
   pin(dem);
   next();
-  stage([&](long g) { return ee[g]; });
+  stage();
+  prefetch([&](long g) { return TT[g]; });
   const double de = D1();
   const double e0 = ctr();
   next();
@@ -749,11 +769,13 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   pin(aE);
   // ---------------- heat conduction (src/rhscal.f:427-797)
   {
-    stage([&](long g) { return TT[g]; });
+    stage();
+    prefetch([&](long g) { return la[g]; });
     double dT = D1(), d2T = D2();
     pin(dT); pin(d2T);
     next();
-    stage([&](long g) { return la[g]; });
+    stage();
+    prefetch([&](long g) { return uu[g]; });
     const double dl = D1();
     const double lam0 = ctr();
     next();
@@ -763,15 +785,18 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   }
   // ---------------- velocity derivatives
   double du[3], d2u[3], u0[3];
-  stage([&](long g) { return uu[g]; });
+  stage();
+  prefetch([&](long g) { return vv[g]; });
   du[0] = D1(); d2u[0] = D2(); u0[0] = ctr();
   pin(du[0]); pin(d2u[0]); pin(u0[0]);
   next();
-  stage([&](long g) { return vv[g]; });
+  stage();
+  prefetch([&](long g) { return ww[g]; });
   du[1] = D1(); d2u[1] = D2(); u0[1] = ctr();
   pin(du[1]); pin(d2u[1]); pin(u0[1]);
   next();
-  stage([&](long g) { return ww[g]; });
+  stage();
+  prefetch(f_flux(0));
   du[2] = D1(); d2u[2] = D2(); u0[2] = ctr();
   pin(du[2]); pin(d2u[2]); pin(u0[2]);
   next();
@@ -779,15 +804,11 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   double aM[3];
   {
     const double r0 = active ? Q[idx] : 0.0;
-    // fluxes as the reference forms them: rho u*u, rho u*v, rho u*w, rho v*v, rho v*w, rho w*w
-    const double *fa[3], *fb[3];
-    if (DIR == 0) { fa[0] = mx; fb[0] = uu; fa[1] = mx; fb[1] = vv; fa[2] = mx; fb[2] = ww; }
-    else if (DIR == 1) { fa[0] = mx; fb[0] = vv; fa[1] = my; fb[1] = vv; fa[2] = my; fb[2] = ww; }
-    else { fa[0] = mx; fb[0] = ww; fa[1] = my; fb[1] = ww; fa[2] = mz; fb[2] = ww; }
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      const double *A_ = fa[a], *B_ = fb[a];
-      stage([&](long g) { return A_[g] * B_[g]; });
+      stage();
+      if (a < 2) prefetch(f_flux(a + 1));
+      else prefetch([&](long g) { return pp[g] - pref; });
       const double flx = D1();
       next();
       if (a == DIR) aM[a] = -0.5 * (md0 * du[a] + flx + u0[a] * dm);
@@ -797,7 +818,8 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   }
   // ---------------- pressure gradient and pressure work (src/rhsvel.f:741-917)
   {
-    stage([&](long g) { return pp[g] - pref; });
+    stage();
+    prefetch([&](long g) { return mu[g]; });
     const double dp = D1();
     const double p0 = ctr() + pref;
     next();
@@ -811,7 +833,8 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
     }
   }
   // ---------------- separable viscous terms (src/rhsvel.f:1462-2305)
-  stage([&](long g) { return mu[g]; });
+  stage();
+  prefetch(f_y(0));
   double dmu = D1(), mu0 = ctr();
   pin(dmu); pin(mu0);
   next();
@@ -838,19 +861,24 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
 #pragma unroll
   for (int sp = 0; sp < NS; sp++) {
     const double *Ys = pr + prim_y(P, sp), *Rs = pr + prim_rd(P, sp), *Hs = pr + prim_h(P, sp);
-    stage([&](long g) { return Ys[g]; });
+    stage();
+    prefetch([&](long g) { return Ys[g] * md[g]; });
     double dY = D1(), d2Y = D2(), yc = ctr();
     pin(dY); pin(d2Y); pin(yc);
     next();
-    stage([&](long g) { return Ys[g] * md[g]; });
+    stage();
+    prefetch([&](long g) { return Rs[g]; });
     double dYm = D1();
     pin(dYm);
     next();
-    stage([&](long g) { return Rs[g]; });
+    stage();
+    prefetch([&](long g) { return Hs[g]; });
     double drD = D1(), rdc = ctr();
     pin(drD); pin(rdc);
     next();
-    stage([&](long g) { return Hs[g]; });
+    stage();
+    if (sp + 1 < NS) prefetch(f_y(sp + 1));
+    else prefetch([&](long g) { return hm[g]; });
     const double dh = D1();
     const double hc = ctr();
     next();
@@ -874,7 +902,7 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
     }
   }
   {
-    stage([&](long g) { return hm[g]; });
+    stage();
     const double dhm = D1();
     const double hm0 = ctr();
     aE = aE - hm0 * (zadb_any ? 0.0 : vt) - (zdifadb ? 0.0 : dhm) * ucor;
