@@ -170,8 +170,20 @@ int sg_set_timestep(sg_handle *h, double tstep);
  * iteration are reported by the next synchronising call. */
 int sg_substage(sg_handle *h, int irk);
 
-/* End of step: ETIME += TSTEP, BOUNTT (src/senga2.f:165-171). */
+/* End of step: ETIME += TSTEP, BOUNTT, and ADAPTT when the adaptive step
+ * is on (src/senga2.f:165-184). */
 int sg_end_step(sg_handle *h);
+
+/* Adaptive time step, cont.dat step switch 1 (src/dtinit.f:101-185,
+ * src/adaptt.f:60-330): with on = 1 every step accumulates the RK error
+ * register (LINCOM/FINCOM) and sg_end_step computes the global error norm,
+ * applies the PID step controller and rescales the RK coefficients.  errold
+ * and errldr are the controller history, the dump's ERROLD and ERRLDR
+ * (1e-4 = ERRTOL on a cold start).  Allocates the error register on first use. */
+int sg_set_adaptive(sg_handle *h, int on, double errold, double errldr);
+
+/* Current time step and controller history. */
+int sg_get_timestep(sg_handle *h, double *tstep, double *errold, double *errldr);
 
 /* nsteps full steps (5 substages + sg_end_step each). */
 int sg_step(sg_handle *h, int nsteps);

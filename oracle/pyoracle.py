@@ -40,6 +40,8 @@ class Oracle:
         lib.orc_get_itndex.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         lib.orc_derivatives.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.orc_destroy.argtypes = [C.c_void_p]
+        lib.orc_set_adaptive.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+        lib.orc_get_timestep.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3
         self.lib = lib
         self.case = case
         self.cfg = config_from_case(case)
@@ -65,6 +67,14 @@ class Oracle:
 
     def set_timestep(self, dt):
         self.lib.orc_set_timestep(self.h, dt)
+
+    def set_adaptive(self, on, errold=1.0e-4, errldr=1.0e-4):
+        self.lib.orc_set_adaptive(self.h, int(on), errold, errldr)
+
+    def timestep(self):
+        v = [C.c_double() for _ in range(3)]
+        self.lib.orc_get_timestep(self.h, *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
 
     def rhs(self, irk=1):
         out = np.zeros((5 + self.ns, self.nz, self.ny, self.nx))

@@ -51,6 +51,8 @@ struct orc {
   double drin, cobcxr, pinfxr;
   double rklhs[5], rkrhs[5], rkerr[5], rktim[5];
   double etime, tstep;
+  int adaptive;            /* NSTPSW = 1: ADAPTT after each step */
+  double errold, errldr;   /* controller history (ERROLD, ERRLDR) */
   /* registers */
   double *run;   /* interior, 5+ns fields */
   double *err;   /* interior, 5+ns fields */
@@ -1632,9 +1634,64 @@ int orc_substage(orc_t *o, int irk) {
   return 0;
 }
 
+/* ADAPTT, src/adaptt.f:60-330, with the tolerances, error norms and PID
+ * controller constants of src/dtinit.f:101-170. */
+static void adaptt(orc_t *o) {
+  const double errtol = 1.0e-4, errlow = 1.0e-30, trmax = 1.01, trmin = 1.0e-2, tsmax = 1.0,
+               tsmin = 1.0e-15;
+  const double ctmult = 9.0e-1, ctalph = 4.9e-1 / 3.0, ctbeta = 3.4e-1 / 3.0, ctgama = 1.0e-1 / 3.0;
+  int nv = 5 + o->ns;
+  double errmax = ZERO;
+  for (int v = 0; v < nv; v++) {
+    /* ERDNRM = 0, ERUNRM = ERVNRM = ERWNRM = 1e-6, ERENRM = 1e-2, ERYNRM = 1e-10 */
+    double nrm = v == 0 ? 0.0 : (v < 4 ? 1.0e-6 : (v == 4 ? 1.0e-2 : 1.0e-10));
+    double tot = ZERO;
+    const double *e = ERR(o, v), *u = RUN(o, v);
+    for (size_t p = 0; p < o->ni; p++) {
+      double fornow = fabs(e[p]) / (fabs(u[p]) + nrm);
+      if (fornow > tot) tot = fornow;
+    }
+    if (tot > errmax) errmax = tot;
+  }
+  if (errmax < errlow) errmax = errlow;
+  double tratio = ctmult * exp(ctalph * log(errtol / errmax) + ctbeta * log(o->errold / errtol) +
+                               ctgama * log(errtol / o->errldr));
+  o->errldr = o->errold;
+  o->errold = errmax;
+  if (tratio > trmax) tratio = trmax;
+  if (tratio < trmin) tratio = trmin;
+  double tstold = o->tstep;
+  o->tstep = o->tstep * tratio;
+  if (o->tstep > tsmax) o->tstep = tsmax;
+  if (o->tstep < tsmin) o->tstep = tsmin;
+  tratio = o->tstep / tstold;
+  for (int r = 0; r < 5; r++) {
+    o->rklhs[r] = o->rklhs[r] * tratio;
+    o->rkrhs[r] = o->rkrhs[r] * tratio;
+    o->rkerr[r] = o->rkerr[r] * tratio;
+    o->rktim[r] = o->rktim[r] * tratio;
+  }
+  memset(o->err, 0, sizeof(double) * nv * o->ni);
+}
+
+int orc_set_adaptive(orc_t *o, int on, double errold, double errldr) {
+  o->adaptive = on;
+  o->errold = errold;
+  o->errldr = errldr;
+  return 0;
+}
+
+int orc_get_timestep(orc_t *o, double *tstep, double *errold, double *errldr) {
+  if (tstep) *tstep = o->tstep;
+  if (errold) *errold = o->errold;
+  if (errldr) *errldr = o->errldr;
+  return 0;
+}
+
 int orc_end_step(orc_t *o) {
   o->etime = o->etime + o->tstep;
   bountt(o);
+  if (o->adaptive) adaptt(o);
   return 0;
 }
 

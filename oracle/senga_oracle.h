@@ -34,6 +34,10 @@ void orc_set_timestep(orc_t *o, double dt);
 int orc_rhs(orc_t *o, int irk, double *rhs);
 int orc_substage(orc_t *o, int irk);
 int orc_end_step(orc_t *o);
+/* Adaptive time step (ADAPTT, src/adaptt.f) after each step when on; errold
+ * and errldr are the controller history (the dump's ERROLD, ERRLDR). */
+int orc_set_adaptive(orc_t *o, int on, double errold, double errldr);
+int orc_get_timestep(orc_t *o, double *tstep, double *errold, double *errldr);
 int orc_step(orc_t *o, int nsteps);
 int orc_get(orc_t *o, double *run, double *temp, double *pres, double *rate);
 /* stage register (DRHS..YRHS interior) */

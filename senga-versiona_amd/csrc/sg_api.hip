@@ -35,6 +35,10 @@ struct sg_handle {
   size_t bytes;
   double etime, tstep;
   double rklhs[5], rkrhs[5], rkerr[5], rktim[5];
+  int adaptive;          // NSTPSW = 1: ADAPTT after each step
+  double errold, errldr; // controller history
+  void *ebase;           // error register allocation (adaptive only)
+  size_t ebytes;
   int have_state;
   int err_code;
   int err_ijk[3];
@@ -266,7 +270,10 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   int rc = setup_params(h);
   if (!rc && h->P.fstride * (long)sizeof(double) >= (1L << 32)) rc = SG_EINVAL;  // 32-bit field offsets
   if (rc) { delete h; return rc; }
-  for (int r = 0; r < SG_NROW; r++) row_weights(h->P, r, h->tab.roww[r]);
+  for (int r = 0; r < SG_NROW; r++) {
+    row_weights(h->P, r, h->tab.roww[r]);
+    mix_row_weights(h->P, r, h->tab.mixw[r]);
+  }
   if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return SG_EDEVICE;
@@ -323,6 +330,7 @@ int sg_destroy(sg_handle *h) {
   hipEventDestroy(h->ev[0]);
   hipEventDestroy(h->ev[1]);
   hipFree(h->base);
+  if (h->ebase) hipFree(h->ebase);
   hipStreamDestroy(h->stream);
   delete h;
   return 0;
@@ -353,6 +361,7 @@ int sg_upload_state(sg_handle *h, const double *run, double etime) {
     HIPCHK(h, hipMemcpyAsync(h->F.q + v * P.fstride, run + v * P.npts, fb, hipMemcpyHostToDevice, h->stream));
   }
   HIPCHK(h, hipMemsetAsync(h->F.rate, 0, sizeof(double) * P.fstride * P.ns, h->stream));
+  if (h->F.erk) launch_zero(h->F.erk, (long)(5 + P.ns) * P.fstride, h->stream);
   int rc = timed(h, K_TEMPIN, [&] { launch_tempin(P, h->dcfg, h->F, h->stream); });
   if (rc) return rc;
   rc = check_device_error(h);
@@ -424,14 +433,81 @@ int sg_substage(sg_handle *h, int irk) {
   int rc = eval_rhs(h);
   if (rc) return rc;
   int mode = irk < 5 ? 0 : 1;
-  double lhs = h->rklhs[irk - 1], rhs = h->rkrhs[irk - 1];
-  return timed(h, K_FINAL, [&] { launch_final(h->P, h->dcfg, h->F, mode, lhs, rhs, nullptr, h->stream); });
+  double lhs = h->rklhs[irk - 1], rhs = h->rkrhs[irk - 1], erc = h->rkerr[irk - 1];
+  return timed(h, K_FINAL, [&] { launch_final(h->P, h->dcfg, h->F, mode, lhs, rhs, erc, nullptr, h->stream); });
+}
+
+// ADAPTT (src/adaptt.f:60-330) with the constants of src/dtinit.f:101-170:
+// global error norm on the device, PID controller on the host, RK
+// coefficients rescaled by the step ratio, error register cleared.
+static int adaptt(sg_handle *h) {
+  const double errtol = 1.0e-4, errlow = 1.0e-30, trmax = 1.01, trmin = 1.0e-2, tsmax = 1.0,
+               tsmin = 1.0e-15;
+  const double ctmult = 9.0e-1, ctalph = 4.9e-1 / 3.0, ctbeta = 3.4e-1 / 3.0, ctgama = 1.0e-1 / 3.0;
+  launch_errmax(h->P, h->F, h->stream);
+  unsigned long long bits = 0;
+  HIPCHK(h, hipMemcpyAsync(&bits, h->F.emax, sizeof bits, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double errmax;
+  memcpy(&errmax, &bits, sizeof errmax);
+  if (errmax < errlow) errmax = errlow;
+  double tratio = ctmult * exp(ctalph * log(errtol / errmax) + ctbeta * log(h->errold / errtol) +
+                               ctgama * log(errtol / h->errldr));
+  h->errldr = h->errold;
+  h->errold = errmax;
+  if (tratio > trmax) tratio = trmax;
+  if (tratio < trmin) tratio = trmin;
+  const double tstold = h->tstep;
+  h->tstep = h->tstep * tratio;
+  if (h->tstep > tsmax) h->tstep = tsmax;
+  if (h->tstep < tsmin) h->tstep = tsmin;
+  tratio = h->tstep / tstold;
+  for (int r = 0; r < 5; r++) {
+    h->rklhs[r] = h->rklhs[r] * tratio;
+    h->rkrhs[r] = h->rkrhs[r] * tratio;
+    h->rkerr[r] = h->rkerr[r] * tratio;
+    h->rktim[r] = h->rktim[r] * tratio;
+  }
+  launch_zero(h->F.erk, (long)(5 + h->P.ns) * h->P.fstride, h->stream);
+  return check_device_error(h);
 }
 
 int sg_end_step(sg_handle *h) {
   if (!h) return SG_EINVAL;
   h->etime = h->etime + h->tstep;
-  if (h->P.inflow_xl) return timed(h, K_BOUNTT, [&] { launch_bountt(h->P, h->dcfg, h->F, h->stream); });
+  int rc = 0;
+  if (h->P.inflow_xl) rc = timed(h, K_BOUNTT, [&] { launch_bountt(h->P, h->dcfg, h->F, h->stream); });
+  if (!rc && h->adaptive) rc = adaptt(h);
+  return rc;
+}
+
+int sg_set_adaptive(sg_handle *h, int on, double errold, double errldr) {
+  if (!h || (on && !(errold > 0.0 && errldr > 0.0))) return SG_EINVAL;
+  if (on && !h->ebase) {
+    const size_t fb = sizeof(double) * (size_t)(5 + h->P.ns) * h->P.fstride;
+    h->ebytes = fb + 256;
+    if (hipMalloc(&h->ebase, h->ebytes) != hipSuccess) {
+      h->ebase = nullptr;
+      return set_err(h, SG_ENOMEM, "error register allocation failed");
+    }
+    HIPCHK(h, hipMemsetAsync(h->ebase, 0, h->ebytes, h->stream));
+  }
+  h->adaptive = on ? 1 : 0;
+  h->F.erk = on ? (double *)h->ebase : nullptr;
+  h->F.emax = on ? (unsigned long long *)((char *)h->ebase + (h->ebytes - 256)) : nullptr;
+  if (on) {
+    h->errold = errold;
+    h->errldr = errldr;
+    launch_zero(h->F.erk, (long)(5 + h->P.ns) * h->P.fstride, h->stream);
+  }
+  return 0;
+}
+
+int sg_get_timestep(sg_handle *h, double *tstep, double *errold, double *errldr) {
+  if (!h) return SG_EINVAL;
+  if (tstep) *tstep = h->tstep;
+  if (errold) *errold = h->errold;
+  if (errldr) *errldr = h->errldr;
   return 0;
 }
 
@@ -460,7 +536,7 @@ int sg_rhs(sg_handle *h, int irk, double *rhs) {
   HIPCHK(h, hipMalloc((void **)&out, qbytes));
   HIPCHK(h, hipMemcpyAsync(save, h->F.q, qbytes, hipMemcpyDeviceToDevice, h->stream));
   int rc = eval_rhs(h);
-  if (!rc) rc = timed(h, K_FINAL, [&] { launch_final(P, h->dcfg, h->F, 2, 0.0, 0.0, out, h->stream); });
+  if (!rc) rc = timed(h, K_FINAL, [&] { launch_final(P, h->dcfg, h->F, 2, 0.0, 0.0, 0.0, out, h->stream); });
   if (!rc) {
     size_t fb = sizeof(double) * P.npts;
     for (int v = 0; v < 5 + P.ns; v++)
@@ -537,7 +613,7 @@ int sg_profile_read(sg_handle *h, int idx, char *name, size_t n, double *total_m
 
 int sg_memory(sg_handle *h, size_t *bytes) {
   if (!h || !bytes) return SG_EINVAL;
-  *bytes = h->bytes;
+  *bytes = h->bytes + (h->ebase ? h->ebytes : 0);
   return 0;
 }
 

@@ -14,7 +14,8 @@ template <int NS> void launch_bountt_t(const Params &, const sg_config *, const 
 template <int NS> void launch_prim_t(const Params &, const sg_config *, const Fields &, hipStream_t);
 template <int NS> void launch_pass_t(int, const Params &, const Fields &, hipStream_t);
 template <int NS>
-void launch_final_t(const Params &, const sg_config *, const Fields &, int, double, double, double *, hipStream_t);
+void launch_final_t(const Params &, const sg_config *, const Fields &, int, double, double, double, double *,
+                    hipStream_t);
 
 bool species_supported(int ns) {
 #define CASE(N) if (ns == N) return true;
@@ -64,9 +65,9 @@ void launch_pass(int dir, const Params &P, const Fields &F, hipStream_t st) {
   }
 }
 void launch_final(const Params &P, const sg_config *c, const Fields &F, int mode, double lhs, double rhs,
-                  double *out, hipStream_t st) {
+                  double erc, double *out, hipStream_t st) {
   switch (P.ns) {
-#define CASE(N) case N: launch_final_t<N>(P, c, F, mode, lhs, rhs, out, st); break;
+#define CASE(N) case N: launch_final_t<N>(P, c, F, mode, lhs, rhs, erc, out, st); break;
     SG_NS_LIST(CASE)
 #undef CASE
     default: break;

@@ -41,6 +41,7 @@ struct Params {
 #define SG_NROW 11    // derivative row kinds: interior, 5 left and 5 right closure rows
 struct Tables {
   double roww[SG_NROW][10];                      // 5-term row weights W1, W2 per row kind
+  double mixw[SG_NROW][7];                       // mixed-derivative row weights W0..W4, V0, V1
   double cp[SG_MAX_SPEC][SG_NTINT][SG_NCPMAX];   // amascp polynomial part
   double ct[SG_MAX_SPEC][SG_NTINT][SG_NCPMAX];   // amasct
   double ch[SG_MAX_SPEC][SG_NTINT][SG_NCPMAX];   // amasch
@@ -68,6 +69,8 @@ struct Fields {
   uint32_t *itndex;  // packed temperature interval indices (2 bits per species)
   double *face;   // NSCBC face data, see FaceSlot, [face 0/1][slot][ny*nz]
   int *err;       // [0]=error code, [1..3] = failing point (1-based)
+  double *erk;    // RK error register (5+ns), adaptive time step only, else null
+  unsigned long long *emax;   // global error-norm maximum (bits of a non-negative double)
 };
 
 enum PrimSlot { P_U = 0, P_V, P_W, P_E, P_T, P_P, P_LAM, P_MU, P_HMIX, P_NSCAL };
@@ -88,7 +91,10 @@ void launch_prim(const Params &P, const sg_config *dcfg, const Fields &F, hipStr
 void launch_pass(int dir, const Params &P, const Fields &F, hipStream_t st);
 // mode: 0 = LINCOM (coefficients lhs, rhs), 1 = FINCOM, 2 = write RHS to out
 void launch_final(const Params &P, const sg_config *dcfg, const Fields &F, int mode, double lhs,
-                  double rhs, double *out, hipStream_t st);
+                  double rhs, double erc, double *out, hipStream_t st);
+// adaptive step: max over points and variables of |err|/(|run| + norm) into F.emax
+void launch_errmax(const Params &P, const Fields &F, hipStream_t st);
+void launch_zero(double *p, long n, hipStream_t st);
 void launch_bountt(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st);
 void launch_derivs(const Params &P, const double *f, double *out, hipStream_t st);
 bool species_supported(int ns);

@@ -39,6 +39,19 @@ __device__ __forceinline__ T &fld(T *base, uint32_t bo) {
   return *(T *)((char *)base + bo);
 }
 
+// XCD-aware block order.  Work-groups are dispatched round-robin over the
+// chip's 8 XCDs, each with its own L2; consecutive block ids therefore land on
+// different L2s and neighbouring tiles (which share stencil halos) miss.  The
+// remap gives each XCD one contiguous chunk of logical blocks: hardware block b
+// runs on XCD b % 8 as that XCD's (b / 8)-th block, and becomes logical block
+// start(XCD) + b / 8.  A bijection on [0, n) for any n.
+#define SG_NXCD 8
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int q = n / SG_NXCD, r = n % SG_NXCD;
+  const int x = b % SG_NXCD, slot = b / SG_NXCD;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + slot;
+}
+
 __device__ __forceinline__ void coords(const Params &P, long idx, int &i, int &j, int &k) {
   int nxy = P.nx * P.ny;
   k = (int)(idx / nxy);
@@ -191,6 +204,12 @@ __global__ void k_bountt(Params P, const sg_config *__restrict__ c, Fields F) {
     double y = strd * c->yrin[s];
     F.q[(5 + s) * fs + p] = y;
     F.run[(5 + s) * fs + p] = y;
+  }
+  // the imposed values carry no RK error (src/bountt.f)
+  if (F.erk) {
+    for (int m = 0; m < 4; m++) F.erk[m * fs + p] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) F.erk[(5 + s) * fs + p] = 0.0;
   }
 }
 
@@ -635,7 +654,9 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   __shared__ double tile[2][NIT * NTH];
   const int t = threadIdx.x;
   const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
-  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * T_::TZ;
+  const int nbx = gridDim.x, nby = gridDim.y;
+  const int lb = xcd_block(blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z), nbx * nby * gridDim.z);
+  const int x0 = (lb % nbx) * TX, y0 = ((lb / nbx) % nby) * TY, z0 = (lb / (nbx * nby)) * T_::TZ;
   const int i = x0 + lx, j = y0 + ly, k = z0 + lz;
   const bool active = i < P.nx && j < P.ny && k < P.nz;
   const long idx = ((long)k * P.ny + j) * P.nx + i;
@@ -937,33 +958,36 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
 // register update (src/lincom.f:53-169, src/fincom.f:54-168).
 template <bool BND, int NS>
 __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__restrict__ c, Fields F, int mode,
-                                               double lhs, double rhsc, double *__restrict__ out) {
-  long idx = (long)blockIdx.x * BLK + threadIdx.x;
+                                               double lhs, double rhsc, double erc, double *__restrict__ out) {
+  // bounded x: mixed-derivative row weights in LDS (uniform row form)
+  __shared__ double mtab[BND ? SG_NROW * 7 : 1];
+  const int tl = threadIdx.x;
+  if (BND) {
+    if (tl < SG_NROW * 7) mtab[tl] = (&F.tab->mixw[0][0])[tl];
+    __syncthreads();
+  }
+  long idx = (long)xcd_block(blockIdx.x, gridDim.x) * BLK + tl;
   if (idx >= P.npts) return;
   int i, j, k;
   coords(P, idx, i, j, k);
   const long fs = P.fstride;
+  const uint32_t bo = (uint32_t)idx * 8u;
   const double *pr = F.prim;
   const double *uu = pr + P_U * fs, *vv = pr + P_V * fs, *ww = pr + P_W * fs;
-  const double u = uu[idx], v = vv[idx], w = ww[idx], mu = pr[P_MU * fs + idx];
+  const double u = fld(uu, bo), v = fld(vv, bo), w = fld(ww, bo), mu = fld(pr + P_MU * fs, bo);
   const double *G = F.grad;
-  const double dux = G[idx], dvx = G[fs + idx], dwx = G[2 * fs + idx], dmux = G[3 * fs + idx];
-  const double duy = G[4 * fs + idx], dvy = G[5 * fs + idx], dwy = G[6 * fs + idx],
-               dmuy = G[7 * fs + idx];
-  const double duz = G[8 * fs + idx], dvz = G[9 * fs + idx], dwz = G[10 * fs + idx],
-               dmuz = G[11 * fs + idx];
+  auto gr = [&](int q) { return fld(G + q * fs, bo); };
   const double *A = F.acc;
-  double rD = A[idx], rU = A[fs + idx], rV = A[2 * fs + idx], rW = A[3 * fs + idx],
-         rE = A[4 * fs + idx];
-  double rY[NS];
-#pragma unroll
-  for (int s = 0; s < NS; s++) rY[s] = A[(5 + s) * fs + idx];
+  double rD = fld(A, bo), rU = fld(A + fs, bo), rV = fld(A + 2 * fs, bo), rW = fld(A + 3 * fs, bo),
+         rE = fld(A + 4 * fs, bo);
   auto fz = [&](const int *fl, int dir) {
     int m = dir == 0 ? i : (dir == 1 ? j : k);
     int n = dir == 0 ? P.nx : (dir == 1 ? P.ny : P.nz);
     return (m == 0 && fl[2 * dir]) || (m == n - 1 && fl[2 * dir + 1]);
   };
   const double fthd = P.fthd, tthd = P.tthd;
+  const double dux = gr(0), dvy = gr(5), dwz = gr(10);
+  const double dmux = gr(3), dmuy = gr(7), dmuz = gr(11);
   // normal stresses with viscosity gradients
   {
     double sxx = fthd * dux - tthd * (dvy + dwz);
@@ -982,22 +1006,53 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
     rW = rW + t;
     rE = rE + w * t;
   }
-  // cross derivatives
-  const Sten Sx = make_sten<BND>(i, P.nx, 1);
-  const Sten Sy = make_sten<false>(j, P.ny, (int)P.sy);
-  const Sten Sz = make_sten<false>(k, P.nz, (int)P.sz);
-  auto mxy = [&](const double *a) {
-    return dmix<BND>(P, Sx, 0, 1, [&](int k1, int k2) { return a[idx + Sx.o[k1] + Sy.o[k2]]; });
+  // Cross derivatives from neighbours at byte offsets: x bounded (closure
+  // rows stay inside) or periodic, y and z periodic.
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  auto ox = [&](int a) -> int {
+    if (BND) return a * 8;
+    int ii = i + a;
+    ii += ii < 0 ? nx : (ii >= nx ? -nx : 0);
+    return (ii - i) * 8;
   };
-  auto mxz = [&](const double *a) {
-    return dmix<BND>(P, Sx, 0, 2, [&](int k1, int k2) { return a[idx + Sx.o[k1] + Sz.o[k2]]; });
+  auto oy = [&](int b) -> int {
+    int jj = j + b;
+    jj += jj < 0 ? ny : (jj >= ny ? -ny : 0);
+    return (jj - j) * nx * 8;
   };
-  auto myz = [&](const double *a) {
-    return dmix<false>(P, Sy, 1, 2, [&](int k1, int k2) { return a[idx + Sy.o[k1] + Sz.o[k2]]; });
+  auto oz = [&](int b) -> int {
+    int kk = k + b;
+    kk += kk < 0 ? nz : (kk >= nz ? -nz : 0);
+    return (kk - k) * nx * ny * 8;
+  };
+  // x closure row of this point; waves holding any closure row use the
+  // uniform row form, others the interior diagonal stencil
+  int row = 0;
+  if (BND) {
+    if (i < 5) row = 1 + i;
+    else if (i > nx - 6) row = 6 + (nx - 1 - i);
+  }
+  const bool xedge = BND && __ballot(row != 0) != 0;
+  int pk[5] = {0, 0, 0, 0, 0};
+  if (xedge) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) pk[q] = c_mix_row_offsets.w[row][q];
+  }
+  Sten S0;
+  S0.mode = 0;
+  auto mixx = [&](const double *f, auto od2, int d2) {
+    auto g = [&](int a, int b) { return fld(f, bo + (uint32_t)(ox(a) + od2(b))); };
+    if (xedge)
+      return dmix_rows(P, pk, 0, d2, [&](int q) { return mtab[BND ? row * 7 + q : 0]; }, g);
+    return dmix<false>(P, S0, 0, d2, [&](int k1, int k2) { return g(k1 - 8, k2 - 8); });
+  };
+  auto myz = [&](const double *f) {
+    return dmix<false>(P, S0, 1, 2, [&](int k1, int k2) { return fld(f, bo + (uint32_t)(oy(k1 - 8) + oz(k2 - 8))); });
   };
   {  // xy pair
-    double S7 = duy + dvx;
-    double ma = mxy(uu), mb = mxy(vv);
+    const double S7 = gr(4) + gr(1);   // du/dy + dv/dx
+    double ma = mixx(uu, oy, 1), mb = mixx(vv, oy, 1);
+    pin(ma); pin(mb);
     double t = fz(P.fvsn, 0) ? 0.0 : tthd * mu * mb;
     rU = rU - t; rE = rE - u * t;
     t = fz(P.fvsn, 1) ? 0.0 : tthd * mu * ma;
@@ -1009,8 +1064,9 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
     rE = rE + mu * S7 * S7;
   }
   {  // xz pair
-    double S7 = duz + dwx;
-    double ma = mxz(uu), mb = mxz(ww);
+    const double S7 = gr(8) + gr(2);   // du/dz + dw/dx
+    double ma = mixx(uu, oz, 2), mb = mixx(ww, oz, 2);
+    pin(ma); pin(mb);
     double t = fz(P.fvsn, 0) ? 0.0 : tthd * mu * mb;
     rU = rU - t; rE = rE - u * t;
     t = fz(P.fvsn, 2) ? 0.0 : tthd * mu * ma;
@@ -1022,8 +1078,9 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
     rE = rE + mu * S7 * S7;
   }
   {  // yz pair
-    double S7 = dvz + dwy;
+    const double S7 = gr(9) + gr(6);   // dv/dz + dw/dy
     double ma = myz(vv), mb = myz(ww);
+    pin(ma); pin(mb);
     double t = fz(P.fvsn, 1) ? 0.0 : tthd * mu * mb;
     rV = rV - t; rE = rE - v * t;
     t = fz(P.fvsn, 2) ? 0.0 : tthd * mu * ma;
@@ -1034,6 +1091,9 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
     rW = rW + t; rE = rE + w * t;
     rE = rE + mu * S7 * S7;
   }
+  double rY[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) rY[s] = fld(A + (5 + s) * fs, bo);
   // ---------------- NSCBC faces
   if (BND && (i == 0 || i == P.nx - 1)) {
     const long nf = (long)P.ny * P.nz, fq = (long)k * P.ny + j;
@@ -1053,8 +1113,9 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
     if (i == 0 && P.inflow_xl) {
       // NSBCI3, src/bounds.f:610-682 (imposed values have zero time derivative)
       // drho/dx at the face (src/rhscal.f:3966-3990), one-sided closure row
+      const Sten Sx = make_sten<BND>(i, P.nx, 1);
       const double drdx = d1<BND>(P, Sx, 0, [&](int kk) { return F.q[idx + Sx.o[kk]]; });
-      double bcl1 = dux, bcl2 = drdx, bcl3 = dvx, bcl4 = dwx, bcl5 = fc[F_DPDX * nf + fq];
+      double bcl1 = dux, bcl2 = drdx, bcl3 = gr(1), bcl4 = gr(2), bcl5 = fc[F_DPDX * nf + fq];
       fornow = strd * acou * bcl1;
       bcl1 = 0.5 * (u - acou) * (bcl5 - fornow);
       bcl2 = u * (bcl2 - bcl5 * ova2);
@@ -1079,7 +1140,7 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
       const double strp = pr[P_P * fs + idx];
       const double t51 = -v * fc[F_DPDY * nf + fq] - w * fc[F_DPDZ * nf + fq];
       const double t52 = -strp * (dvy + dwz);
-      const double t2b = -v * duy - w * duz;
+      const double t2b = -v * gr(4) - w * gr(8);
       const double tt1 = t51 + t52 * (gam1 + 1.0) - strd * acou * t2b;
       double sorp = 0.0;
 #pragma unroll
@@ -1109,19 +1170,20 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
 #pragma unroll
   for (int vv2 = 0; vv2 < 5 + NS; vv2++) {
     const int slot = vv2 < 5 ? vv2 : 5;
-    const long o = vv2 * fs + idx;
+    double *runv = F.run + vv2 * fs, *qv = F.q + vv2 * fs;
     if (i >= P.rklo[slot]) {
-      double old = F.run[o];
+      if (F.erk) fld(F.erk + vv2 * fs, bo) += erc * R[vv2];   // *ERR += RKERR * R
+      double old = fld(runv, bo);
       if (mode == 0) {
-        F.run[o] = old + lhs * R[vv2];
-        F.q[o] = old + rhsc * R[vv2];
+        fld(runv, bo) = old + lhs * R[vv2];
+        fld(qv, bo) = old + rhsc * R[vv2];
       } else {
         double nw = old + lhs * R[vv2];
-        F.run[o] = nw;
-        F.q[o] = nw;
+        fld(runv, bo) = nw;
+        fld(qv, bo) = nw;
       }
     } else {
-      F.q[o] = R[vv2];
+      fld(qv, bo) = R[vv2];
     }
   }
 }
@@ -1191,10 +1253,10 @@ void launch_pass_t(int dir, const Params &P, const Fields &F, hipStream_t st) {
 }
 template <int NS>
 void launch_final_t(const Params &P, const sg_config *dcfg, const Fields &F, int mode, double lhs,
-                    double rhs, double *out, hipStream_t st) {
+                    double rhs, double erc, double *out, hipStream_t st) {
   dim3 g(grid_for(P.npts)), b(BLK);
-  if (P.bnd[0]) hipLaunchKernelGGL((k_final<true, NS>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, out);
-  else hipLaunchKernelGGL((k_final<false, NS>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, out);
+  if (P.bnd[0]) hipLaunchKernelGGL((k_final<true, NS>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, erc, out);
+  else hipLaunchKernelGGL((k_final<false, NS>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, erc, out);
 }
 
 #ifdef SG_NS
@@ -1204,10 +1266,53 @@ template void launch_bountt_t<SG_NS>(const Params &, const sg_config *, const Fi
 template void launch_prim_t<SG_NS>(const Params &, const sg_config *, const Fields &, hipStream_t);
 template void launch_pass_t<SG_NS>(int, const Params &, const Fields &, hipStream_t);
 template void launch_final_t<SG_NS>(const Params &, const sg_config *, const Fields &, int, double, double,
-                                    double *, hipStream_t);
+                                    double, double *, hipStream_t);
 #endif
 
 #ifdef SG_DERIVS
+// ADAPTT error norm (src/adaptt.f:60-160): max over variables and points of
+// |ERR| / (|RUN| + norm_v) with the norms of src/dtinit.f:138-145.  A NaN
+// never exceeds the running maximum, as in the reference's comparisons.
+__global__ void __launch_bounds__(BLK) k_errmax(Params P, Fields F) {
+  __shared__ double red[BLK / 64];
+  const long idx = (long)blockIdx.x * BLK + threadIdx.x;
+  double tot = 0.0;
+  if (idx < P.npts) {
+    const long fs = P.fstride;
+    for (int v = 0; v < 5 + P.ns; v++) {
+      const double nrm = v == 0 ? 0.0 : (v < 4 ? 1.0e-6 : (v == 4 ? 1.0e-2 : 1.0e-10));
+      const double f = fabs(F.erk[v * fs + idx]) / (fabs(F.run[v * fs + idx]) + nrm);
+      if (f > tot) tot = f;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double u = __shfl_down(tot, o, 64);
+    if (u > tot) tot = u;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = tot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BLK / 64; w++)
+      if (red[w] > tot) tot = red[w];
+    // non-negative doubles order like their bit patterns
+    atomicMax(F.emax, (unsigned long long)__double_as_longlong(tot));
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_zero(double *p, long n) {
+  const long idx = (long)blockIdx.x * BLK + threadIdx.x;
+  if (idx < n) p[idx] = 0.0;
+}
+
+void launch_errmax(const Params &P, const Fields &F, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(BLK), 0, st, (double *)F.emax, 1L);
+  hipLaunchKernelGGL(k_errmax, dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, F);
+}
+
+void launch_zero(double *p, long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero, dim3(grid_for(n)), dim3(BLK), 0, st, p, n);
+}
+
 void launch_derivs(const Params &P, const double *f, double *out, hipStream_t st) {
   dim3 g(grid_for(P.npts)), b(BLK);
   if (P.bnd[0]) hipLaunchKernelGGL((k_derivs<true, false, false>), g, b, 0, st, P, f, out);

@@ -224,6 +224,79 @@ __device__ __forceinline__ double d2_rows(const Params &P, const int packed[5], 
   return r * P.ovdl2[dir];
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-derivative rows in one uniform form.  With
+//   Q(p, m, q) = g(p, q) - g(p, -q) - g(m, q) + g(m, -q)
+// every row of dmix below is
+//   sum_k W_k * (V_0 * Q(P_k, M_k, q0_k) + V_1 * Q(P_k, M_k, q1_k)),   k = 0..4,
+// with V = (1, 0) and q0_k = P_k for the diagonal rows (interior and the 4th,
+// 6th, 8th-order rows) and V = (my1[0], my1[1]), q = (1, 2) for the one-sided
+// rows.  Unused terms have W = 0 and P = M = q = 0, whose Q is an exact zero,
+// and V_1 = 0 multiplies an exact-zero Q, so results are bit-identical to dmix.
+struct MixRowOffsets {
+  int w[SG_NROW][5];   // (P, M, q0, q1) of term k as signed bytes in word k
+};
+constexpr MixRowOffsets make_mix_row_offsets() {
+  MixRowOffsets r{};
+  for (int row = 0; row < SG_NROW; row++) {
+    int p[5] = {}, m[5] = {}, q0[5] = {}, q1[5] = {};
+    auto diag = [&](int n) {
+      for (int k = 0; k < n; k++) { p[k] = k + 1; m[k] = -(k + 1); q0[k] = k + 1; }
+    };
+    auto tr = [&](int a0, int a1, int a2, int a3, int b0, int b1, int b2, int b3) {
+      const int a[4] = {a0, a1, a2, a3}, b[4] = {b0, b1, b2, b3};
+      for (int k = 0; k < 4; k++) { p[k] = a[k]; m[k] = b[k]; q0[k] = 1; q1[k] = 2; }
+    };
+    switch (row) {
+      case 0: diag(5); break;
+      case 1: tr(1, 2, 3, 4, 0, 0, 0, 0); break;
+      case 2: tr(-1, 1, 2, 3, 0, 0, 0, 0); break;
+      case 3: case 8: diag(2); break;
+      case 4: case 9: diag(3); break;
+      case 5: case 10: diag(4); break;
+      case 7: tr(0, 0, 0, 0, 1, -1, -2, -3); break;
+      default: tr(0, 0, 0, 0, -1, -2, -3, -4); break;   // 6: last point
+    }
+    for (int k = 0; k < 5; k++)
+      r.w[row][k] = (int)((unsigned)(p[k] & 255) | ((unsigned)(m[k] & 255) << 8) |
+                          ((unsigned)(q0[k] & 255) << 16) | ((unsigned)(q1[k] & 255) << 24));
+  }
+  return r;
+}
+__constant__ static const MixRowOffsets c_mix_row_offsets = make_mix_row_offsets();
+
+// Weights W_0..W_4, V_0, V_1 of a mixed-derivative row (host side).
+inline void mix_row_weights(const Params &P, int row, double *w) {
+  for (int k = 0; k < 7; k++) w[k] = 0.0;
+  w[5] = 1.0;
+  switch (row) {
+    case 0: for (int k = 0; k < 5; k++) w[k] = P.mc[k]; break;
+    case 1: case 6: for (int k = 0; k < 4; k++) w[k] = P.m1[k]; w[5] = P.my1[0]; w[6] = P.my1[1]; break;
+    case 2: case 7: for (int k = 0; k < 4; k++) w[k] = P.m2[k]; w[5] = P.my1[0]; w[6] = P.my1[1]; break;
+    case 3: case 8: for (int k = 0; k < 2; k++) w[k] = P.m3[k]; break;
+    case 4: case 9: for (int k = 0; k < 3; k++) w[k] = P.m4[k]; break;
+    default: for (int k = 0; k < 4; k++) w[k] = P.m5[k]; break;
+  }
+}
+
+// Mixed derivative in the uniform row form; g(a, b) gives the value at offset
+// a along d1 and b along d2, w(k) the row's 7 weights, packed its offsets.
+template <class Gn, class Wn>
+__device__ __forceinline__ double dmix_rows(const Params &P, const int packed[5], int d1, int d2, Wn w, Gn g) {
+  auto Qv = [&](int p, int m, int q) { return g(p, q) - g(p, -q) - g(m, q) + g(m, -q); };
+  const double v0 = w(5), v1 = w(6);
+  double r = 0.0;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const int pk = packed[k];
+    const int p = __builtin_amdgcn_sbfe(pk, 0, 8), m = __builtin_amdgcn_sbfe(pk, 8, 8);
+    const int q0 = __builtin_amdgcn_sbfe(pk, 16, 8), q1 = __builtin_amdgcn_sbfe(pk, 24, 8);
+    const double t = w(k) * (v0 * Qv(p, m, q0) + v1 * Qv(p, m, q1));
+    r = k == 0 ? t : r + t;
+  }
+  return r * P.ovdel[d1] * P.ovdel[d2];
+}
+
 // Mixed derivative in the (d1,d2) plane; d2 periodic, d1 bounded if BND.
 // g(k1, k2) returns the value at offsets q1 = k1 - 8 (along d1), q2 = k2 - 8.
 #define G(a, b) g((a) + 8, (b) + 8)

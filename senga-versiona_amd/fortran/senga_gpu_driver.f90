@@ -77,11 +77,14 @@ contains
   ! On restart the reference keeps cont.dat's TSTEP through the REAL variable
   ! TSOLD (src/indata.f:1420-1421,1476), i.e. rounded to single precision.
   tstep = real(real(ctl%tstep, c_float), c_double)
-  if (ctl%nstpsw /= 0) call fail('only the fixed time step (step switch 0) is supported')
 
   rc = sg_create(cfg, h)
   if (rc /= SG_OK) call fail('sg_create failed (unsupported configuration or no device)')
   rc = sg_set_timestep(h, tstep)
+  if (ctl%nstpsw == 1) then
+    rc = sg_set_adaptive(h, 1_c_int, errold, errldr)   ! ADAPTT after each step
+    if (rc /= SG_OK) call sgfail(h, rc)
+  end if
   rc = sg_upload_state(h, run, etime)
   if (rc /= SG_OK) call sgfail(h, rc)
 
@@ -93,9 +96,10 @@ contains
       rc = sg_substage(h, int(irk, c_int))         ! BOUNDT .. LINCOM / FINCOM
       if (rc /= SG_OK) call sgfail(h, rc)
     end do
-    rc = sg_end_step(h)                             ! ETIME += TSTEP; BOUNTT
+    rc = sg_end_step(h)                             ! ETIME += TSTEP; BOUNTT; ADAPTT
     if (rc /= SG_OK) call sgfail(h, rc)
     etime = etime + tstep
+    rc = sg_get_timestep(h, tstep, errold, errldr)
     if (mod(itime, ctl%ntdump) == 0) then
       rc = sg_download_state(h, c_loc(run), c_null_ptr, c_null_ptr, c_null_ptr)
       if (rc /= SG_OK) call sgfail(h, rc)

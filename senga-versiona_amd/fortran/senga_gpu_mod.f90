@@ -80,7 +80,7 @@ module senga_gpu
   public :: sg_abi_version, sg_config_size, sg_create, sg_destroy
   public :: sg_upload_state, sg_download_state, sg_set_timestep
   public :: sg_substage, sg_end_step, sg_step, sg_rhs, sg_derivatives
-  public :: sg_sync, sg_last_error, sg_memory
+  public :: sg_sync, sg_last_error, sg_memory, sg_set_adaptive, sg_get_timestep
 
   interface
     integer(c_int) function sg_abi_version() bind(C, name="sg_abi_version")
@@ -153,6 +153,18 @@ module senga_gpu
       integer(c_int), intent(out) :: ijk(3)
       character(kind=c_char), intent(out) :: msg(*)
       integer(c_size_t), value :: n
+    end function
+    ! cont.dat step switch 1: ADAPTT after every step (src/adaptt.f)
+    integer(c_int) function sg_set_adaptive(h, on, errold, errldr) bind(C, name="sg_set_adaptive")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: on
+      real(c_double), value :: errold, errldr
+    end function
+    integer(c_int) function sg_get_timestep(h, tstep, errold, errldr) bind(C, name="sg_get_timestep")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: tstep, errold, errldr
     end function
     integer(c_int) function sg_memory(h, bytes) bind(C, name="sg_memory")
       import :: c_int, c_ptr, c_size_t

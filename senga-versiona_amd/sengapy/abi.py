@@ -166,7 +166,7 @@ def _dp(a):
 ABI_FUNCS = ["sg_abi_version", "sg_config_size", "sg_create", "sg_destroy", "sg_upload_state",
              "sg_download_state", "sg_set_timestep", "sg_substage", "sg_end_step", "sg_step",
              "sg_rhs", "sg_derivatives", "sg_sync", "sg_last_error", "sg_get_stream",
-             "sg_profile", "sg_profile_read", "sg_memory"]
+             "sg_profile", "sg_profile_read", "sg_memory", "sg_set_adaptive", "sg_get_timestep"]
 
 
 def load_library(path=None):
@@ -196,6 +196,8 @@ def load_library(path=None):
     lib.sg_profile.argtypes = [vp, C.c_int]
     lib.sg_profile_read.argtypes = [vp, C.c_int, C.c_char_p, C.c_size_t, dp, P(C.c_int64)]
     lib.sg_memory.argtypes = [vp, P(C.c_size_t)]
+    lib.sg_set_adaptive.argtypes = [vp, C.c_int, C.c_double, C.c_double]
+    lib.sg_get_timestep.argtypes = [vp, dp, dp, dp]
     for f in ABI_FUNCS:
         if f not in ("sg_config_size",):
             getattr(lib, f).restype = C.c_int
@@ -247,6 +249,16 @@ class Solver:
 
     def set_timestep(self, dt):
         self._chk(self.lib.sg_set_timestep(self.h, dt))
+
+    def set_adaptive(self, on, errold=1.0e-4, errldr=1.0e-4):
+        """Adaptive time step (ADAPTT after each step); errold/errldr are the
+        controller history from the dump."""
+        self._chk(self.lib.sg_set_adaptive(self.h, int(on), errold, errldr))
+
+    def timestep(self):
+        v = [C.c_double() for _ in range(3)]
+        self._chk(self.lib.sg_get_timestep(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def substage(self, irk):
         self._chk(self.lib.sg_substage(self.h, irk))

@@ -178,28 +178,35 @@ def h2_mechanism() -> _chem.Mechanism:
     return _chem.read_chem(os.path.join(DATA, "h2_9sp", "chem.dat"))
 
 
-def h2_nscbc(n=(128, 128, 128), length=(5.0e-3, 5.0e-3, 5.0e-3), seed=7, phi=0.7,
-             uin=1.25, uprime_over_sl=5.0, sl=1.2, mixavg=True, cfl=0.5, nsteps=100,
-             periodic_x=False, radicals=True) -> Case:
-    nx, ny, nz = n
-    mech = h2_mechanism()
-    tab = _chem.derive(mech)
-    dif = _diff.read_diff(os.path.join(DATA, "h2_9sp", "diff.dat"), mech.wmolar) if mixavg else None
+def _h2_mixtures(mech, phi):
+    """Unburnt H2 + air (O2 + 3.76 N2) at equivalence ratio phi and its complete
+    combustion products (H2O, O2 or H2 in excess, N2) as mass fractions."""
     W = mech.wmolar
-    # unburnt phi-mixture of H2 + air(O2 + 3.76 N2), products H2O + excess O2 + N2
     nO2 = 0.5 / phi
     mol_u = {"H2": 1.0, "O2": nO2, "N2": 3.76 * nO2}
-    mol_b = {"H2O": 1.0, "O2": nO2 - 0.5, "N2": 3.76 * nO2}
+    if phi <= 1.0:
+        mol_b = {"H2O": 1.0, "O2": nO2 - 0.5, "N2": 3.76 * nO2}
+    else:
+        mol_b = {"H2O": 2.0 * nO2, "H2": 1.0 - 2.0 * nO2, "N2": 3.76 * nO2}
     idx = {s: i for i, s in enumerate(mech.species)}
 
     def massfrac(mol):
         y = np.zeros(mech.nspec)
-        for s, nm in mol.items():
-            y[idx[s]] = nm * W[idx[s]]
+        for sp, nm in mol.items():
+            y[idx[sp]] = nm * W[idx[sp]]
         return y / y.sum()
 
-    Yu, Yb = massfrac(mol_u), massfrac(mol_b)
-    Tu = 300.0
+    return massfrac(mol_u), massfrac(mol_b), idx
+
+
+def h2_nscbc(n=(128, 128, 128), length=(5.0e-3, 5.0e-3, 5.0e-3), seed=7, phi=0.7,
+             uin=1.25, uprime_over_sl=5.0, sl=1.2, mixavg=True, cfl=0.5, nsteps=100,
+             periodic_x=False, radicals=True, Tu=300.0) -> Case:
+    nx, ny, nz = n
+    mech = h2_mechanism()
+    tab = _chem.derive(mech)
+    dif = _diff.read_diff(os.path.join(DATA, "h2_9sp", "diff.dat"), mech.wmolar) if mixavg else None
+    Yu, Yb, idx = _h2_mixtures(mech, phi)
     Tb = _ic.adiabatic_temperature(tab, Yu, Yb, Tu)
     d = tuple(length[i] / (n[i] - 1) for i in range(3))
     x = _ic.grid(nx, length[0])
@@ -242,5 +249,35 @@ def h2_nscbc(n=(128, 128, 128), length=(5.0e-3, 5.0e-3, 5.0e-3), seed=7, phi=0.7
     ctl.nfmavt = 1 if mixavg else 0
     name = f"h2_{'per' if periodic_x else 'nscbc'}_{'mix' if mixavg else 'cle'}_{nx}x{ny}x{nz}"
     return Case(name, ctl, mech, tab, dif, Dump(st, 0.0, dt),
+                os.path.join(DATA, "h2_9sp", "chem.dat"),
+                os.path.join(DATA, "h2_9sp", "diff.dat") if mixavg else None)
+
+
+def h2_kernel(n=(64, 64, 64), length=(4.0e-3, 4.0e-3, 4.0e-3), seed=11, phi=0.8, Tu=300.0,
+              t_peak=1800.0, radius=1.0e-3, uprime_over_sl=4.0, sl=1.2, mixavg=True, cfl=0.5,
+              nsteps=200) -> Case:
+    """Flame kernel: a spherical Gaussian hot spot T = Tu + (T_peak - Tu) exp(-(r/radius)^2)
+    at the box centre in an unburnt H2-air mixture, seeded Passot-Pouquet
+    turbulence, triply periodic, 1 atm."""
+    nx, ny, nz = n
+    mech = h2_mechanism()
+    tab = _chem.derive(mech)
+    dif = _diff.read_diff(os.path.join(DATA, "h2_9sp", "diff.dat"), mech.wmolar) if mixavg else None
+    Yu, _, _ = _h2_mixtures(mech, phi)
+    d = tuple(length[i] / (n[i] - 1) for i in range(3))
+    shape = (nz, ny, nx)
+    X = [_ic.grid(n[i], length[i]) - 0.5 * n[i] * d[i] for i in range(3)]
+    r2 = X[2][:, None, None] ** 2 + X[1][None, :, None] ** 2 + X[0][None, None, :] ** 2
+    T = Tu + (t_peak - Tu) * np.exp(-r2 / radius ** 2)
+    Y = [np.full(shape, Yu[i]) for i in range(mech.nspec)]
+    u, v, w = _ic.solenoidal_field(shape, (d[2], d[1], d[0]), uprime_over_sl * sl, 1.0e-3, seed,
+                                   spectrum="passot-pouquet")
+    st = _ic.conservative_state(tab, T, np.full(shape, 1.0e5), Y, u, v, w)
+    dt = _f32(cfl * min(d) / sound_speed_max(tab, st, T))
+    ctl = Control(length[0], length[1], length[2], nx, ny, nz, dt, 1, nsteps, 0, prin=1.0e5,
+                  trin=Tu, yrin=[float(y) for y in Yu])
+    ctl.bc = [(1, [0, 0, 0, 0], [0.0] * 4)] * 6
+    ctl.nfmavt = 1 if mixavg else 0
+    return Case(f"h2_kernel_{nx}x{ny}x{nz}", ctl, mech, tab, dif, Dump(st, 0.0, dt),
                 os.path.join(DATA, "h2_9sp", "chem.dat"),
                 os.path.join(DATA, "h2_9sp", "diff.dat") if mixavg else None)

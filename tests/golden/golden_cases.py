@@ -20,7 +20,14 @@ def build(name):
         return C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=False)
     if name == "h2_per_mix_11x12x11":
         return C.h2_nscbc(n=(11, 12, 11), length=(1.4e-3, 1.5e-3, 1.4e-3), mixavg=True, periodic_x=True)
+    if name == ADAPT:
+        c = C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=True)
+        c.control.nstpsw = 1
+        return c
     raise KeyError(name)
+
+
+ADAPT = "h2_nscbc_mix_adapt_12x11x11"
 
 
 NAMES = ["periodic_1step_16x12x11", "h2_nscbc_mix_12x11x11", "h2_nscbc_cle_12x11x11", "h2_per_mix_11x12x11"]

@@ -14,6 +14,8 @@ Per case the fixture holds
   rhs, T1, P1, rate1   one RHS evaluation at RK substage 1 (mode 1)
   run2, T2, P2, rate2  solution after NSTEPS full steps (mode 2)
 and for the operator fixtures (mode 3) the input field and the nine outputs.
+The adaptive-step fixture holds state0, tstep, errold, errldr, and after
+nsteps steps run2, T2, etime2 and the adapted tstep2.
 """
 import os
 import sys
@@ -55,5 +57,25 @@ def main():
         print("wrote", name)
 
 
+ADAPT_STEPS = 4
+
+
+def adaptive():
+    """The H2 NSCBC mixture-averaged case with the adaptive time step
+    (cont.dat step switch 1, ADAPTT after every step): state after ADAPT_STEPS
+    steps, elapsed time and the adapted time step."""
+    name = golden_cases.ADAPT
+    c = golden_cases.build(name)
+    r2 = run_ref.run(c, 2, nsteps=ADAPT_STEPS)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), state0=c.dump.state, tstep=c.dump.tstep,
+                        nsteps=ADAPT_STEPS, errold=c.dump.errold, errldr=c.dump.errldr,
+                        run2=r2["run"], T2=r2["T"], etime2=r2["etime"], tstep2=r2["tstep"])
+    print("wrote", name, "tstep", c.dump.tstep, "->", r2["tstep"])
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["adaptive"]:
+        adaptive()
+    else:
+        main()
+        adaptive()

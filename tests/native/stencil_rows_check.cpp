@@ -40,5 +40,29 @@ int main() {
     std::printf("mode %2d row %2d d1 %.17g %.17g d2 %.17g %.17g %s\n", mode, row, a1, b1, a2, b2, ok ? "ok" : "MISMATCH");
     bad += !ok;
   }
+  // mixed derivatives: every row kind against dmix<true> on a 2-D field
+  for (int k = 0; k < 5; k++) P.mc[k] = 0.3 + 0.7 * k;
+  for (int k = 0; k < 4; k++) { P.m1[k] = 1.9 + k; P.m2[k] = 2.9 - k; P.m5[k] = 0.11 * (k + 1); }
+  for (int k = 0; k < 3; k++) P.m4[k] = 1.3 - 0.2 * k;
+  for (int k = 0; k < 2; k++) { P.m3[k] = 0.77 + k; P.my1[k] = 0.66 - 0.5 * k; }
+  P.ovdel[1] = 1.3;
+  double gv[17][17];
+  for (int a = 0; a < 17; a++)
+    for (int b = 0; b < 17; b++) gv[a][b] = std::cos(0.21 * a + 0.1) * std::sin(0.33 * b + 0.2) + 0.01 * a * b;
+  for (int mode : modes) {
+    Sten S;
+    S.mode = mode;
+    auto g = [&](int a, int b) { return gv[a][b]; };
+    const double a3 = dmix<true>(P, S, 0, 1, g);
+    const int row = row_of_mode(mode);
+    double w[7];
+    mix_row_weights(P, row, w);
+    auto gq = [&](int a, int b) { return gv[a + 8][b + 8]; };
+    auto wq = [&](int k) { return w[k]; };
+    const double b3 = dmix_rows(P, c_mix_row_offsets.w[row], 0, 1, wq, gq);
+    const bool ok = a3 == b3;
+    std::printf("mixed mode %2d row %2d %.17g %.17g %s\n", mode, row, a3, b3, ok ? "ok" : "MISMATCH");
+    bad += !ok;
+  }
   return bad != 0;
 }

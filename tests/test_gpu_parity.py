@@ -126,3 +126,30 @@ def test_temper_failure_is_reported():
         s.sync()
     assert e.value.code in (4, 5)
     assert e.value.ijk == (2, 3, 4)
+
+
+def test_adaptive_step_matches_reference():
+    """ADAPTT on the GPU: error register, device-wide error norm, PID
+    controller; state and adapted time step after several steps against the
+    reference run with the adaptive step switch on, and the oracle."""
+    from pyoracle import Oracle
+    c, fx = golden_cases.load(golden_cases.ADAPT)
+    print("case", golden_cases.ADAPT, flush=True)
+    s = _solver(c)
+    s.set_adaptive(True, float(fx["errold"]), float(fx["errldr"]))
+    s.upload(c.dump.state, 0.0)
+    s.step(int(fx["nsteps"]))
+    run, T2, _, _ = s.download()
+    tstep, _, _ = s.timestep()
+    print("tstep", c.dump.tstep, "->", tstep, "reference", float(fx["tstep2"]), flush=True)
+    assert abs(tstep - float(fx["tstep2"])) <= 1e-10 * float(fx["tstep2"])
+    errs = [rel_err(run[v], fx["run2"][v]) for v in range(run.shape[0])]
+    _report("run", errs)
+    assert max(errs) < TOL
+    assert rel_err(T2, fx["T2"]) < 1e-12
+    o = Oracle(c)
+    o.set_timestep(c.dump.tstep)
+    o.set_adaptive(True, float(fx["errold"]), float(fx["errldr"]))
+    o.set_state(c.dump.state, 0.0)
+    o.step(int(fx["nsteps"]))
+    assert abs(o.timestep()[0] - tstep) <= 1e-10 * tstep

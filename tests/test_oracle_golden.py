@@ -41,3 +41,20 @@ def test_operators_bitexact(name, oracle_built):
     o = Oracle(c)
     out = o.derivatives(fx["field"])
     assert np.array_equal(out, fx["ops"])
+
+
+def test_adaptive_step_bitexact(oracle_built):
+    """ADAPTT: error register, global error norm and PID controller, against
+    the reference run with the adaptive step switch on."""
+    c, fx = golden_cases.load(golden_cases.ADAPT)
+    o = Oracle(c)
+    o.set_timestep(c.dump.tstep)
+    o.set_adaptive(True, float(fx["errold"]), float(fx["errldr"]))
+    o.set_state(c.dump.state, 0.0)
+    o.step(int(fx["nsteps"]))
+    run, T2, _, _ = o.get()
+    tstep, _, _ = o.timestep()
+    assert tstep == float(fx["tstep2"])
+    assert tstep != float(fx["tstep"])
+    assert np.array_equal(run, fx["run2"])
+    assert np.array_equal(T2, fx["T2"])

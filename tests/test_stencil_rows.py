@@ -3,7 +3,8 @@
 tests/native/stencil_rows_check.cpp against the kernel headers and requires
 the 5-term form to equal the row-by-row stencils bit for bit on all 11 row
 kinds (interior, 5 left, 5 right rows of src/dfbydx.f:116-228 and
-src/d2fdx2.f:116-260)."""
+src/d2fdx2.f:116-260), and likewise the mixed-derivative rows of
+src/d2fdxy.f:74-391."""
 import os
 import shutil
 import subprocess
@@ -26,4 +27,4 @@ def test_row_form_equals_switch_form(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     print(r.stdout)
     assert r.returncode == 0, r.stdout
-    assert r.stdout.count(" ok") == 11
+    assert r.stdout.count(" ok") == 22
