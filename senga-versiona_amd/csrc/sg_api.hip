@@ -228,6 +228,9 @@ static int setup_params(sg_handle *h) {
   rtin = rtin * c.trin;
   P.drin = c.prin / rtin;
   P.mixavg = c.mixavg;
+  P.ychunk = 1;
+  for (int d = 32; d >= 1; d--)
+    if (P.ny % d == 0) { P.ychunk = d; break; }
   P.alamda = c.alamda; P.rlamda = c.rlamda; P.prantl = c.prantl;
   P.tdifgb = c.tdifgb; P.pdifgb = c.pdifgb;
   return 0;

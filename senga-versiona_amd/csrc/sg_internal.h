@@ -28,6 +28,7 @@ struct Params {
   int inflow_xl, outflow_xr;   // NSBCI3 at x-left, NSBCO1 at x-right
   int rklo[6];                 // first x index of the RK update for D,U,V,W,E,Y
   double drin, uin, cobcxr, pinfxr;
+  int ychunk;                  // rows per y-chunk of the final kernel's point order (divides ny)
   int mixavg;
   double alamda, rlamda, prantl, tdifgb, pdifgb;
 };

@@ -966,10 +966,22 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
     if (tl < SG_NROW * 7) mtab[tl] = (&F.tab->mixw[0][0])[tl];
     __syncthreads();
   }
-  long idx = (long)xcd_block(blockIdx.x, gridDim.x) * BLK + tl;
-  if (idx >= P.npts) return;
+  // Point order: x fastest, then z, then y within chunks of P.ychunk rows,
+  // so that the +-5-plane z neighbours of the mixed derivatives stay in the
+  // XCD's L2 while its blocks sweep a chunk.
+  const long lp = (long)xcd_block(blockIdx.x, gridDim.x) * BLK + tl;
+  if (lp >= P.npts) return;
   int i, j, k;
-  coords(P, idx, i, j, k);
+  {
+    const long row = lp / P.nx;
+    i = (int)(lp - row * P.nx);
+    const long per = (long)P.ychunk * P.nz;
+    const int jc = (int)(row / per);
+    const int rem = (int)(row - (long)jc * per);
+    k = rem / P.ychunk;
+    j = jc * P.ychunk + (rem - k * P.ychunk);
+  }
+  const long idx = ((long)k * P.ny + j) * P.nx + i;
   const long fs = P.fstride;
   const uint32_t bo = (uint32_t)idx * 8u;
   const double *pr = F.prim;
@@ -1236,7 +1248,9 @@ void launch_prim_t(const Params &P, const sg_config *dcfg, const Fields &F, hipS
 template <int DIR>
 static dim3 pass_grid(const Params &P) {
   using T_ = PassTile<DIR>;
-  return dim3((P.nx + T_::TX - 1) / T_::TX, (P.ny + T_::TY - 1) / T_::TY, (P.nz + T_::TZ - 1) / T_::TZ);
+  const unsigned gx = (P.nx + T_::TX - 1) / T_::TX, gy = (P.ny + T_::TY - 1) / T_::TY,
+                 gz = (P.nz + T_::TZ - 1) / T_::TZ;
+  return dim3(gx, gy, gz);
 }
 
 template <int NS>
