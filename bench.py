@@ -97,6 +97,45 @@ def cpu_baseline(seconds_target=15.0):
                       f"({nsub // 5} RK steps) in {dt:.1f} s on one host core"}
 
 
+def mixed_run(abi, c, a, h, s64):
+    """The mixed-precision mode on the same case and step counts: throughput,
+    speed-up over FP64 and max relative error per field (normalised by the
+    field's maximum) against the FP64 state after warmup + steps steps."""
+    import numpy as np
+    s = abi.Solver(c, precision=1)
+    s.set_timestep(c.dump.tstep)
+    s.upload(c.dump.state, 0.0)
+    stream = C.c_void_p(s.stream())
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    h.hipEventCreate(C.byref(ev0))
+    h.hipEventCreate(C.byref(ev1))
+    s.step(a.warmup)
+    s.sync()
+    h.hipEventRecord(ev0, stream)
+    s.step(a.steps)
+    h.hipEventRecord(ev1, stream)
+    h.hipEventSynchronize(ev1)
+    s.sync()
+    ms = C.c_float(0)
+    h.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+    s.step(a.profile_steps)        # the FP64 run's profiled steps, so both states are at the same step
+    r32, T32, _, _ = s.download()
+    r64, T64, _, _ = s64.download()
+
+    def rel(x, y):
+        return float(np.max(np.abs(x - y)) / max(float(np.max(np.abs(y))), 1e-300))
+
+    ns = c.nspec
+    n3 = c.shape[0] * c.shape[1] * c.shape[2]
+    val = n3 * 5 * a.steps / (ms.value / 1e3) / 1e6
+    print(f"mixed: timed {a.steps} steps: {ms.value:.2f} ms", file=sys.stderr, flush=True)
+    return {"value": round(val, 3), "unit": "Mpoint-RK-substages/s", "ms_per_step": round(ms.value / a.steps, 4),
+            "dtype": "f32 compute / f64 registers",
+            "max_rel_err_T": rel(T32, T64), "max_rel_err_Y": max(rel(r32[5 + k], r64[5 + k]) for k in range(ns)),
+            "max_rel_err_conserved": max(rel(r32[v], r64[v]) for v in range(5 + ns)),
+            "steps_compared": a.warmup + a.steps + a.profile_steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +146,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy (counter calibration)")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision run reported beside the FP64 figure")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (multi-GPU sharding is "
@@ -188,6 +229,8 @@ def main():
         "substage_roofline_frac": substage_bytes * value * 1e6 / HBM_PEAK,
         "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kern.items()},
     }
+    if not a.no_mixed:
+        out["mixed"] = mixed_run(abi, c, a, h, s)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

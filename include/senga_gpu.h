@@ -66,7 +66,11 @@ typedef struct sg_config {
   int32_t nx, ny, nz;
   int32_t nspec, nstep;
   int32_t mixavg;             /* cont.dat mix.av switch (src/diffin.f:64) */
-  int32_t precision;          /* 0 = FP64 throughout (only mode in ABI v1) */
+  int32_t precision;          /* 0 = FP64 throughout; 1 = mixed: FP32 primitives,
+                                 transport, chemistry, derivatives and RHS
+                                 assembly; FP64 RK registers, temperature
+                                 Newton iteration, enthalpies and rate
+                                 exponents */
   int32_t reserved0;
   double xgdlen, ygdlen, zgdlen;  /* domain lengths; spacing L/(N-1) (src/dfinit.f:42-50) */
 

@@ -228,6 +228,8 @@ static int setup_params(sg_handle *h) {
   rtin = rtin * c.trin;
   P.drin = c.prin / rtin;
   P.mixavg = c.mixavg;
+  P.precision = c.precision;
+  P.pofs = c.precision ? c.prin : 0.0;
   P.ychunk = 1;
   for (int d = 32; d >= 1; d--)
     if (P.ny % d == 0) { P.ychunk = d; break; }
@@ -248,7 +250,7 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   if (cfg->nstep < 0 || cfg->nstep > SG_MAX_STEP) return SG_EINVAL;
   if (cfg->nx < 11 || cfg->ny < 11 || cfg->nz < 11) return SG_EINVAL;
   if ((long)cfg->nx * cfg->ny * 8 >= (1L << 31)) return SG_EINVAL;
-  if (cfg->precision != 0) return SG_EINVAL;
+  if (cfg->precision != 0 && cfg->precision != 1) return SG_EINVAL;
   if (cfg->mixavg && (cfg->ncovis < 1 || cfg->ncocon < 1 || cfg->ncodif < 1 || cfg->ncovis > SG_NTRC ||
                       cfg->ncocon > SG_NTRC || cfg->ncodif > SG_NTRC))
     return SG_EINVAL;
@@ -284,8 +286,12 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   const Params &P = h->P;
   int ns = P.ns;
   long fs = P.fstride;
-  long nfields = (5 + ns) * 2 + nprim(ns) + ns + (5 + ns) + 12;
-  size_t bytes = sizeof(double) * (size_t)fs * nfields;
+  // FP64 registers; primitives, rates, accumulators and gradients in the
+  // working precision (FP32 in the mixed mode)
+  const size_t esz = P.precision ? sizeof(float) : sizeof(double);
+  const long nreg = (5 + ns) * 2, nwork = nprim(ns) + ns + (5 + ns) + 12;
+  size_t bytes = sizeof(double) * (size_t)fs * nreg + esz * (size_t)fs * nwork;
+  bytes = ((bytes + 255) / 256) * 256;
   size_t itn = ((sizeof(uint32_t) * (size_t)fs + 255) / 256) * 256;
   size_t facesz = sizeof(double) * 2 * (size_t)nface(ns) * P.ny * P.nz;
   size_t cfgsz = ((sizeof(sg_config) + 255) / 256) * 256;
@@ -297,15 +303,14 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
     return SG_ENOMEM;
   }
   char *b = (char *)h->base;
-  double *d = (double *)b;
   Fields &F = h->F;
-  F.q = d; d += (5 + ns) * fs;
-  F.run = d; d += (5 + ns) * fs;
-  F.prim = d; d += nprim(ns) * fs;
-  F.rate = d; d += ns * fs;
-  F.acc = d; d += (5 + ns) * fs;
-  F.grad = d; d += 12 * fs;
-  b = (char *)d;
+  F.q = (double *)b; b += sizeof(double) * (5 + ns) * fs;
+  F.run = (double *)b; b += sizeof(double) * (5 + ns) * fs;
+  F.prim = (double *)b; b += esz * nprim(ns) * fs;   // element type per precision
+  F.rate = (double *)b; b += esz * ns * fs;
+  F.acc = (double *)b; b += esz * (5 + ns) * fs;
+  F.grad = (double *)b; b += esz * 12 * fs;
+  b = (char *)h->base + bytes;
   F.itndex = (uint32_t *)b; b += itn;
   F.face = (double *)b; b += ((facesz + 255) / 256) * 256;
   h->dcfg = (sg_config *)b; b += cfgsz;
@@ -363,7 +368,7 @@ int sg_upload_state(sg_handle *h, const double *run, double etime) {
     HIPCHK(h, hipMemcpyAsync(h->F.run + v * P.fstride, run + v * P.npts, fb, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->F.q + v * P.fstride, run + v * P.npts, fb, hipMemcpyHostToDevice, h->stream));
   }
-  HIPCHK(h, hipMemsetAsync(h->F.rate, 0, sizeof(double) * P.fstride * P.ns, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->F.rate, 0, (P.precision ? sizeof(float) : sizeof(double)) * P.fstride * P.ns, h->stream));
   if (h->F.erk) launch_zero(h->F.erk, (long)(5 + P.ns) * P.fstride, h->stream);
   int rc = timed(h, K_TEMPIN, [&] { launch_tempin(P, h->dcfg, h->F, h->stream); });
   if (rc) return rc;
@@ -382,12 +387,31 @@ int sg_download_state(sg_handle *h, double *run, double *temp, double *pres, dou
   if (run)
     for (int v = 0; v < 5 + P.ns; v++)
       HIPCHK(h, hipMemcpyAsync(run + v * P.npts, h->F.run + v * P.fstride, fb, hipMemcpyDeviceToHost, h->stream));
-  if (temp) HIPCHK(h, hipMemcpyAsync(temp, h->F.prim + P_T * P.fstride, fb, hipMemcpyDeviceToHost, h->stream));
-  if (pres) HIPCHK(h, hipMemcpyAsync(pres, h->F.prim + P_P * P.fstride, fb, hipMemcpyDeviceToHost, h->stream));
+  if (!P.precision) {
+    if (temp) HIPCHK(h, hipMemcpyAsync(temp, h->F.prim + P_T * P.fstride, fb, hipMemcpyDeviceToHost, h->stream));
+    if (pres) HIPCHK(h, hipMemcpyAsync(pres, h->F.prim + P_P * P.fstride, fb, hipMemcpyDeviceToHost, h->stream));
+    if (rate)
+      for (int s = 0; s < P.ns; s++)
+        HIPCHK(h, hipMemcpyAsync(rate + s * P.npts, h->F.rate + s * P.fstride, fb, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return check_device_error(h);
+  }
+  // mixed: FP32 fields, the stored pressure is p - pofs
+  std::vector<float> tmp((size_t)P.npts);
+  const float *prf = (const float *)h->F.prim, *rtf = (const float *)h->F.rate;
+  auto fetch = [&](const float *src, double *dst, double add) -> int {
+    HIPCHK(h, hipMemcpyAsync(tmp.data(), src, sizeof(float) * P.npts, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (long p = 0; p < P.npts; p++) dst[p] = (double)tmp[p] + add;
+    return 0;
+  };
+  int rc = 0;
+  if (temp && !rc) rc = fetch(prf + P_T * P.fstride, temp, 0.0);
+  if (pres && !rc) rc = fetch(prf + P_P * P.fstride, pres, P.pofs);
   if (rate)
-    for (int s = 0; s < P.ns; s++)
-      HIPCHK(h, hipMemcpyAsync(rate + s * P.npts, h->F.rate + s * P.fstride, fb, hipMemcpyDeviceToHost, h->stream));
+    for (int s = 0; s < P.ns && !rc; s++) rc = fetch(rtf + s * P.fstride, rate + s * P.npts, 0.0);
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (rc) return rc;
   return check_device_error(h);
 }
 

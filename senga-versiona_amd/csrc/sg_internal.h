@@ -29,6 +29,8 @@ struct Params {
   int rklo[6];                 // first x index of the RK update for D,U,V,W,E,Y
   double drin, uin, cobcxr, pinfxr;
   int ychunk;                  // rows per y-chunk of the final kernel's point order (divides ny)
+  int precision;               // 0: FP64; 1: mixed (FP32 primitives, rates, RHS; FP64 registers, T)
+  double pofs;                 // stored pressure is p - pofs (mixed: the default pressure, else 0)
   int mixavg;
   double alamda, rlamda, prantl, tdifgb, pdifgb;
 };

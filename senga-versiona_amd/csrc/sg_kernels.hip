@@ -25,6 +25,7 @@ namespace sg {
 // volatile asm is ordered against the LDS barriers, so each derivative is
 // formed within its own stage.
 __device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void pin(float &x) { asm volatile("" : "+v"(x)); }
 
 // Scheduling boundary between the phases of a long straight-line kernel, so
 // that independent phases are not interleaved (which multiplies live values).
@@ -78,7 +79,7 @@ __device__ void set_error(int *err, int code, int i, int j, int k) {
 
 // ---------------------------------------------------------------------------
 // TEMPIN: bisection from TRIN in 50 K steps, src/tempin.f:55-512
-template <int NS>
+template <int NS, class R>
 __global__ void __launch_bounds__(BLK) k_tempin(Params P, const sg_config *__restrict__ c, Fields F) {
   long idx = (long)blockIdx.x * BLK + threadIdx.x;
   if (idx >= P.npts) return;
@@ -161,11 +162,12 @@ __global__ void __launch_bounds__(BLK) k_tempin(Params P, const sg_config *__res
 #pragma unroll
   for (int s = 0; s < NS; s++) nidx += (uint32_t)locate(c, s, t) << (2 * s);
   F.itndex[idx] = nidx;
-  F.prim[P_T * fs + idx] = t;
+  R *pr = (R *)F.prim;
+  pr[P_T * fs + idx] = (R)t;
   double s7 = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; s++) s7 = s7 + ry[s] * c->rgspec[s];
-  F.prim[P_P * fs + idx] = t * s7;
+  pr[P_P * fs + idx] = (R)(t * s7 - P.pofs);
 }
 
 // ---------------------------------------------------------------------------
@@ -621,6 +623,375 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
   }
 }
 
+// Mixed-precision point physics: the temperature Newton iteration, the
+// primitive variables and the enthalpies are formed in FP64 as in k_prim and
+// stored in FP32; transport and chemistry run in FP32, except that the rate
+// exponents (ln k = ln A + n ln T - E/RT, Gibbs sums) are assembled in FP64
+// before the FP32 exponential, since they are differences of O(10..100) terms.
+template <int NS>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_prim_mx(Params P, const sg_config *__restrict__ c,
+                                              const Tables *__restrict__ tb, Fields F) {
+  long idx = (long)blockIdx.x * BLK + threadIdx.x;
+  if (idx >= P.npts) return;
+  const uint32_t bo = (uint32_t)idx * 8u, b4 = (uint32_t)idx * 4u;
+  const long fs = P.fstride;
+  const double *q = F.q;
+  float *pr = (float *)F.prim;
+  double d = fld(q, bo), ru = fld(q + fs, bo), rv = fld(q + 2 * fs, bo), rw = fld(q + 3 * fs, bo);
+  double re = fld(q + 4 * fs, bo);
+  double ry[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) ry[s] = fld(q + (5 + s) * fs, bo);
+  // --- TEMPER (src/temper.f:58-160) ---
+  double ukuk = (ru * ru + rv * rv + rw * rw) / d;
+  double tc[6], td[6];
+  tc[0] = 0.5 * ukuk - re;
+#pragma unroll
+  for (int m = 1; m <= 5; m++) { tc[m] = 0.0; td[m] = 0.0; }
+  double s7 = 0.0;
+  uint32_t idxo = fld(F.itndex, bo / 2u);
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    int it = itint_of(idxo, s);
+    tc[0] = tc[0] + ry[s] * pick1(tb->hcp[s], it);
+    tc[1] = tc[1] + ry[s] * pick(tb->ct[s], it, 0);
+    td[1] = tc[1];
+    // terms above the interval's order add ry*0, which leaves the sums unchanged
+#pragma unroll
+    for (int m = 2; m <= SG_NCPMAX; m++) {
+      tc[m] = tc[m] + ry[s] * pick(tb->ct[s], it, m - 1);
+      td[m] = td[m] + ry[s] * pick(tb->cp[s], it, m - 1);
+    }
+    s7 = s7 + ry[s] * c->rgspec[s];
+  }
+  const double toltmp = (double)1.0e-10f;  // REAL literal, src/temper.f:33-34
+  double t = (double)fld(pr + P_T * fs, b4);
+  int iter = 1;
+  bool ok = true;
+  for (;;) {
+    double fp = tc[5], dp = td[5];
+#pragma unroll
+    for (int m = 4; m >= 1; m--) {
+      fp = tc[m] + fp * t;
+      dp = td[m] + dp * t;
+    }
+    fp = tc[0] + fp * t;
+    double dt = -fp / dp;
+    if (fabs(dt) > toltmp) {
+      if (iter < 100) {
+        t = t + dt;
+        iter++;
+        continue;
+      }
+      ok = false;
+    }
+    break;
+  }
+  if (!ok) {
+    int i, j, k;
+    coords(P, idx, i, j, k);
+    set_error(F.err, SG_ETEMPER, i, j, k);
+  }
+  uint32_t nidx = 0;
+  double cp = 0.0;
+  int itn[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    int it = locate(c, s, t);
+    itn[s] = it;
+    nidx += (uint32_t)it << (2 * s);
+    cp = cp + ry[s] * horner(tb->cp[s], it, t);
+  }
+  cp = cp / d;
+  fld(F.itndex, bo / 2u) = nidx;
+  SG_PHASE();
+  double pres = t * s7;
+  fld(pr + P_T * fs, b4) = (float)(t);
+  fld(pr + P_P * fs, b4) = (float)(pres - P.pofs);
+  // primitives
+  fld(pr + P_U * fs, b4) = (float)(ru / d);
+  fld(pr + P_V * fs, b4) = (float)(rv / d);
+  fld(pr + P_W * fs, b4) = (float)(rw / d);
+  fld(pr + P_E * fs, b4) = (float)(re / d);
+  // --- conductivity (src/rhscal.f:515-573), FP32 ---
+  const float tf = (float)t;
+  float lam, transp, wmomix = 0.0f;
+  {
+    float fornow = (float)P.alamda * expf((float)P.rlamda * logf(tf));
+    lam = fornow * (float)cp;
+    transp = fornow;
+  }
+  float ryf[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) ryf[s] = (float)ry[s];
+  const float df = (float)d;
+  if (P.mixavg) {
+    transp = logf(tf / (float)P.tdifgb);
+    float c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const float *cc = nullptr;
+      (void)cc;
+      float f = (float)c->condco[s][SG_NTRC - 1];
+#pragma unroll
+      for (int m = SG_NTRC - 2; m >= 0; m--) f = f * transp + (float)c->condco[s][m];
+      float ctr = expf(f);
+      float fornow = ryf[s] * (float)c->ovwmol[s];
+      c1 = c1 + fornow * ctr;
+      c2 = c2 + fornow / ctr;
+      c3 = c3 + fornow;
+    }
+    c3 = 1.0f / c3;
+    c1 = c1 * c3;
+    c2 = c2 * c3;
+    lam = 0.5f * (c1 + 1.0f / c2);
+    wmomix = df * c3;
+  }
+  fld(pr + P_LAM * fs, b4) = lam;
+  SG_PHASE();
+  // --- chemistry (src/chrate.f:146-760) on rho Y ---
+  {
+    const float ysmall = 1.0e-30f, ydenom = 1.0e-15f;
+    float rate[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) rate[s] = 0.0f;
+    const double lt = log(t), ovt = 1.0 / t;
+    double gibbs[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      int it = itn[s];
+      double g = horner(tb->gb[s], it, t);
+      gibbs[s] = pick1(tb->gbh[s], it) * ovt - pick1(tb->gbs[s], it) * lt - g;
+    }
+    __shared__ float lcy[NS][BLK];
+    __shared__ double lgb[NS][BLK];
+    const int tl = threadIdx.x;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      lcy[s][tl] = ryf[s] * (float)c->ovwmol[s];
+      lgb[s][tl] = gibbs[s];
+    }
+    auto cyv = [&](int s) { return lcy[s][tl]; };
+    auto gbv = [&](int s) { return lgb[s][tl]; };
+    for (int st = 0; st < c->nstep; st++) {
+      bool flthrd = true;
+      float s3 = 0.0f;
+      int mb = c->mblist[st];
+      if (mb != 0) {
+#pragma unroll
+        for (int s = 0; s < NS; s++) s3 = s3 + ryf[s] * ((float)c->ovwmol[s] * (float)c->effy3b[mb - 1][s]);
+      }
+      double s2d = c->rparam[st][0] + c->rparam[st][1] * lt - c->rparam[st][2] * ovt;
+      float s1 = expf((float)s2d);
+      int ml = c->mllist[st];
+      if (ml != 0) {
+        const double *rl = c->rclind[ml - 1];
+        float s4 = expf((float)(rl[0] + rl[1] * lt - rl[2] * ovt));
+        float preduc = s3 * s4 / s1;
+        float fornow = (float)rl[3] * preduc / (1.0f + preduc);
+        s1 = s1 * fornow;
+        s2d = (double)logf(s1);
+        flthrd = false;
+      }
+      int mt = c->mtlist[st];
+      if (mt != 0) {
+        const double *rt = c->rctroe[mt - 1];
+        float omalph = 1.0f - (float)rt[3];
+        float s4 = expf((float)(rt[0] + rt[1] * lt - rt[2] * ovt));
+        float preduc = s3 * s4 / s1;
+        float trats1 = tf * (float)rt[4], trats2 = (float)(-rt[5] * ovt), trats3 = tf * (float)rt[6];
+        float ftcent = omalph * expf(trats3) + (float)rt[3] * expf(trats1) + expf(trats2);
+        ftcent = log10f(ftcent);
+        float cfactr = (float)rt[7] + (float)rt[8] * ftcent;
+        float enfact = (float)rt[9] + (float)rt[10] * ftcent;
+        float fornow = log10f(preduc) + cfactr;
+        fornow = fornow / (enfact - (float)rt[11] * fornow);
+        fornow = 1.0f + fornow * fornow;
+        float fbroad = ftcent / fornow;
+        fbroad = expf(fbroad * (float)c->clnten);
+        fornow = fbroad * preduc / (1.0f + preduc);
+        s1 = s1 * fornow;
+        s2d = (double)logf(s1);
+        flthrd = false;
+      }
+      int ms = c->mslist[st];
+      if (ms != 0) {
+        const double *rs = c->rcsrif[ms - 1];
+        float s4 = expf((float)(rs[0] + rs[1] * lt - rs[2] * ovt));
+        float preduc = s3 * s4 / s1;
+        float fornow = (float)(rs[4] * ovt);
+        float fbroad = (float)rs[3] * expf(fornow);
+        fornow = tf * (float)rs[5];
+        fbroad = fbroad + expf(fornow);
+        fornow = log10f(preduc);
+        fornow = 1.0f / (1.0f + log10f(fornow));
+        fbroad = (float)rs[6] * expf(fornow * logf(fbroad));
+        fornow = fbroad * preduc / (1.0f + preduc);
+        s1 = s1 * fornow;
+        s2d = (double)logf(s1);
+        flthrd = false;
+      }
+      for (int m = 0; m < c->nrslen[st]; m++) {
+        float fornow = cyv(c->nrspec[st][m]);
+        fornow = fornow > 0.0f ? fornow : 0.0f;
+        s1 = s1 * fornow;
+      }
+      for (int m = 0; m < c->nrclen[st]; m++) {
+        float fornow = cyv(c->nrcpec[st][m]);
+        fornow = fornow > ysmall ? fornow : ysmall;
+        fornow = expf((float)c->crspec[st][m] * logf(fornow));
+        fornow = fornow / (1.0f + ydenom * fornow);
+        s1 = s1 * fornow;
+      }
+      if (c->mglist[st] != 0) {
+        for (int m = 0; m < c->nsslen[st]; m++) s2d = s2d + c->diffmu[st][m] * gbv(c->nsspec[st][m]);
+        float s2 = expf((float)s2d);
+        for (int m = 0; m < c->npslen[st]; m++) {
+          float fornow = cyv(c->npspec[st][m]);
+          fornow = fornow > 0.0f ? fornow : 0.0f;
+          s2 = s2 * fornow;
+        }
+        for (int m = 0; m < c->npclen[st]; m++) {
+          float fornow = cyv(c->npcpec[st][m]);
+          fornow = fornow > ysmall ? fornow : ysmall;
+          fornow = expf((float)c->cpspec[st][m] * logf(fornow));
+          fornow = fornow / (1.0f + ydenom * fornow);
+          s2 = s2 * fornow;
+        }
+        s1 = s1 - s2;
+      }
+      if (mb != 0 && flthrd) s1 = s1 * s3;
+      if (tb->ratedense) {
+#pragma unroll
+        for (int u = 0; u < NS; u++) rate[u] = rate[u] + s1 * (float)tb->nuw[st][u];
+      } else {
+        for (int m = 0; m < c->nsslen[st]; m++) {
+          int s = c->nsspec[st][m];
+          float add = s1 * (float)c->diffmw[st][m];
+#pragma unroll
+          for (int u = 0; u < NS; u++)
+            if (u == s) rate[u] = rate[u] + add;
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) fld((float *)F.rate + s * fs, b4) = rate[s];
+  }
+  // --- mass fractions and enthalpies (src/rhscal.f:1444-1470), FP64, stored FP32 ---
+  float y[NS];
+  double hmix = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const double yd = ry[s] / d;
+    y[s] = (float)yd;
+    fld(pr + prim_y(P, s), b4) = y[s];
+    int it = itn[s];
+    double h = pick1(tb->hch[s], it) + horner(tb->ch[s], it, t) * t;
+    fld(pr + prim_h(P, s), b4) = (float)h;
+    hmix = hmix + h * yd;
+  }
+  fld(pr + P_HMIX * fs, b4) = (float)hmix;
+  SG_PHASE();
+  if (!P.mixavg) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) fld(pr + prim_rd(P, s), b4) = transp * (float)c->olewis[s];
+    fld(pr + P_MU * fs, b4) = transp * (float)P.prantl;
+  } else {
+    // binary diffusion (src/rhscal.f:1311-1346), FP32
+    float yt[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) yt[s] = y[s] + 1.0e-12f;
+    float c1t = 0.0f;
+#pragma unroll
+    for (int js = 0; js < NS; js++) c1t = c1t + yt[js];
+    const float ppd = (float)(pres / P.pdifgb);
+    float yw2[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) yw2[s] = yt[s] * (float)c->ovwmol[s];
+    auto fit = [&](const double *a) {
+      float f = (float)a[SG_NTRC - 1];
+#pragma unroll
+      for (int m = SG_NTRC - 2; m >= 0; m--) f = f * transp + (float)a[m];
+      return f;
+    };
+    if (tb->diffsym) {
+      float cs[NS], self[NS];
+#pragma unroll
+      for (int s = 0; s < NS; s++) cs[s] = 0.0f;
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+#pragma unroll
+        for (int b = a; b < NS; b++) {
+          const float ictd = expf(-fit(c->diffco[a][b])) * ppd;
+          const float tab = yw2[b] * ictd;
+          cs[a] = cs[a] + tab;
+          if (b == a) {
+            self[a] = tab;
+          } else {
+            cs[b] = cs[b] + yw2[a] * ictd;
+          }
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        float c1 = c1t - yt[s];
+        float c2 = (cs[s] - self[s]) * wmomix;
+        fld(pr + prim_rd(P, s), b4) = df * c1 / c2;
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        float c2 = 0.0f, self = 0.0f;
+#pragma unroll
+        for (int js = 0; js < NS; js++) {
+          const float term = yw2[js] * (expf(-fit(c->diffco[s][js])) * ppd);
+          c2 = c2 + term;
+          if (js == s) self = term;
+        }
+        float c1 = c1t - yt[s];
+        c2 = (c2 - self) * wmomix;
+        fld(pr + prim_rd(P, s), b4) = df * c1 / c2;
+      }
+    }
+    SG_PHASE();
+    // Wilke viscosity (src/rhscal.f:3854-3891), FP32
+    float mus[NS], sq[NS], rsq[NS], yw[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      mus[s] = expf(fit(c->viscco[s]));
+      sq[s] = sqrtf(mus[s]);
+      rsq[s] = 1.0f / sq[s];
+      yw[s] = y[s] * (float)c->ovwmol[s];
+    }
+    float c1 = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      float c2 = 0.0f;
+#pragma unroll
+      for (int js = 0; js < NS; js++) {
+        float fornow = js == s ? 1.0f : sq[s] * rsq[js];
+        fornow = 1.0f + fornow * (float)c->wilko2[s][js];
+        fornow = (float)c->wilko1[s][js] * fornow * fornow;
+        c2 = c2 + yw[js] * fornow;
+      }
+      c1 = c1 + yw[s] * (mus[s] / c2);
+    }
+    fld(pr + P_MU * fs, b4) = c1;
+  }
+  // face data for NSCBC: cp and mixture gas constant (src/rhscal.f:72-131)
+  if (P.inflow_xl || P.outflow_xr) {
+    int i, j, k;
+    coords(P, idx, i, j, k);
+    if (i == 0 || i == P.nx - 1) {
+      long nf = (long)P.ny * P.nz;
+      double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
+      long fq = (long)k * P.ny + j;
+      fc[F_CP * nf + fq] = cp;
+      fc[F_RR * nf + fq] = s7 / d;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Direction pass: all derivatives along DIR and the RHS terms separable in
 // DIR.  The block owns a tile of points; every field (or product of fields)
@@ -643,7 +1014,7 @@ struct PassTile {
   static constexpr int LSTR = DIR == 0 ? 1 : (DIR == 1 ? LX : LX * LY);   // LDS stride along DIR
 };
 
-template <int DIR, bool BND, int NS>
+template <int DIR, bool BND, int NS, class R>
 __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves_per_eu(4))) k_pass(Params P, Fields F) {
   using T_ = PassTile<DIR>;
   constexpr int TX = T_::TX, TY = T_::TY, LX = T_::LX, LY = T_::LY, LN = T_::LN, LSTR = T_::LSTR;
@@ -651,7 +1022,7 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   // LN staged values per field, padded to whole block-sized rows so that
   // every thread loads and stores unconditionally
   constexpr int NIT = (LN + NTH - 1) / NTH;
-  __shared__ double tile[2][NIT * NTH];
+  __shared__ R tile[2][NIT * NTH];
   const int t = threadIdx.x;
   const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
   const int nbx = gridDim.x, nby = gridDim.y;
@@ -711,16 +1082,16 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   // field into registers; stage() stores them into the current buffer and
   // synchronises.  Each stage's loads are thus in flight while the previous
   // stage's stencils are evaluated.
-  double pf[NIT];
+  R pf[NIT];
   auto prefetch = [&](auto fn) {
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
       const int l = t + it * NTH;
-      pf[it] = fn(gsrc(l < LN ? l : LN - 1));
+      pf[it] = (R)fn(gsrc(l < LN ? l : LN - 1));
     }
   };
   auto stage = [&]() {
-    double *tb = tile[buf];
+    R *tb = tile[buf];
 #pragma unroll
     for (int it = 0; it < NIT; it++) tb[t + it * NTH] = pf[it];
     __syncthreads();
@@ -730,23 +1101,23 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   auto fq = [&](int q) { return tile[buf][lbase + q * LSTR]; };
   auto wq = [&](int q) { return wtab[BND ? row : 0][q]; };
   auto D1 = [&]() {
-    if (BND && edge) return d1_rows(P, pk, DIR, wq, fq);
-    return d1<false>(P, S, DIR, at);
+    if (BND && edge) return d1_rows<R>(P, pk, DIR, wq, fq);
+    return d1<false, R>(P, S, DIR, at);
   };
   auto D2 = [&]() {
-    if (BND && edge) return d2_rows(P, pk, DIR, wq, fq);
-    return d2<false>(P, S, DIR, at);
+    if (BND && edge) return d2_rows<R>(P, pk, DIR, wq, fq);
+    return d2<false, R>(P, S, DIR, at);
   };
   auto next = [&]() { buf ^= 1; };
   const long fs = P.fstride;
   const double *Q = F.q;
-  const double *pr = F.prim;
+  const R *pr = (const R *)F.prim;
   const double *mx = Q + fs, *my = Q + 2 * fs, *mz = Q + 3 * fs;
   const double *md = DIR == 0 ? mx : (DIR == 1 ? my : mz);
-  const double *uu = pr + P_U * fs, *vv = pr + P_V * fs, *ww = pr + P_W * fs;
-  const double *ee = pr + P_E * fs, *TT = pr + P_T * fs, *pp = pr + P_P * fs;
-  const double *la = pr + P_LAM * fs, *mu = pr + P_MU * fs, *hm = pr + P_HMIX * fs;
-  const double pref = pp[0];   // PRUN(IPREF,JPREF,KPREF), src/rhsvel.f:741
+  const R *uu = pr + P_U * fs, *vv = pr + P_V * fs, *ww = pr + P_W * fs;
+  const R *ee = pr + P_E * fs, *TT = pr + P_T * fs, *pp = pr + P_P * fs;
+  const R *la = pr + P_LAM * fs, *mu = pr + P_MU * fs, *hm = pr + P_HMIX * fs;
+  const R pref = pp[0];   // PRUN(IPREF,JPREF,KPREF), src/rhsvel.f:741
   // boundary-plane flags of this direction (src/bcinit.f)
   const bool atL = BND && m == 0, atR = BND && m == n - 1;
   const int fl = 2 * DIR, fr = 2 * DIR + 1;
@@ -759,53 +1130,55 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
                         (j == 0 && P.fadb[2]) || (j == P.ny - 1 && P.fadb[3]) ||
                         (k == 0 && P.fadb[4]) || (k == P.nz - 1 && P.fadb[5]);
   // fluxes as the reference forms them: rho u*u, rho u*v, rho u*w, rho v*v, rho v*w, rho w*w
-  const double *fa[3], *fb[3];
+  const double *fa[3];
+  const R *fb[3];
   if (DIR == 0) { fa[0] = mx; fb[0] = uu; fa[1] = mx; fb[1] = vv; fa[2] = mx; fb[2] = ww; }
   else if (DIR == 1) { fa[0] = mx; fb[0] = vv; fa[1] = my; fb[1] = vv; fa[2] = my; fb[2] = ww; }
   else { fa[0] = mx; fb[0] = ww; fa[1] = my; fb[1] = ww; fa[2] = mz; fb[2] = ww; }
   auto f_flux = [&](int a) {
-    const double *A_ = fa[a], *B_ = fb[a];
+    const double *A_ = fa[a];
+    const R *B_ = fb[a];
     return [A_, B_](long g) { return A_[g] * B_[g]; };
   };
-  auto f_y = [&](int sp) { const double *Y_ = pr + prim_y(P, sp); return [Y_](long g) { return Y_[g]; }; };
+  auto f_y = [&](int sp) { const R *Y_ = pr + prim_y(P, sp); return [Y_](long g) { return Y_[g]; }; };
   // ---------------- continuity and energy convection (src/rhscal.f:141-415)
   prefetch([&](long g) { return md[g]; });
   stage();
   prefetch([&](long g) { return ee[g] * md[g]; });
-  double dm = D1(), md0 = ctr();
+  R dm = D1(), md0 = ctr();
   pin(dm); pin(md0);
   next();
-  double aD = -dm;
+  R aD = -dm;
   stage();
   prefetch([&](long g) { return ee[g]; });
-  double dem = D1();
+  R dem = D1();
   pin(dem);
   next();
   stage();
   prefetch([&](long g) { return TT[g]; });
-  const double de = D1();
-  const double e0 = ctr();
+  const R de = D1();
+  const R e0 = ctr();
   next();
-  double aE = -0.5 * (e0 * dm + dem + md0 * de);
+  R aE = -R(0.5) * (e0 * dm + dem + md0 * de);
   pin(aE);
   // ---------------- heat conduction (src/rhscal.f:427-797)
   {
     stage();
     prefetch([&](long g) { return la[g]; });
-    double dT = D1(), d2T = D2();
+    R dT = D1(), d2T = D2();
     pin(dT); pin(d2T);
     next();
     stage();
     prefetch([&](long g) { return uu[g]; });
-    const double dl = D1();
-    const double lam0 = ctr();
+    const R dl = D1();
+    const R lam0 = ctr();
     next();
-    aE = aE + (zcon ? 0.0 : dl) * dT;
-    aE = aE + lam0 * (zcon ? 0.0 : d2T);
+    aE = aE + (zcon ? R(0) : dl) * dT;
+    aE = aE + lam0 * (zcon ? R(0) : d2T);
     pin(aE);
   }
   // ---------------- velocity derivatives
-  double du[3], d2u[3], u0[3];
+  R du[3], d2u[3], u0[3];
   stage();
   prefetch([&](long g) { return vv[g]; });
   du[0] = D1(); d2u[0] = D2(); u0[0] = ctr();
@@ -822,18 +1195,18 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   pin(du[2]); pin(d2u[2]); pin(u0[2]);
   next();
   // ---------------- momentum convection (src/rhsvel.f:348-729, 1312-2110)
-  double aM[3];
+  R aM[3];
   {
-    const double r0 = active ? Q[idx] : 0.0;
+    const R r0 = active ? (R)Q[idx] : R(0);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       stage();
       if (a < 2) prefetch(f_flux(a + 1));
       else prefetch([&](long g) { return pp[g] - pref; });
-      const double flx = D1();
+      const R flx = D1();
       next();
-      if (a == DIR) aM[a] = -0.5 * (md0 * du[a] + flx + u0[a] * dm);
-      else aM[a] = -0.5 * (flx + u0[a] * dm) - 0.5 * r0 * u0[DIR] * du[a];
+      if (a == DIR) aM[a] = -R(0.5) * (md0 * du[a] + flx + u0[a] * dm);
+      else aM[a] = -R(0.5) * (flx + u0[a] * dm) - R(0.5) * r0 * u0[DIR] * du[a];
       pin(aM[a]);
     }
   }
@@ -841,8 +1214,8 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   {
     stage();
     prefetch([&](long g) { return mu[g]; });
-    const double dp = D1();
-    const double p0 = ctr() + pref;
+    const R dp = D1();
+    const R p0 = ctr() + pref + (R)P.pofs;   // stored pressure is p - pofs
     next();
     aM[DIR] = aM[DIR] - dp;
     aE = aE - u0[DIR] * dp - p0 * du[DIR];
@@ -856,64 +1229,64 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   // ---------------- separable viscous terms (src/rhsvel.f:1462-2305)
   stage();
   prefetch(f_y(0));
-  double dmu = D1(), mu0 = ctr();
+  R dmu = D1(), mu0 = ctr();
   pin(dmu); pin(mu0);
   next();
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    double tv;
-    if (a == DIR) tv = zvsn ? 0.0 : mu0 * P.fthd * d2u[a];
-    else tv = zvst ? 0.0 : mu0 * d2u[a];
+    R tv;
+    if (a == DIR) tv = zvsn ? R(0) : mu0 * (R)P.fthd * d2u[a];
+    else tv = zvst ? R(0) : mu0 * d2u[a];
     aM[a] = aM[a] + tv;
     aE = aE + tv * u0[a];
     pin(aM[a]);
   }
   pin(aE);
   if (active) {
-    double *G = F.grad + (long)DIR * 4 * fs;
+    R *G = (R *)F.grad + (long)DIR * 4 * fs;
     G[idx] = du[0];
     G[fs + idx] = du[1];
     G[2 * fs + idx] = du[2];
     G[3 * fs + idx] = dmu;
   }
   // ---------------- species (src/rhscal.f:1030-3944)
-  double vt = 0.0, ucor = 0.0;
-  double aY[NS], dYz[NS], yv[NS];
+  R vt = 0.0, ucor = 0.0;
+  R aY[NS], dYz[NS], yv[NS];
 #pragma unroll
   for (int sp = 0; sp < NS; sp++) {
-    const double *Ys = pr + prim_y(P, sp), *Rs = pr + prim_rd(P, sp), *Hs = pr + prim_h(P, sp);
+    const R *Ys = pr + prim_y(P, sp), *Rs = pr + prim_rd(P, sp), *Hs = pr + prim_h(P, sp);
     stage();
     prefetch([&](long g) { return Ys[g] * md[g]; });
-    double dY = D1(), d2Y = D2(), yc = ctr();
+    R dY = D1(), d2Y = D2(), yc = ctr();
     pin(dY); pin(d2Y); pin(yc);
     next();
     stage();
     prefetch([&](long g) { return Rs[g]; });
-    double dYm = D1();
+    R dYm = D1();
     pin(dYm);
     next();
     stage();
     prefetch([&](long g) { return Hs[g]; });
-    double drD = D1(), rdc = ctr();
+    R drD = D1(), rdc = ctr();
     pin(drD); pin(rdc);
     next();
     stage();
     if (sp + 1 < NS) prefetch(f_y(sp + 1));
     else prefetch([&](long g) { return hm[g]; });
-    const double dh = D1();
-    const double hc = ctr();
+    const R dh = D1();
+    const R hc = ctr();
     next();
-    double ys = -0.5 * yc * dm - 0.5 * dYm - 0.5 * md0 * dY;
-    const double g1 = (zdif ? 0.0 : drD) * dY;
-    const double g2 = rdc * (zdif ? 0.0 : d2Y);
+    R ys = -R(0.5) * yc * dm - R(0.5) * dYm - R(0.5) * md0 * dY;
+    const R g1 = (zdif ? R(0) : drD) * dY;
+    const R g2 = rdc * (zdif ? R(0) : d2Y);
     ys = ys + g1 + g2;
     vt = vt + g1 + g2;
-    aE = aE + hc * ((zdifadb ? 0.0 : drD) * dY);
-    aE = aE + ((zdifadb ? 0.0 : dh) * dY) * rdc;
-    aE = aE + (rdc * (zdifadb ? 0.0 : d2Y)) * hc;
+    aE = aE + hc * ((zdifadb ? R(0) : drD) * dY);
+    aE = aE + ((zdifadb ? R(0) : dh) * dY) * rdc;
+    aE = aE + (rdc * (zdifadb ? R(0) : d2Y)) * hc;
     ucor = ucor + rdc * dY;
     aY[sp] = ys;
-    dYz[sp] = zdif ? 0.0 : dY;
+    dYz[sp] = zdif ? R(0) : dY;
     yv[sp] = yc;
     pin(aY[sp]); pin(dYz[sp]); pin(yv[sp]); pin(vt); pin(ucor); pin(aE);
     if (DIR == 0 && BND && active && (i == 0 || i == P.nx - 1)) {
@@ -924,13 +1297,13 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   }
   {
     stage();
-    const double dhm = D1();
-    const double hm0 = ctr();
-    aE = aE - hm0 * (zadb_any ? 0.0 : vt) - (zdifadb ? 0.0 : dhm) * ucor;
+    const R dhm = D1();
+    const R hm0 = ctr();
+    aE = aE - hm0 * (zadb_any ? R(0) : vt) - (zdifadb ? R(0) : dhm) * ucor;
   }
   if (!active) return;
   // ---------------- store
-  double *A = F.acc;
+  R *A = (R *)F.acc;
   if (DIR == 0) {
     A[idx] = aD;
     A[fs + idx] = aM[0];
@@ -946,8 +1319,8 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   }
 #pragma unroll
   for (int sp = 0; sp < NS; sp++) {
-    const double ay = aY[sp] - yv[sp] * vt - dYz[sp] * ucor;
-    if (DIR == 0) A[(5 + sp) * fs + idx] = F.rate[sp * fs + idx] + ay;
+    const R ay = aY[sp] - yv[sp] * vt - dYz[sp] * ucor;
+    if (DIR == 0) A[(5 + sp) * fs + idx] = ((const R *)F.rate)[sp * fs + idx] + ay;
     else A[(5 + sp) * fs + idx] += ay;
   }
 }
@@ -956,7 +1329,7 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
 // Final kernel: cross-direction viscous terms (src/rhsvel.f:992-2185), NSCBC
 // face corrections (src/bounds.f:115-206, 610-682, 1196-1458) and the RK
 // register update (src/lincom.f:53-169, src/fincom.f:54-168).
-template <bool BND, int NS>
+template <bool BND, int NS, class R>
 __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__restrict__ c, Fields F, int mode,
                                                double lhs, double rhsc, double erc, double *__restrict__ out) {
   // bounded x: mixed-derivative row weights in LDS (uniform row form)
@@ -983,38 +1356,39 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
   }
   const long idx = ((long)k * P.ny + j) * P.nx + i;
   const long fs = P.fstride;
-  const uint32_t bo = (uint32_t)idx * 8u;
-  const double *pr = F.prim;
-  const double *uu = pr + P_U * fs, *vv = pr + P_V * fs, *ww = pr + P_W * fs;
-  const double u = fld(uu, bo), v = fld(vv, bo), w = fld(ww, bo), mu = fld(pr + P_MU * fs, bo);
-  const double *G = F.grad;
+  constexpr int ES = (int)sizeof(R);      // bytes per element of the FP32/FP64 fields
+  const uint32_t bo = (uint32_t)idx * (uint32_t)ES, bo8 = (uint32_t)idx * 8u;
+  const R *pr = (const R *)F.prim;
+  const R *uu = pr + P_U * fs, *vv = pr + P_V * fs, *ww = pr + P_W * fs;
+  const R u = fld(uu, bo), v = fld(vv, bo), w = fld(ww, bo), mu = fld(pr + P_MU * fs, bo);
+  const R *G = (const R *)F.grad;
   auto gr = [&](int q) { return fld(G + q * fs, bo); };
-  const double *A = F.acc;
-  double rD = fld(A, bo), rU = fld(A + fs, bo), rV = fld(A + 2 * fs, bo), rW = fld(A + 3 * fs, bo),
-         rE = fld(A + 4 * fs, bo);
+  const R *A = (const R *)F.acc;
+  R rD = fld(A, bo), rU = fld(A + fs, bo), rV = fld(A + 2 * fs, bo), rW = fld(A + 3 * fs, bo),
+    rE = fld(A + 4 * fs, bo);
   auto fz = [&](const int *fl, int dir) {
     int m = dir == 0 ? i : (dir == 1 ? j : k);
     int n = dir == 0 ? P.nx : (dir == 1 ? P.ny : P.nz);
     return (m == 0 && fl[2 * dir]) || (m == n - 1 && fl[2 * dir + 1]);
   };
-  const double fthd = P.fthd, tthd = P.tthd;
-  const double dux = gr(0), dvy = gr(5), dwz = gr(10);
-  const double dmux = gr(3), dmuy = gr(7), dmuz = gr(11);
+  const R fthd = (R)P.fthd, tthd = (R)P.tthd;
+  const R dux = gr(0), dvy = gr(5), dwz = gr(10);
+  const R dmux = gr(3), dmuy = gr(7), dmuz = gr(11);
   // normal stresses with viscosity gradients
   {
-    double sxx = fthd * dux - tthd * (dvy + dwz);
+    R sxx = fthd * dux - tthd * (dvy + dwz);
     rE = rE + mu * sxx * dux;
-    double t = fz(P.fvsn, 0) ? 0.0 : sxx * dmux;
+    R t = fz(P.fvsn, 0) ? R(0) : sxx * dmux;
     rU = rU + t;
     rE = rE + u * t;
-    double syy = fthd * dvy - tthd * (dux + dwz);
+    R syy = fthd * dvy - tthd * (dux + dwz);
     rE = rE + mu * syy * dvy;
-    t = fz(P.fvsn, 1) ? 0.0 : syy * dmuy;
+    t = fz(P.fvsn, 1) ? R(0) : syy * dmuy;
     rV = rV + t;
     rE = rE + v * t;
-    double szz = fthd * dwz - tthd * (dux + dvy);
+    R szz = fthd * dwz - tthd * (dux + dvy);
     rE = rE + mu * szz * dwz;
-    t = fz(P.fvsn, 2) ? 0.0 : szz * dmuz;
+    t = fz(P.fvsn, 2) ? R(0) : szz * dmuz;
     rW = rW + t;
     rE = rE + w * t;
   }
@@ -1022,20 +1396,20 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
   // rows stay inside) or periodic, y and z periodic.
   const int nx = P.nx, ny = P.ny, nz = P.nz;
   auto ox = [&](int a) -> int {
-    if (BND) return a * 8;
+    if (BND) return a * ES;
     int ii = i + a;
     ii += ii < 0 ? nx : (ii >= nx ? -nx : 0);
-    return (ii - i) * 8;
+    return (ii - i) * ES;
   };
   auto oy = [&](int b) -> int {
     int jj = j + b;
     jj += jj < 0 ? ny : (jj >= ny ? -ny : 0);
-    return (jj - j) * nx * 8;
+    return (jj - j) * nx * ES;
   };
   auto oz = [&](int b) -> int {
     int kk = k + b;
     kk += kk < 0 ? nz : (kk >= nz ? -nz : 0);
-    return (kk - k) * nx * ny * 8;
+    return (kk - k) * nx * ny * ES;
   };
   // x closure row of this point; waves holding any closure row use the
   // uniform row form, others the interior diagonal stencil
@@ -1052,58 +1426,58 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
   }
   Sten S0;
   S0.mode = 0;
-  auto mixx = [&](const double *f, auto od2, int d2) {
+  auto mixx = [&](const R *f, auto od2, int d2) {
     auto g = [&](int a, int b) { return fld(f, bo + (uint32_t)(ox(a) + od2(b))); };
     if (xedge)
-      return dmix_rows(P, pk, 0, d2, [&](int q) { return mtab[BND ? row * 7 + q : 0]; }, g);
-    return dmix<false>(P, S0, 0, d2, [&](int k1, int k2) { return g(k1 - 8, k2 - 8); });
+      return dmix_rows<R>(P, pk, 0, d2, [&](int q) { return mtab[BND ? row * 7 + q : 0]; }, g);
+    return dmix<false, R>(P, S0, 0, d2, [&](int k1, int k2) { return g(k1 - 8, k2 - 8); });
   };
-  auto myz = [&](const double *f) {
-    return dmix<false>(P, S0, 1, 2, [&](int k1, int k2) { return fld(f, bo + (uint32_t)(oy(k1 - 8) + oz(k2 - 8))); });
+  auto myz = [&](const R *f) {
+    return dmix<false, R>(P, S0, 1, 2, [&](int k1, int k2) { return fld(f, bo + (uint32_t)(oy(k1 - 8) + oz(k2 - 8))); });
   };
   {  // xy pair
-    const double S7 = gr(4) + gr(1);   // du/dy + dv/dx
-    double ma = mixx(uu, oy, 1), mb = mixx(vv, oy, 1);
+    const R S7 = gr(4) + gr(1);   // du/dy + dv/dx
+    R ma = mixx(uu, oy, 1), mb = mixx(vv, oy, 1);
     pin(ma); pin(mb);
-    double t = fz(P.fvsn, 0) ? 0.0 : tthd * mu * mb;
+    R t = fz(P.fvsn, 0) ? R(0) : tthd * mu * mb;
     rU = rU - t; rE = rE - u * t;
-    t = fz(P.fvsn, 1) ? 0.0 : tthd * mu * ma;
+    t = fz(P.fvsn, 1) ? R(0) : tthd * mu * ma;
     rV = rV - t; rE = rE - v * t;
-    t = fz(P.fvst, 1) ? 0.0 : mu * mb + S7 * dmuy;
+    t = fz(P.fvst, 1) ? R(0) : mu * mb + S7 * dmuy;
     rU = rU + t; rE = rE + u * t;
-    t = fz(P.fvst, 0) ? 0.0 : mu * ma + S7 * dmux;
+    t = fz(P.fvst, 0) ? R(0) : mu * ma + S7 * dmux;
     rV = rV + t; rE = rE + v * t;
     rE = rE + mu * S7 * S7;
   }
   {  // xz pair
-    const double S7 = gr(8) + gr(2);   // du/dz + dw/dx
-    double ma = mixx(uu, oz, 2), mb = mixx(ww, oz, 2);
+    const R S7 = gr(8) + gr(2);   // du/dz + dw/dx
+    R ma = mixx(uu, oz, 2), mb = mixx(ww, oz, 2);
     pin(ma); pin(mb);
-    double t = fz(P.fvsn, 0) ? 0.0 : tthd * mu * mb;
+    R t = fz(P.fvsn, 0) ? R(0) : tthd * mu * mb;
     rU = rU - t; rE = rE - u * t;
-    t = fz(P.fvsn, 2) ? 0.0 : tthd * mu * ma;
+    t = fz(P.fvsn, 2) ? R(0) : tthd * mu * ma;
     rW = rW - t; rE = rE - w * t;
-    t = fz(P.fvst, 2) ? 0.0 : mu * mb + S7 * dmuz;
+    t = fz(P.fvst, 2) ? R(0) : mu * mb + S7 * dmuz;
     rU = rU + t; rE = rE + u * t;
-    t = fz(P.fvst, 0) ? 0.0 : mu * ma + S7 * dmux;
+    t = fz(P.fvst, 0) ? R(0) : mu * ma + S7 * dmux;
     rW = rW + t; rE = rE + w * t;
     rE = rE + mu * S7 * S7;
   }
   {  // yz pair
-    const double S7 = gr(9) + gr(6);   // dv/dz + dw/dy
-    double ma = myz(vv), mb = myz(ww);
+    const R S7 = gr(9) + gr(6);   // dv/dz + dw/dy
+    R ma = myz(vv), mb = myz(ww);
     pin(ma); pin(mb);
-    double t = fz(P.fvsn, 1) ? 0.0 : tthd * mu * mb;
+    R t = fz(P.fvsn, 1) ? R(0) : tthd * mu * mb;
     rV = rV - t; rE = rE - v * t;
-    t = fz(P.fvsn, 2) ? 0.0 : tthd * mu * ma;
+    t = fz(P.fvsn, 2) ? R(0) : tthd * mu * ma;
     rW = rW - t; rE = rE - w * t;
-    t = fz(P.fvst, 2) ? 0.0 : mu * mb + S7 * dmuz;
+    t = fz(P.fvst, 2) ? R(0) : mu * mb + S7 * dmuz;
     rV = rV + t; rE = rE + v * t;
-    t = fz(P.fvst, 1) ? 0.0 : mu * ma + S7 * dmuy;
+    t = fz(P.fvst, 1) ? R(0) : mu * ma + S7 * dmuy;
     rW = rW + t; rE = rE + w * t;
     rE = rE + mu * S7 * S7;
   }
-  double rY[NS];
+  R rY[NS];
 #pragma unroll
   for (int s = 0; s < NS; s++) rY[s] = fld(A + (5 + s) * fs, bo);
   // ---------------- NSCBC faces
@@ -1142,21 +1516,21 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
       rE = rE - bcl2 * stre - bcl3 * strd * v - bcl4 * strd * w - bcl5 * (ova2 * stre + u / acou + ovgm);
 #pragma unroll
       for (int s = 0; s < NS; s++) {
-        double bcly = F.rate[s * fs + idx] / strd - 0.0 - u * fc[(F_NSCAL + s) * nf + fq];
+        double bcly = ((const R *)F.rate)[s * fs + idx] / strd - 0.0 - u * fc[(F_NSCAL + s) * nf + fq];
         rE = rE - bcly * strd * strh[s];
       }
     }
     if (i == P.nx - 1 && P.outflow_xr) {
       // NSBCO1, src/bounds.f:1296-1458 (pointwise in/outflow switch and local-Mach beta off)
       const double bet = 0.050;
-      const double strp = pr[P_P * fs + idx];
+      const double strp = (double)pr[P_P * fs + idx] + P.pofs;
       const double t51 = -v * fc[F_DPDY * nf + fq] - w * fc[F_DPDZ * nf + fq];
       const double t52 = -strp * (dvy + dwz);
       const double t2b = -v * gr(4) - w * gr(8);
       const double tt1 = t51 + t52 * (gam1 + 1.0) - strd * acou * t2b;
       double sorp = 0.0;
 #pragma unroll
-      for (int s = 0; s < NS; s++) sorp = sorp + strh[s] * F.rate[s * fs + idx];
+      for (int s = 0; s < NS; s++) sorp = sorp + strh[s] * ((const R *)F.rate)[s * fs + idx];
       sorp = -sorp * gam1;
       double bcl1 = 0.5 * (u - acou) * (fc[F_DPDX * nf + fq] - strd * acou * dux);
       bcl1 = 0.5 * sorp + P.cobcxr * acou * (strp - P.pinfxr) + 0.5 * (1.0 - bet) * tt1 - bcl1;
@@ -1170,13 +1544,13 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
     }
   }
   // ---------------- register update
-  double R[5 + NS];
-  R[0] = rD; R[1] = rU; R[2] = rV; R[3] = rW; R[4] = rE;
+  R Rv[5 + NS];
+  Rv[0] = rD; Rv[1] = rU; Rv[2] = rV; Rv[3] = rW; Rv[4] = rE;
 #pragma unroll
-  for (int s = 0; s < NS; s++) R[5 + s] = rY[s];
+  for (int s = 0; s < NS; s++) Rv[5 + s] = rY[s];
   if (mode == 2) {
 #pragma unroll
-    for (int vv2 = 0; vv2 < 5 + NS; vv2++) out[vv2 * fs + idx] = R[vv2];
+    for (int vv2 = 0; vv2 < 5 + NS; vv2++) out[vv2 * fs + idx] = Rv[vv2];
     return;
   }
 #pragma unroll
@@ -1184,18 +1558,18 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
     const int slot = vv2 < 5 ? vv2 : 5;
     double *runv = F.run + vv2 * fs, *qv = F.q + vv2 * fs;
     if (i >= P.rklo[slot]) {
-      if (F.erk) fld(F.erk + vv2 * fs, bo) += erc * R[vv2];   // *ERR += RKERR * R
-      double old = fld(runv, bo);
+      if (F.erk) fld(F.erk + vv2 * fs, bo8) += erc * (double)Rv[vv2];   // *ERR += RKERR * R
+      double old = fld(runv, bo8);
       if (mode == 0) {
-        fld(runv, bo) = old + lhs * R[vv2];
-        fld(qv, bo) = old + rhsc * R[vv2];
+        fld(runv, bo8) = old + lhs * (double)Rv[vv2];
+        fld(qv, bo8) = old + rhsc * (double)Rv[vv2];
       } else {
-        double nw = old + lhs * R[vv2];
-        fld(runv, bo) = nw;
-        fld(qv, bo) = nw;
+        double nw = old + lhs * (double)Rv[vv2];
+        fld(runv, bo8) = nw;
+        fld(qv, bo8) = nw;
       }
     } else {
-      fld(qv, bo) = R[vv2];
+      fld(qv, bo8) = (double)Rv[vv2];
     }
   }
 }
@@ -1231,7 +1605,8 @@ static inline unsigned grid_for(long n) { return (unsigned)((n + BLK - 1) / BLK)
 
 template <int NS>
 void launch_tempin_t(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st) {
-  hipLaunchKernelGGL(k_tempin<NS>, dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F);
+  if (P.precision) hipLaunchKernelGGL((k_tempin<NS, float>), dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F);
+  else hipLaunchKernelGGL((k_tempin<NS, double>), dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F);
 }
 template <int NS>
 void launch_boundt_t(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st) {
@@ -1243,7 +1618,8 @@ void launch_bountt_t(const Params &P, const sg_config *dcfg, const Fields &F, hi
 }
 template <int NS>
 void launch_prim_t(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st) {
-  hipLaunchKernelGGL(k_prim<NS>, dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F.tab, F);
+  if (P.precision) hipLaunchKernelGGL(k_prim_mx<NS>, dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F.tab, F);
+  else hipLaunchKernelGGL(k_prim<NS>, dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F.tab, F);
 }
 template <int DIR>
 static dim3 pass_grid(const Params &P) {
@@ -1253,24 +1629,34 @@ static dim3 pass_grid(const Params &P) {
   return dim3(gx, gy, gz);
 }
 
-template <int NS>
-void launch_pass_t(int dir, const Params &P, const Fields &F, hipStream_t st) {
+template <int NS, class R>
+static void launch_pass_r(int dir, const Params &P, const Fields &F, hipStream_t st) {
   if (dir == 0) {
     dim3 b(PassTile<0>::NT);
-    if (P.bnd[0]) hipLaunchKernelGGL((k_pass<0, true, NS>), pass_grid<0>(P), b, 0, st, P, F);
-    else hipLaunchKernelGGL((k_pass<0, false, NS>), pass_grid<0>(P), b, 0, st, P, F);
+    if (P.bnd[0]) hipLaunchKernelGGL((k_pass<0, true, NS, R>), pass_grid<0>(P), b, 0, st, P, F);
+    else hipLaunchKernelGGL((k_pass<0, false, NS, R>), pass_grid<0>(P), b, 0, st, P, F);
   } else if (dir == 1) {
-    hipLaunchKernelGGL((k_pass<1, false, NS>), pass_grid<1>(P), dim3(PassTile<1>::NT), 0, st, P, F);
+    hipLaunchKernelGGL((k_pass<1, false, NS, R>), pass_grid<1>(P), dim3(PassTile<1>::NT), 0, st, P, F);
   } else {
-    hipLaunchKernelGGL((k_pass<2, false, NS>), pass_grid<2>(P), dim3(PassTile<2>::NT), 0, st, P, F);
+    hipLaunchKernelGGL((k_pass<2, false, NS, R>), pass_grid<2>(P), dim3(PassTile<2>::NT), 0, st, P, F);
   }
+}
+template <int NS>
+void launch_pass_t(int dir, const Params &P, const Fields &F, hipStream_t st) {
+  if (P.precision) launch_pass_r<NS, float>(dir, P, F, st);
+  else launch_pass_r<NS, double>(dir, P, F, st);
 }
 template <int NS>
 void launch_final_t(const Params &P, const sg_config *dcfg, const Fields &F, int mode, double lhs,
                     double rhs, double erc, double *out, hipStream_t st) {
   dim3 g(grid_for(P.npts)), b(BLK);
-  if (P.bnd[0]) hipLaunchKernelGGL((k_final<true, NS>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, erc, out);
-  else hipLaunchKernelGGL((k_final<false, NS>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, erc, out);
+  if (P.precision) {
+    if (P.bnd[0]) hipLaunchKernelGGL((k_final<true, NS, float>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, erc, out);
+    else hipLaunchKernelGGL((k_final<false, NS, float>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, erc, out);
+  } else {
+    if (P.bnd[0]) hipLaunchKernelGGL((k_final<true, NS, double>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, erc, out);
+    else hipLaunchKernelGGL((k_final<false, NS, double>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, erc, out);
+  }
 }
 
 #ifdef SG_NS

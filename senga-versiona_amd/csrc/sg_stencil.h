@@ -45,99 +45,99 @@ __device__ __forceinline__ Sten make_sten(int m, int n, int s) {
 // f(k) returns the value at offset index k (q = k - 8).
 #define FQ(q) f((q) + 8)
 
-template <bool BND, class Fn>
-__device__ __forceinline__ double d1(const Params &P, const Sten &S, int dir, Fn f) {
-  double r;
+template <bool BND, class R = double, class Fn>
+__device__ __forceinline__ R d1(const Params &P, const Sten &S, int dir, Fn f) {
+  R r;
   if (!BND || S.mode == 0) {
     double fa = FQ(1) - FQ(-1), fb = FQ(2) - FQ(-2), fc = FQ(3) - FQ(-3), fd = FQ(4) - FQ(-4),
            fe = FQ(5) - FQ(-5);
-    r = P.ac[0] * fa + P.ac[1] * fb + P.ac[2] * fc + P.ac[3] * fd + P.ac[4] * fe;
+    r = (R)P.ac[0] * fa + (R)P.ac[1] * fb + (R)P.ac[2] * fc + (R)P.ac[3] * fd + (R)P.ac[4] * fe;
   } else {
     switch (S.mode) {
       case 1: {
         double fa = FQ(1) - FQ(0), fb = FQ(2) - FQ(0), fc = FQ(3) - FQ(0), fd = FQ(4) - FQ(0);
-        r = P.c1[0] * fa + P.c1[1] * fb + P.c1[2] * fc + P.c1[3] * fd;
+        r = (R)P.c1[0] * fa + (R)P.c1[1] * fb + (R)P.c1[2] * fc + (R)P.c1[3] * fd;
       } break;
       case 2: {
         double fa = FQ(-1) - FQ(0), fb = FQ(1) - FQ(0), fc = FQ(2) - FQ(0), fd = FQ(3) - FQ(0);
-        r = P.c2[0] * fa + P.c2[1] * fb + P.c2[2] * fc + P.c2[3] * fd;
+        r = (R)P.c2[0] * fa + (R)P.c2[1] * fb + (R)P.c2[2] * fc + (R)P.c2[3] * fd;
       } break;
       case 3:
       case 13: {
         double fa = FQ(1) - FQ(-1), fb = FQ(2) - FQ(-2);
-        r = P.c3[0] * fa + P.c3[1] * fb;
+        r = (R)P.c3[0] * fa + (R)P.c3[1] * fb;
       } break;
       case 4:
       case 14: {
         double fa = FQ(1) - FQ(-1), fb = FQ(2) - FQ(-2), fc = FQ(3) - FQ(-3);
-        r = P.c4[0] * fa + P.c4[1] * fb + P.c4[2] * fc;
+        r = (R)P.c4[0] * fa + (R)P.c4[1] * fb + (R)P.c4[2] * fc;
       } break;
       case 5:
       case 15: {
         double fa = FQ(1) - FQ(-1), fb = FQ(2) - FQ(-2), fc = FQ(3) - FQ(-3), fd = FQ(4) - FQ(-4);
-        r = P.c5[0] * fa + P.c5[1] * fb + P.c5[2] * fc + P.c5[3] * fd;
+        r = (R)P.c5[0] * fa + (R)P.c5[1] * fb + (R)P.c5[2] * fc + (R)P.c5[3] * fd;
       } break;
       case 12: {
         double fa = FQ(0) - FQ(1), fb = FQ(0) - FQ(-1), fc = FQ(0) - FQ(-2), fd = FQ(0) - FQ(-3);
-        r = P.c2[0] * fa + P.c2[1] * fb + P.c2[2] * fc + P.c2[3] * fd;
+        r = (R)P.c2[0] * fa + (R)P.c2[1] * fb + (R)P.c2[2] * fc + (R)P.c2[3] * fd;
       } break;
       default: {  // 11: last point
         double fa = FQ(0) - FQ(-1), fb = FQ(0) - FQ(-2), fc = FQ(0) - FQ(-3), fd = FQ(0) - FQ(-4);
-        r = P.c1[0] * fa + P.c1[1] * fb + P.c1[2] * fc + P.c1[3] * fd;
+        r = (R)P.c1[0] * fa + (R)P.c1[1] * fb + (R)P.c1[2] * fc + (R)P.c1[3] * fd;
       } break;
     }
   }
-  return r * P.ovdel[dir];
+  return r * (R)P.ovdel[dir];
 }
 
-template <bool BND, class Fn>
-__device__ __forceinline__ double d2(const Params &P, const Sten &S, int dir, Fn f) {
-  double r;
+template <bool BND, class R = double, class Fn>
+__device__ __forceinline__ R d2(const Params &P, const Sten &S, int dir, Fn f) {
+  R r;
   if (!BND || S.mode == 0) {
     double c = FQ(0);
     double ap = FQ(1) - c, am = c - FQ(-1), bp = FQ(2) - c, bm = c - FQ(-2), cp = FQ(3) - c,
            cm = c - FQ(-3), dp = FQ(4) - c, dm = c - FQ(-4), ep = FQ(5) - c, em = c - FQ(-5);
-    r = P.sc[0] * (ap - am) + P.sc[1] * (bp - bm) + P.sc[2] * (cp - cm) + P.sc[3] * (dp - dm) +
-        P.sc[4] * (ep - em);
+    r = (R)P.sc[0] * (ap - am) + (R)P.sc[1] * (bp - bm) + (R)P.sc[2] * (cp - cm) + (R)P.sc[3] * (dp - dm) +
+        (R)P.sc[4] * (ep - em);
   } else {
     double c = FQ(0);
     switch (S.mode) {
       case 1: {
         double ap = FQ(1) - c, bp = FQ(2) - c, cp = FQ(3) - c, dp = FQ(4) - c, ep = FQ(5) - c;
-        r = P.s1[0] * ap + P.s1[1] * bp + P.s1[2] * cp + P.s1[3] * dp + P.s1[4] * ep;
+        r = (R)P.s1[0] * ap + (R)P.s1[1] * bp + (R)P.s1[2] * cp + (R)P.s1[3] * dp + (R)P.s1[4] * ep;
       } break;
       case 2: {
         double ap = FQ(-1) - c, bp = FQ(1) - c, cp = FQ(2) - c, dp = FQ(3) - c, ep = FQ(4) - c;
-        r = P.s2[0] * ap + P.s2[1] * bp + P.s2[2] * cp + P.s2[3] * dp + P.s2[4] * ep;
+        r = (R)P.s2[0] * ap + (R)P.s2[1] * bp + (R)P.s2[2] * cp + (R)P.s2[3] * dp + (R)P.s2[4] * ep;
       } break;
       case 3:
       case 13: {
         double ap = FQ(1) - c, am = c - FQ(-1), bp = FQ(2) - c, bm = c - FQ(-2);
-        r = P.s3[0] * (ap - am) + P.s3[1] * (bp - bm);
+        r = (R)P.s3[0] * (ap - am) + (R)P.s3[1] * (bp - bm);
       } break;
       case 4:
       case 14: {
         double ap = FQ(1) - c, am = c - FQ(-1), bp = FQ(2) - c, bm = c - FQ(-2), cp = FQ(3) - c,
                cm = c - FQ(-3);
-        r = P.s4[0] * (ap - am) + P.s4[1] * (bp - bm) + P.s4[2] * (cp - cm);
+        r = (R)P.s4[0] * (ap - am) + (R)P.s4[1] * (bp - bm) + (R)P.s4[2] * (cp - cm);
       } break;
       case 5:
       case 15: {
         double ap = FQ(1) - c, am = c - FQ(-1), bp = FQ(2) - c, bm = c - FQ(-2), cp = FQ(3) - c,
                cm = c - FQ(-3), dp = FQ(4) - c, dm = c - FQ(-4);
-        r = P.s5[0] * (ap - am) + P.s5[1] * (bp - bm) + P.s5[2] * (cp - cm) + P.s5[3] * (dp - dm);
+        r = (R)P.s5[0] * (ap - am) + (R)P.s5[1] * (bp - bm) + (R)P.s5[2] * (cp - cm) + (R)P.s5[3] * (dp - dm);
       } break;
       case 12: {
         double ap = FQ(1) - c, bp = FQ(-1) - c, cp = FQ(-2) - c, dp = FQ(-3) - c, ep = FQ(-4) - c;
-        r = P.s2[0] * ap + P.s2[1] * bp + P.s2[2] * cp + P.s2[3] * dp + P.s2[4] * ep;
+        r = (R)P.s2[0] * ap + (R)P.s2[1] * bp + (R)P.s2[2] * cp + (R)P.s2[3] * dp + (R)P.s2[4] * ep;
       } break;
       default: {
         double ap = FQ(-1) - c, bp = FQ(-2) - c, cp = FQ(-3) - c, dp = FQ(-4) - c, ep = FQ(-5) - c;
-        r = P.s1[0] * ap + P.s1[1] * bp + P.s1[2] * cp + P.s1[3] * dp + P.s1[4] * ep;
+        r = (R)P.s1[0] * ap + (R)P.s1[1] * bp + (R)P.s1[2] * cp + (R)P.s1[3] * dp + (R)P.s1[4] * ep;
       } break;
     }
   }
-  return r * P.ovdl2[dir];
+  return r * (R)P.ovdl2[dir];
 }
 
 #undef FQ
@@ -208,20 +208,20 @@ __device__ __forceinline__ int row_off(const int packed[5], int e) {
 
 // First/second derivative in the 5-term form; f(q) gives the value at offset
 // q, w the row's 10 weights.
-template <class Fn, class Wn>
-__device__ __forceinline__ double d1_rows(const Params &P, const int packed[5], int dir, Wn w, Fn f) {
-  double r = w(0) * (f(row_off(packed, 0)) - f(row_off(packed, 5)));
+template <class R = double, class Fn, class Wn>
+__device__ __forceinline__ R d1_rows(const Params &P, const int packed[5], int dir, Wn w, Fn f) {
+  R r = (R)w(0) * (f(row_off(packed, 0)) - f(row_off(packed, 5)));
 #pragma unroll
-  for (int k = 1; k < 5; k++) r = r + w(k) * (f(row_off(packed, k)) - f(row_off(packed, 5 + k)));
-  return r * P.ovdel[dir];
+  for (int k = 1; k < 5; k++) r = r + (R)w(k) * (f(row_off(packed, k)) - f(row_off(packed, 5 + k)));
+  return r * (R)P.ovdel[dir];
 }
-template <class Fn, class Wn>
-__device__ __forceinline__ double d2_rows(const Params &P, const int packed[5], int dir, Wn w, Fn f) {
-  const double c = f(0);
-  double r = w(5) * ((f(row_off(packed, 10)) - c) - (c - f(row_off(packed, 15))));
+template <class R = double, class Fn, class Wn>
+__device__ __forceinline__ R d2_rows(const Params &P, const int packed[5], int dir, Wn w, Fn f) {
+  const R c = f(0);
+  R r = (R)w(5) * ((f(row_off(packed, 10)) - c) - (c - f(row_off(packed, 15))));
 #pragma unroll
-  for (int k = 1; k < 5; k++) r = r + w(5 + k) * ((f(row_off(packed, 10 + k)) - c) - (c - f(row_off(packed, 15 + k))));
-  return r * P.ovdl2[dir];
+  for (int k = 1; k < 5; k++) r = r + (R)w(5 + k) * ((f(row_off(packed, 10 + k)) - c) - (c - f(row_off(packed, 15 + k))));
+  return r * (R)P.ovdl2[dir];
 }
 
 // ---------------------------------------------------------------------------
@@ -281,20 +281,20 @@ inline void mix_row_weights(const Params &P, int row, double *w) {
 
 // Mixed derivative in the uniform row form; g(a, b) gives the value at offset
 // a along d1 and b along d2, w(k) the row's 7 weights, packed its offsets.
-template <class Gn, class Wn>
-__device__ __forceinline__ double dmix_rows(const Params &P, const int packed[5], int d1, int d2, Wn w, Gn g) {
+template <class R = double, class Gn, class Wn>
+__device__ __forceinline__ R dmix_rows(const Params &P, const int packed[5], int d1, int d2, Wn w, Gn g) {
   auto Qv = [&](int p, int m, int q) { return g(p, q) - g(p, -q) - g(m, q) + g(m, -q); };
-  const double v0 = w(5), v1 = w(6);
-  double r = 0.0;
+  const R v0 = (R)w(5), v1 = (R)w(6);
+  R r = 0.0;
 #pragma unroll
   for (int k = 0; k < 5; k++) {
     const int pk = packed[k];
     const int p = __builtin_amdgcn_sbfe(pk, 0, 8), m = __builtin_amdgcn_sbfe(pk, 8, 8);
     const int q0 = __builtin_amdgcn_sbfe(pk, 16, 8), q1 = __builtin_amdgcn_sbfe(pk, 24, 8);
-    const double t = w(k) * (v0 * Qv(p, m, q0) + v1 * Qv(p, m, q1));
+    const R t = (R)w(k) * (v0 * Qv(p, m, q0) + v1 * Qv(p, m, q1));
     r = k == 0 ? t : r + t;
   }
-  return r * P.ovdel[d1] * P.ovdel[d2];
+  return r * (R)P.ovdel[d1] * (R)P.ovdel[d2];
 }
 
 // Mixed derivative in the (d1,d2) plane; d2 periodic, d1 bounded if BND.
@@ -302,51 +302,51 @@ __device__ __forceinline__ double dmix_rows(const Params &P, const int packed[5]
 #define G(a, b) g((a) + 8, (b) + 8)
 #define DIAG(q, b) (G(q, b) - G(q, -(b)) - G(-(q), b) + G(-(q), -(b)))
 #define TR(a1, b1)                                                       \
-  (P.my1[0] * (G(a1, 1) - G(a1, -1) - G(b1, 1) + G(b1, -1)) +            \
-   P.my1[1] * (G(a1, 2) - G(a1, -2) - G(b1, 2) + G(b1, -2)))
+  ((R)P.my1[0] * (G(a1, 1) - G(a1, -1) - G(b1, 1) + G(b1, -1)) +            \
+   (R)P.my1[1] * (G(a1, 2) - G(a1, -2) - G(b1, 2) + G(b1, -2)))
 
-template <bool BND, class Gn>
-__device__ __forceinline__ double dmix(const Params &P, const Sten &S1, int d1, int d2, Gn g) {
-  double r;
+template <bool BND, class R = double, class Gn>
+__device__ __forceinline__ R dmix(const Params &P, const Sten &S1, int d1, int d2, Gn g) {
+  R r;
   if (!BND || S1.mode == 0) {
     double fa = DIAG(1, 1), fb = DIAG(2, 2), fc = DIAG(3, 3), fd = DIAG(4, 4), fe = DIAG(5, 5);
-    r = P.mc[0] * fa + P.mc[1] * fb + P.mc[2] * fc + P.mc[3] * fd + P.mc[4] * fe;
+    r = (R)P.mc[0] * fa + (R)P.mc[1] * fb + (R)P.mc[2] * fc + (R)P.mc[3] * fd + (R)P.mc[4] * fe;
   } else {
     switch (S1.mode) {
       case 1: {
         double fa = TR(1, 0), fb = TR(2, 0), fc = TR(3, 0), fd = TR(4, 0);
-        r = P.m1[0] * fa + P.m1[1] * fb + P.m1[2] * fc + P.m1[3] * fd;
+        r = (R)P.m1[0] * fa + (R)P.m1[1] * fb + (R)P.m1[2] * fc + (R)P.m1[3] * fd;
       } break;
       case 2: {
         double fa = TR(-1, 0), fb = TR(1, 0), fc = TR(2, 0), fd = TR(3, 0);
-        r = P.m2[0] * fa + P.m2[1] * fb + P.m2[2] * fc + P.m2[3] * fd;
+        r = (R)P.m2[0] * fa + (R)P.m2[1] * fb + (R)P.m2[2] * fc + (R)P.m2[3] * fd;
       } break;
       case 3:
       case 13: {
         double fa = DIAG(1, 1), fb = DIAG(2, 2);
-        r = P.m3[0] * fa + P.m3[1] * fb;
+        r = (R)P.m3[0] * fa + (R)P.m3[1] * fb;
       } break;
       case 4:
       case 14: {
         double fa = DIAG(1, 1), fb = DIAG(2, 2), fc = DIAG(3, 3);
-        r = P.m4[0] * fa + P.m4[1] * fb + P.m4[2] * fc;
+        r = (R)P.m4[0] * fa + (R)P.m4[1] * fb + (R)P.m4[2] * fc;
       } break;
       case 5:
       case 15: {
         double fa = DIAG(1, 1), fb = DIAG(2, 2), fc = DIAG(3, 3), fd = DIAG(4, 4);
-        r = P.m5[0] * fa + P.m5[1] * fb + P.m5[2] * fc + P.m5[3] * fd;
+        r = (R)P.m5[0] * fa + (R)P.m5[1] * fb + (R)P.m5[2] * fc + (R)P.m5[3] * fd;
       } break;
       case 12: {
         double fa = TR(0, 1), fb = TR(0, -1), fc = TR(0, -2), fd = TR(0, -3);
-        r = P.m2[0] * fa + P.m2[1] * fb + P.m2[2] * fc + P.m2[3] * fd;
+        r = (R)P.m2[0] * fa + (R)P.m2[1] * fb + (R)P.m2[2] * fc + (R)P.m2[3] * fd;
       } break;
       default: {
         double fa = TR(0, -1), fb = TR(0, -2), fc = TR(0, -3), fd = TR(0, -4);
-        r = P.m1[0] * fa + P.m1[1] * fb + P.m1[2] * fc + P.m1[3] * fd;
+        r = (R)P.m1[0] * fa + (R)P.m1[1] * fb + (R)P.m1[2] * fc + (R)P.m1[3] * fd;
       } break;
     }
   }
-  return r * P.ovdel[d1] * P.ovdel[d2];
+  return r * (R)P.ovdel[d1] * (R)P.ovdel[d2];
 }
 
 #undef TR

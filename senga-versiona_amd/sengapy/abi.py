@@ -67,8 +67,9 @@ class Config(C.Structure):
     ]
 
 
-def config_from_case(case) -> Config:
-    """Fill sg_config from a Case (control + derived tables)."""
+def config_from_case(case, precision=0) -> Config:
+    """Fill sg_config from a Case (control + derived tables); precision 0 is
+    FP64, 1 the mixed FP32/FP64 mode."""
     c = Config()
     ctl, t, m = case.control, case.tables, case.tables.mech
     ns, nst = t.nspec, t.nstep
@@ -77,7 +78,7 @@ def config_from_case(case) -> Config:
     c.nx, c.ny, c.nz = ctl.nx, ctl.ny, ctl.nz
     c.nspec, c.nstep = ns, nst
     c.mixavg = 1 if ctl.nfmavt == 1 else 0
-    c.precision = 0
+    c.precision = precision
     c.xgdlen, c.ygdlen, c.zgdlen = ctl.xgdlen, ctl.ygdlen, ctl.zgdlen
     for f, (typ, ip, rp) in enumerate(ctl.bc):
         c.bc_type[f] = typ
@@ -213,12 +214,12 @@ class SgError(RuntimeError):
 class Solver:
     """One case on the current GPU through the C ABI."""
 
-    def __init__(self, case, lib=None):
+    def __init__(self, case, lib=None, precision=0):
         self.lib = lib or load_library()
         if self.lib.sg_config_size() != C.sizeof(Config):
             raise RuntimeError("sg_config layout mismatch between library and binding")
         self.case = case
-        self.cfg = config_from_case(case)
+        self.cfg = config_from_case(case, precision)
         h = C.c_void_p()
         rc = self.lib.sg_create(C.byref(self.cfg), C.byref(h))
         if rc != 0:

@@ -159,3 +159,35 @@ def test_adaptive_step_matches_reference():
     o.set_state(c.dump.state, 0.0)
     o.step(int(fx["nsteps"]))
     assert abs(o.timestep()[0] - tstep) <= 1e-7 * tstep
+
+
+@pytest.mark.parametrize("kind,n,steps", [("h2_nscbc_mix", (40, 24, 24), 20), ("periodic_1step", (32, 32, 32), 20)])
+def test_mixed_precision_against_fp64(kind, n, steps):
+    """Mixed mode (FP32 primitives, transport, chemistry, derivatives and RHS;
+    FP64 registers and temperature iteration) against the FP64 path: max
+    relative error per field (normalised by the field's maximum) after
+    `steps` steps.  Bar: 1e-5 on T and Y_k (the mixed-precision target), and
+    on the other conserved fields."""
+    from sengapy import case as C
+    from sengapy.abi import Solver
+    print("case", kind, n, steps, flush=True)
+    if kind == "periodic_1step":
+        c = C.periodic_1step(n=n, nsteps=steps)
+    else:
+        c = C.h2_nscbc(n=n, length=(4e-3, 2.4e-3, 2.4e-3), mixavg=True, nsteps=steps)
+    out = {}
+    for prec in (0, 1):
+        s = Solver(c, precision=prec)
+        s.set_timestep(c.dump.tstep)
+        s.upload(c.dump.state, 0.0)
+        s.step(steps)
+        out[prec] = s.download()
+        s.close()
+    run64, T64 = out[0][0], out[0][1]
+    run32, T32 = out[1][0], out[1][1]
+    errs = [rel_err(run32[v], run64[v]) for v in range(run64.shape[0])]
+    _report("mixed vs fp64 run", errs)
+    eT = rel_err(T32, T64)
+    print("T", f"{eT:.1e}", flush=True)
+    assert eT < 1e-5
+    assert max(errs) < 1e-5
