@@ -118,7 +118,12 @@ def mixed_run(abi, c, a, h, s64):
     s.sync()
     ms = C.c_float(0)
     h.hipEventElapsedTime(C.byref(ms), ev0, ev1)
-    s.step(a.profile_steps)        # the FP64 run's profiled steps, so both states are at the same step
+    # the FP64 run's profiled steps, so both states are at the same step
+    s.profile(True)
+    s.step(a.profile_steps)
+    s.sync()
+    prof = s.profile_read()
+    s.profile(False)
     r32, T32, _, _ = s.download()
     r64, T64, _, _ = s64.download()
 
@@ -133,7 +138,8 @@ def mixed_run(abi, c, a, h, s64):
             "dtype": "f32 compute / f64 registers",
             "max_rel_err_T": rel(T32, T64), "max_rel_err_Y": max(rel(r32[5 + k], r64[5 + k]) for k in range(ns)),
             "max_rel_err_conserved": max(rel(r32[v], r64[v]) for v in range(5 + ns)),
-            "steps_compared": a.warmup + a.steps + a.profile_steps}
+            "steps_compared": a.warmup + a.steps + a.profile_steps,
+            "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in prof.items() if v[1] > 0}}
 
 
 def main():

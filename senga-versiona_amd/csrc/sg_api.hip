@@ -104,6 +104,29 @@ static void build_tables(sg_config &c, Tables &t) {
       }
     }
   }
+  for (int s = 0; s < SG_MAX_SPEC; s++) {
+    t.ovwf[s] = (float)c.ovwmol[s];
+    t.olewf[s] = (float)c.olewis[s];
+    for (int m = 0; m < SG_NTRC; m++) {
+      t.condf[s][m] = (float)c.condco[s][m];
+      t.viscf[s][m] = (float)c.viscco[s][m];
+      for (int j = 0; j < SG_MAX_SPEC; j++) t.difff[s][j][m] = (float)c.diffco[s][j][m];
+    }
+    for (int j = 0; j < SG_MAX_SPEC; j++) {
+      t.wilk1f[s][j] = (float)c.wilko1[s][j];
+      t.wilk2f[s][j] = (float)c.wilko2[s][j];
+    }
+  }
+  for (int b = 0; b < SG_MAX_BODY; b++)
+    for (int s = 0; s < SG_MAX_SPEC; s++) t.effyf[b][s] = (float)c.effy3b[b][s];
+  for (int st = 0; st < SG_MAX_STEP; st++) {
+    for (int s = 0; s < SG_MAX_SPEC; s++) t.nuwf[st][s] = (float)t.nuw[st][s];
+    for (int m = 0; m < SG_MAX_SSL; m++) {
+      t.diffmwf[st][m] = (float)c.diffmw[st][m];
+      t.crspecf[st][m] = (float)c.crspec[st][m];
+      t.cpspecf[st][m] = (float)c.cpspec[st][m];
+    }
+  }
   t.diffsym = 1;
   for (int i = 0; i < c.nspec; i++)
     for (int j = i + 1; j < c.nspec; j++)

@@ -58,6 +58,16 @@ struct Tables {
   double nuw[SG_MAX_STEP][SG_MAX_SPEC];
   int diffsym;                                   // binary diffusion fits symmetric in (i, j)
   int ratedense;                                 // nuw is usable
+  // FP32 copies of the tables the mixed-precision kernels read, so that
+  // uniform coefficients arrive as scalar FP32 loads instead of per-lane
+  // FP64 -> FP32 conversions
+  float ovwf[SG_MAX_SPEC], olewf[SG_MAX_SPEC];
+  float condf[SG_MAX_SPEC][SG_NTRC], viscf[SG_MAX_SPEC][SG_NTRC];
+  float difff[SG_MAX_SPEC][SG_MAX_SPEC][SG_NTRC];
+  float wilk1f[SG_MAX_SPEC][SG_MAX_SPEC], wilk2f[SG_MAX_SPEC][SG_MAX_SPEC];
+  float effyf[SG_MAX_BODY][SG_MAX_SPEC];
+  float nuwf[SG_MAX_STEP][SG_MAX_SPEC];
+  float diffmwf[SG_MAX_STEP][SG_MAX_SSL], crspecf[SG_MAX_STEP][SG_MAX_SSL], cpspecf[SG_MAX_STEP][SG_MAX_SSL];
 };
 
 // Device field set: all arrays are [field][z][y][x] with field stride fstride.

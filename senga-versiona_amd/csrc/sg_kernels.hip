@@ -240,12 +240,18 @@ __device__ __forceinline__ double pick1(const double (&a)[SG_NTINT], int it) {
   return it ? a1 : a0;
 }
 
-// Horner over the zero-padded SG_NCPMAX-term polynomial of interval it.
+// Horner over the zero-padded SG_NCPMAX-term polynomial of interval it.  Both
+// intervals are evaluated with uniform (scalar) coefficients and the result
+// is chosen per point, which is cheaper than choosing every coefficient.
 __device__ __forceinline__ double horner(const double (&a)[SG_NTINT][SG_NCPMAX], int it, double x) {
-  double f = pick(a, it, SG_NCPMAX - 1);
+  static_assert(SG_NTINT == 2, "two temperature intervals");
+  double f0 = a[0][SG_NCPMAX - 1], f1 = a[1][SG_NCPMAX - 1];
 #pragma unroll
-  for (int m = SG_NCPMAX - 2; m >= 0; m--) f = f * x + pick(a, it, m);
-  return f;
+  for (int m = SG_NCPMAX - 2; m >= 0; m--) {
+    f0 = f0 * x + a[0][m];
+    f1 = f1 * x + a[1][m];
+  }
+  return it ? f1 : f0;
 }
 
 // Transport fits, zero-padded to SG_NTRC terms.
@@ -280,15 +286,23 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
   uint32_t idxo = fld(F.itndex, bo / 2u);
 #pragma unroll
   for (int s = 0; s < NS; s++) {
-    int it = itint_of(idxo, s);
-    tc[0] = tc[0] + ry[s] * pick1(tb->hcp[s], it);
-    tc[1] = tc[1] + ry[s] * pick(tb->ct[s], it, 0);
+    // y * c(interval) as y0 * c(0) + y1 * c(1) with exactly one of y0, y1
+    // zero: the zero term adds an exact zero, so every sum is the one formed
+    // with the chosen interval alone, and the coefficients stay uniform
+    const int it = itint_of(idxo, s);
+    const double y0 = it ? 0.0 : ry[s], y1 = it ? ry[s] : 0.0;
+    tc[0] = tc[0] + y0 * tb->hcp[s][0];
+    tc[0] = tc[0] + y1 * tb->hcp[s][1];
+    tc[1] = tc[1] + y0 * tb->ct[s][0][0];
+    tc[1] = tc[1] + y1 * tb->ct[s][1][0];
     td[1] = tc[1];
-    // terms above the interval's order add ry*0, which leaves the sums unchanged
+    // terms above the interval's order add y*0, which leaves the sums unchanged
 #pragma unroll
     for (int m = 2; m <= SG_NCPMAX; m++) {
-      tc[m] = tc[m] + ry[s] * pick(tb->ct[s], it, m - 1);
-      td[m] = td[m] + ry[s] * pick(tb->cp[s], it, m - 1);
+      tc[m] = tc[m] + y0 * tb->ct[s][0][m - 1];
+      tc[m] = tc[m] + y1 * tb->ct[s][1][m - 1];
+      td[m] = td[m] + y0 * tb->cp[s][0][m - 1];
+      td[m] = td[m] + y1 * tb->cp[s][1][m - 1];
     }
     s7 = s7 + ry[s] * c->rgspec[s];
   }
@@ -625,9 +639,10 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
 
 // Mixed-precision point physics: the temperature Newton iteration, the
 // primitive variables and the enthalpies are formed in FP64 as in k_prim and
-// stored in FP32; transport and chemistry run in FP32, except that the rate
-// exponents (ln k = ln A + n ln T - E/RT, Gibbs sums) are assembled in FP64
-// before the FP32 exponential, since they are differences of O(10..100) terms.
+// stored in FP32; transport and chemistry run in FP32 with the hardware
+// exp2/log2 based intrinsics, except that the rate exponents (ln k = ln A +
+// n ln T - E/RT, Gibbs sums) are assembled in FP64 before the FP32
+// exponential, since they are differences of O(10..100) terms.
 template <int NS>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_prim_mx(Params P, const sg_config *__restrict__ c,
                                               const Tables *__restrict__ tb, Fields F) {
@@ -652,15 +667,23 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   uint32_t idxo = fld(F.itndex, bo / 2u);
 #pragma unroll
   for (int s = 0; s < NS; s++) {
-    int it = itint_of(idxo, s);
-    tc[0] = tc[0] + ry[s] * pick1(tb->hcp[s], it);
-    tc[1] = tc[1] + ry[s] * pick(tb->ct[s], it, 0);
+    // y * c(interval) as y0 * c(0) + y1 * c(1) with exactly one of y0, y1
+    // zero: the zero term adds an exact zero, so every sum is the one formed
+    // with the chosen interval alone, and the coefficients stay uniform
+    const int it = itint_of(idxo, s);
+    const double y0 = it ? 0.0 : ry[s], y1 = it ? ry[s] : 0.0;
+    tc[0] = tc[0] + y0 * tb->hcp[s][0];
+    tc[0] = tc[0] + y1 * tb->hcp[s][1];
+    tc[1] = tc[1] + y0 * tb->ct[s][0][0];
+    tc[1] = tc[1] + y1 * tb->ct[s][1][0];
     td[1] = tc[1];
-    // terms above the interval's order add ry*0, which leaves the sums unchanged
+    // terms above the interval's order add y*0, which leaves the sums unchanged
 #pragma unroll
     for (int m = 2; m <= SG_NCPMAX; m++) {
-      tc[m] = tc[m] + ry[s] * pick(tb->ct[s], it, m - 1);
-      td[m] = td[m] + ry[s] * pick(tb->cp[s], it, m - 1);
+      tc[m] = tc[m] + y0 * tb->ct[s][0][m - 1];
+      tc[m] = tc[m] + y1 * tb->ct[s][1][m - 1];
+      td[m] = td[m] + y0 * tb->cp[s][0][m - 1];
+      td[m] = td[m] + y1 * tb->cp[s][1][m - 1];
     }
     s7 = s7 + ry[s] * c->rgspec[s];
   }
@@ -717,7 +740,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   const float tf = (float)t;
   float lam, transp, wmomix = 0.0f;
   {
-    float fornow = (float)P.alamda * expf((float)P.rlamda * logf(tf));
+    float fornow = (float)P.alamda * __expf((float)P.rlamda * __logf(tf));
     lam = fornow * (float)cp;
     transp = fornow;
   }
@@ -726,25 +749,25 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   for (int s = 0; s < NS; s++) ryf[s] = (float)ry[s];
   const float df = (float)d;
   if (P.mixavg) {
-    transp = logf(tf / (float)P.tdifgb);
+    transp = __logf(tf / (float)P.tdifgb);
     float c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       const float *cc = nullptr;
       (void)cc;
-      float f = (float)c->condco[s][SG_NTRC - 1];
+      float f = tb->condf[s][SG_NTRC - 1];
 #pragma unroll
-      for (int m = SG_NTRC - 2; m >= 0; m--) f = f * transp + (float)c->condco[s][m];
-      float ctr = expf(f);
-      float fornow = ryf[s] * (float)c->ovwmol[s];
+      for (int m = SG_NTRC - 2; m >= 0; m--) f = f * transp + tb->condf[s][m];
+      float ctr = __expf(f);
+      float fornow = ryf[s] * tb->ovwf[s];
       c1 = c1 + fornow * ctr;
-      c2 = c2 + fornow / ctr;
+      c2 = c2 + __fdividef(fornow, ctr);
       c3 = c3 + fornow;
     }
-    c3 = 1.0f / c3;
+    c3 = __frcp_rn(c3);
     c1 = c1 * c3;
     c2 = c2 * c3;
-    lam = 0.5f * (c1 + 1.0f / c2);
+    lam = 0.5f * (c1 + __frcp_rn(c2));
     wmomix = df * c3;
   }
   fld(pr + P_LAM * fs, b4) = lam;
@@ -768,7 +791,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
     const int tl = threadIdx.x;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      lcy[s][tl] = ryf[s] * (float)c->ovwmol[s];
+      lcy[s][tl] = ryf[s] * tb->ovwf[s];
       lgb[s][tl] = gibbs[s];
     }
     auto cyv = [&](int s) { return lcy[s][tl]; };
@@ -779,56 +802,56 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       int mb = c->mblist[st];
       if (mb != 0) {
 #pragma unroll
-        for (int s = 0; s < NS; s++) s3 = s3 + ryf[s] * ((float)c->ovwmol[s] * (float)c->effy3b[mb - 1][s]);
+        for (int s = 0; s < NS; s++) s3 = s3 + ryf[s] * (tb->ovwf[s] * tb->effyf[mb - 1][s]);
       }
       double s2d = c->rparam[st][0] + c->rparam[st][1] * lt - c->rparam[st][2] * ovt;
-      float s1 = expf((float)s2d);
+      float s1 = __expf((float)s2d);
       int ml = c->mllist[st];
       if (ml != 0) {
         const double *rl = c->rclind[ml - 1];
-        float s4 = expf((float)(rl[0] + rl[1] * lt - rl[2] * ovt));
+        float s4 = __expf((float)(rl[0] + rl[1] * lt - rl[2] * ovt));
         float preduc = s3 * s4 / s1;
         float fornow = (float)rl[3] * preduc / (1.0f + preduc);
         s1 = s1 * fornow;
-        s2d = (double)logf(s1);
+        s2d = (double)__logf(s1);
         flthrd = false;
       }
       int mt = c->mtlist[st];
       if (mt != 0) {
         const double *rt = c->rctroe[mt - 1];
         float omalph = 1.0f - (float)rt[3];
-        float s4 = expf((float)(rt[0] + rt[1] * lt - rt[2] * ovt));
+        float s4 = __expf((float)(rt[0] + rt[1] * lt - rt[2] * ovt));
         float preduc = s3 * s4 / s1;
         float trats1 = tf * (float)rt[4], trats2 = (float)(-rt[5] * ovt), trats3 = tf * (float)rt[6];
-        float ftcent = omalph * expf(trats3) + (float)rt[3] * expf(trats1) + expf(trats2);
-        ftcent = log10f(ftcent);
+        float ftcent = omalph * __expf(trats3) + (float)rt[3] * __expf(trats1) + __expf(trats2);
+        ftcent = __log10f(ftcent);
         float cfactr = (float)rt[7] + (float)rt[8] * ftcent;
         float enfact = (float)rt[9] + (float)rt[10] * ftcent;
-        float fornow = log10f(preduc) + cfactr;
+        float fornow = __log10f(preduc) + cfactr;
         fornow = fornow / (enfact - (float)rt[11] * fornow);
         fornow = 1.0f + fornow * fornow;
         float fbroad = ftcent / fornow;
-        fbroad = expf(fbroad * (float)c->clnten);
+        fbroad = __expf(fbroad * (float)c->clnten);
         fornow = fbroad * preduc / (1.0f + preduc);
         s1 = s1 * fornow;
-        s2d = (double)logf(s1);
+        s2d = (double)__logf(s1);
         flthrd = false;
       }
       int ms = c->mslist[st];
       if (ms != 0) {
         const double *rs = c->rcsrif[ms - 1];
-        float s4 = expf((float)(rs[0] + rs[1] * lt - rs[2] * ovt));
+        float s4 = __expf((float)(rs[0] + rs[1] * lt - rs[2] * ovt));
         float preduc = s3 * s4 / s1;
         float fornow = (float)(rs[4] * ovt);
-        float fbroad = (float)rs[3] * expf(fornow);
+        float fbroad = (float)rs[3] * __expf(fornow);
         fornow = tf * (float)rs[5];
-        fbroad = fbroad + expf(fornow);
-        fornow = log10f(preduc);
-        fornow = 1.0f / (1.0f + log10f(fornow));
-        fbroad = (float)rs[6] * expf(fornow * logf(fbroad));
+        fbroad = fbroad + __expf(fornow);
+        fornow = __log10f(preduc);
+        fornow = 1.0f / (1.0f + __log10f(fornow));
+        fbroad = (float)rs[6] * __expf(fornow * __logf(fbroad));
         fornow = fbroad * preduc / (1.0f + preduc);
         s1 = s1 * fornow;
-        s2d = (double)logf(s1);
+        s2d = (double)__logf(s1);
         flthrd = false;
       }
       for (int m = 0; m < c->nrslen[st]; m++) {
@@ -839,13 +862,13 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       for (int m = 0; m < c->nrclen[st]; m++) {
         float fornow = cyv(c->nrcpec[st][m]);
         fornow = fornow > ysmall ? fornow : ysmall;
-        fornow = expf((float)c->crspec[st][m] * logf(fornow));
+        fornow = __expf(tb->crspecf[st][m] * __logf(fornow));
         fornow = fornow / (1.0f + ydenom * fornow);
         s1 = s1 * fornow;
       }
       if (c->mglist[st] != 0) {
         for (int m = 0; m < c->nsslen[st]; m++) s2d = s2d + c->diffmu[st][m] * gbv(c->nsspec[st][m]);
-        float s2 = expf((float)s2d);
+        float s2 = __expf((float)s2d);
         for (int m = 0; m < c->npslen[st]; m++) {
           float fornow = cyv(c->npspec[st][m]);
           fornow = fornow > 0.0f ? fornow : 0.0f;
@@ -854,7 +877,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
         for (int m = 0; m < c->npclen[st]; m++) {
           float fornow = cyv(c->npcpec[st][m]);
           fornow = fornow > ysmall ? fornow : ysmall;
-          fornow = expf((float)c->cpspec[st][m] * logf(fornow));
+          fornow = __expf(tb->cpspecf[st][m] * __logf(fornow));
           fornow = fornow / (1.0f + ydenom * fornow);
           s2 = s2 * fornow;
         }
@@ -863,11 +886,11 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       if (mb != 0 && flthrd) s1 = s1 * s3;
       if (tb->ratedense) {
 #pragma unroll
-        for (int u = 0; u < NS; u++) rate[u] = rate[u] + s1 * (float)tb->nuw[st][u];
+        for (int u = 0; u < NS; u++) rate[u] = rate[u] + s1 * tb->nuwf[st][u];
       } else {
         for (int m = 0; m < c->nsslen[st]; m++) {
           int s = c->nsspec[st][m];
-          float add = s1 * (float)c->diffmw[st][m];
+          float add = s1 * tb->diffmwf[st][m];
 #pragma unroll
           for (int u = 0; u < NS; u++)
             if (u == s) rate[u] = rate[u] + add;
@@ -894,7 +917,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   SG_PHASE();
   if (!P.mixavg) {
 #pragma unroll
-    for (int s = 0; s < NS; s++) fld(pr + prim_rd(P, s), b4) = transp * (float)c->olewis[s];
+    for (int s = 0; s < NS; s++) fld(pr + prim_rd(P, s), b4) = transp * tb->olewf[s];
     fld(pr + P_MU * fs, b4) = transp * (float)P.prantl;
   } else {
     // binary diffusion (src/rhscal.f:1311-1346), FP32
@@ -907,11 +930,11 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
     const float ppd = (float)(pres / P.pdifgb);
     float yw2[NS];
 #pragma unroll
-    for (int s = 0; s < NS; s++) yw2[s] = yt[s] * (float)c->ovwmol[s];
-    auto fit = [&](const double *a) {
-      float f = (float)a[SG_NTRC - 1];
+    for (int s = 0; s < NS; s++) yw2[s] = yt[s] * tb->ovwf[s];
+    auto fit = [&](const float *a) {
+      float f = a[SG_NTRC - 1];
 #pragma unroll
-      for (int m = SG_NTRC - 2; m >= 0; m--) f = f * transp + (float)a[m];
+      for (int m = SG_NTRC - 2; m >= 0; m--) f = f * transp + a[m];
       return f;
     };
     if (tb->diffsym) {
@@ -922,7 +945,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       for (int a = 0; a < NS; a++) {
 #pragma unroll
         for (int b = a; b < NS; b++) {
-          const float ictd = expf(-fit(c->diffco[a][b])) * ppd;
+          const float ictd = __expf(-fit(tb->difff[a][b])) * ppd;
           const float tab = yw2[b] * ictd;
           cs[a] = cs[a] + tab;
           if (b == a) {
@@ -936,7 +959,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       for (int s = 0; s < NS; s++) {
         float c1 = c1t - yt[s];
         float c2 = (cs[s] - self[s]) * wmomix;
-        fld(pr + prim_rd(P, s), b4) = df * c1 / c2;
+        fld(pr + prim_rd(P, s), b4) = __fdividef(df * c1, c2);
       }
     } else {
 #pragma unroll
@@ -944,13 +967,13 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
         float c2 = 0.0f, self = 0.0f;
 #pragma unroll
         for (int js = 0; js < NS; js++) {
-          const float term = yw2[js] * (expf(-fit(c->diffco[s][js])) * ppd);
+          const float term = yw2[js] * (__expf(-fit(tb->difff[s][js])) * ppd);
           c2 = c2 + term;
           if (js == s) self = term;
         }
         float c1 = c1t - yt[s];
         c2 = (c2 - self) * wmomix;
-        fld(pr + prim_rd(P, s), b4) = df * c1 / c2;
+        fld(pr + prim_rd(P, s), b4) = __fdividef(df * c1, c2);
       }
     }
     SG_PHASE();
@@ -958,10 +981,10 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
     float mus[NS], sq[NS], rsq[NS], yw[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      mus[s] = expf(fit(c->viscco[s]));
+      mus[s] = __expf(fit(tb->viscf[s]));
       sq[s] = sqrtf(mus[s]);
-      rsq[s] = 1.0f / sq[s];
-      yw[s] = y[s] * (float)c->ovwmol[s];
+      rsq[s] = __frcp_rn(sq[s]);
+      yw[s] = y[s] * tb->ovwf[s];
     }
     float c1 = 0.0f;
 #pragma unroll
@@ -970,11 +993,11 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
       for (int js = 0; js < NS; js++) {
         float fornow = js == s ? 1.0f : sq[s] * rsq[js];
-        fornow = 1.0f + fornow * (float)c->wilko2[s][js];
-        fornow = (float)c->wilko1[s][js] * fornow * fornow;
+        fornow = 1.0f + fornow * tb->wilk2f[s][js];
+        fornow = tb->wilk1f[s][js] * fornow * fornow;
         c2 = c2 + yw[js] * fornow;
       }
-      c1 = c1 + yw[s] * (mus[s] / c2);
+      c1 = c1 + yw[s] * __fdividef(mus[s], c2);
     }
     fld(pr + P_MU * fs, b4) = c1;
   }
