@@ -1152,33 +1152,58 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   const bool zadb_any = (i == 0 && P.fadb[0]) || (i == P.nx - 1 && P.fadb[1]) ||
                         (j == 0 && P.fadb[2]) || (j == P.ny - 1 && P.fadb[3]) ||
                         (k == 0 && P.fadb[4]) || (k == P.nz - 1 && P.fadb[5]);
-  // fluxes as the reference forms them: rho u*u, rho u*v, rho u*w, rho v*v, rho v*w, rho w*w
-  const double *fa[3];
-  const R *fb[3];
-  if (DIR == 0) { fa[0] = mx; fb[0] = uu; fa[1] = mx; fb[1] = vv; fa[2] = mx; fb[2] = ww; }
-  else if (DIR == 1) { fa[0] = mx; fb[0] = vv; fa[1] = my; fb[1] = vv; fa[2] = my; fb[2] = ww; }
-  else { fa[0] = mx; fb[0] = ww; fa[1] = my; fb[1] = ww; fa[2] = mz; fb[2] = ww; }
-  auto f_flux = [&](int a) {
-    const double *A_ = fa[a];
-    const R *B_ = fb[a];
-    return [A_, B_](long g) { return A_[g] * B_[g]; };
+  // rho u_d stays resident in its own LDS tile for the whole pass: products
+  // with it (e rho u_d, Y rho u_d and the momentum fluxes rho u_d u_b) are
+  // formed inside the stencils from two staged tiles instead of being staged
+  // from memory, which is the same multiplication of the same two values.
+  __shared__ R tmd[NIT * NTH];
+  auto stage_md = [&]() {
+#pragma unroll
+    for (int it = 0; it < NIT; it++) tmd[t + it * NTH] = pf[it];
+    __syncthreads();
   };
+  auto atm = [&](int kk) { return tmd[lbase + (kk - 8) * LSTR]; };
+  auto fqm = [&](int q) { return tmd[lbase + q * LSTR]; };
+  auto atp = [&](int kk) {
+    const int l = lbase + (kk - 8) * LSTR;
+    return tile[buf][l] * tmd[l];
+  };
+  auto fqp = [&](int q) {
+    const int l = lbase + q * LSTR;
+    return tile[buf][l] * tmd[l];
+  };
+  auto D1M = [&]() {
+    if (BND && edge) return d1_rows<R>(P, pk, DIR, wq, fqm);
+    return d1<false, R>(P, S, DIR, atm);
+  };
+  // The product stencil reads twice the operands of a plain one; its loads
+  // are kept from being hoisted over the preceding stencils so that they do
+  // not hold registers alongside them.
+  auto D1P = [&]() {
+    SG_PHASE();
+    R r;
+    if (BND && edge) r = d1_rows<R>(P, pk, DIR, wq, fqp);
+    else r = d1<false, R>(P, S, DIR, atp);
+    pin(r);
+    SG_PHASE();
+    return r;
+  };
+  // Momentum fluxes as the reference forms them: rho u*u, rho u*v, rho u*w,
+  // rho v*v, rho v*w, rho w*w.  Those of the form rho u_d * u_b come from the
+  // resident tile in the u_b stage; the others (rho u*v in y; rho u*w and
+  // rho v*w in z) are staged as products.
   auto f_y = [&](int sp) { const R *Y_ = pr + prim_y(P, sp); return [Y_](long g) { return Y_[g]; }; };
+  R flux[3];
   // ---------------- continuity and energy convection (src/rhscal.f:141-415)
   prefetch([&](long g) { return md[g]; });
-  stage();
-  prefetch([&](long g) { return ee[g] * md[g]; });
-  R dm = D1(), md0 = ctr();
+  stage_md();
+  prefetch([&](long g) { return ee[g]; });
+  R dm = D1M(), md0 = tmd[lbase];
   pin(dm); pin(md0);
-  next();
   R aD = -dm;
   stage();
-  prefetch([&](long g) { return ee[g]; });
-  R dem = D1();
-  pin(dem);
-  next();
-  stage();
   prefetch([&](long g) { return TT[g]; });
+  R dem = D1P();
   const R de = D1();
   const R e0 = ctr();
   next();
@@ -1200,36 +1225,51 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
     aE = aE + lam0 * (zcon ? R(0) : d2T);
     pin(aE);
   }
-  // ---------------- velocity derivatives
+  // ---------------- velocity derivatives and rho u_d u_b fluxes
   R du[3], d2u[3], u0[3];
   stage();
   prefetch([&](long g) { return vv[g]; });
   du[0] = D1(); d2u[0] = D2(); u0[0] = ctr();
+  if (DIR == 0) flux[0] = D1P();
   pin(du[0]); pin(d2u[0]); pin(u0[0]);
   next();
   stage();
   prefetch([&](long g) { return ww[g]; });
   du[1] = D1(); d2u[1] = D2(); u0[1] = ctr();
+  if (DIR <= 1) flux[1] = D1P();
   pin(du[1]); pin(d2u[1]); pin(u0[1]);
   next();
   stage();
-  prefetch(f_flux(0));
+  if (DIR == 0) prefetch([&](long g) { return pp[g] - pref; });
+  else if (DIR == 1) prefetch([&](long g) { return mx[g] * vv[g]; });
+  else prefetch([&](long g) { return mx[g] * ww[g]; });
   du[2] = D1(); d2u[2] = D2(); u0[2] = ctr();
+  flux[2] = D1P();
   pin(du[2]); pin(d2u[2]); pin(u0[2]);
   next();
+  if (DIR == 1) {
+    stage();
+    prefetch([&](long g) { return pp[g] - pref; });
+    flux[0] = D1();
+    next();
+  } else if (DIR == 2) {
+    stage();
+    prefetch([&](long g) { return my[g] * ww[g]; });
+    flux[0] = D1();
+    next();
+    stage();
+    prefetch([&](long g) { return pp[g] - pref; });
+    flux[1] = D1();
+    next();
+  }
   // ---------------- momentum convection (src/rhsvel.f:348-729, 1312-2110)
   R aM[3];
   {
     const R r0 = active ? (R)Q[idx] : R(0);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      stage();
-      if (a < 2) prefetch(f_flux(a + 1));
-      else prefetch([&](long g) { return pp[g] - pref; });
-      const R flx = D1();
-      next();
-      if (a == DIR) aM[a] = -R(0.5) * (md0 * du[a] + flx + u0[a] * dm);
-      else aM[a] = -R(0.5) * (flx + u0[a] * dm) - R(0.5) * r0 * u0[DIR] * du[a];
+      if (a == DIR) aM[a] = -R(0.5) * (md0 * du[a] + flux[a] + u0[a] * dm);
+      else aM[a] = -R(0.5) * (flux[a] + u0[a] * dm) - R(0.5) * r0 * u0[DIR] * du[a];
       pin(aM[a]);
     }
   }
@@ -1279,14 +1319,10 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   for (int sp = 0; sp < NS; sp++) {
     const R *Ys = pr + prim_y(P, sp), *Rs = pr + prim_rd(P, sp), *Hs = pr + prim_h(P, sp);
     stage();
-    prefetch([&](long g) { return Ys[g] * md[g]; });
-    R dY = D1(), d2Y = D2(), yc = ctr();
-    pin(dY); pin(d2Y); pin(yc);
-    next();
-    stage();
     prefetch([&](long g) { return Rs[g]; });
-    R dYm = D1();
-    pin(dYm);
+    R dY = D1(), d2Y = D2(), yc = ctr();
+    R dYm = D1P();
+    pin(dY); pin(d2Y); pin(yc); pin(dYm);
     next();
     stage();
     prefetch([&](long g) { return Hs[g]; });
