@@ -8,6 +8,13 @@ mixture-averaged transport, FP64, seeded Passot-Pouquet turbulence
 right-hand-side evaluations and register updates); the metric counts
 grid-point-substages per second with the state resident in HBM.
 
+Beside the headline figure, "batched" advances 64 uneven cases together
+(sg_batch: one launch per substage kernel over all cases, one HIP graph per
+RK step) against the same cases stepped one after another, and "serving"
+pushes a Poisson stream of flame-kernel requests through the continuous-
+batching loop (sengapy.serve) against sequential FP64 solves, with p50/p99
+time to solution.
+
 Output: one JSON line.  roofline: the dominant kernel's algorithmic HBM bytes
 per launch over its mean duration (HIP events on the library's stream);
 cpu_baseline: the C oracle (a scalar restatement of the reference, one core)
@@ -142,6 +149,95 @@ def mixed_run(abi, c, a, h, s64):
             "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in prof.items() if v[1] > 0}}
 
 
+def _make(spec):
+    from sengapy import case as Cs
+    return Cs.make_case(spec)
+
+
+def make_cases(specs, workers=8):
+    """Initial dumps of a case set, built by a pool of host processes (before
+    any GPU use in this process)."""
+    import multiprocessing as mp
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(workers) as pool:
+        cases = pool.map(_make, specs, chunksize=1)
+    print(f"{len(cases)} cases built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    return cases
+
+
+def batched_run(abi, h, cases, steps):
+    """Uneven cases advanced together (one launch per substage kernel over all
+    cases, one graph per RK step) against the same cases advanced one after
+    another on their own handles; states resident in HBM in both runs.  Rates
+    are normalised to 100-step cases."""
+    import numpy as np
+    out = {}
+    pts = sum(c.shape[0] * c.shape[1] * c.shape[2] for c in cases)
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    h.hipEventCreate(C.byref(ev0))
+    h.hipEventCreate(C.byref(ev1))
+    for prec in (0, 1):
+        sols = []
+        for c in cases:
+            s = abi.Solver(c, precision=prec)
+            s.set_timestep(c.dump.tstep)
+            s.upload(c.dump.state, 0.0)
+            sols.append(s)
+        # sequential: each case on its own, one after the other
+        if prec == 0:
+            for s in sols:
+                s.step(1)
+            h.hipDeviceSynchronize()
+            t0 = time.perf_counter()
+            for s in sols:
+                s.step(steps)
+            h.hipDeviceSynchronize()
+            t_seq = time.perf_counter() - t0
+        b = abi.Batch()
+        for s in sols:
+            b.add(s)
+        b.step(1)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        t_b = time.perf_counter() - t0
+        tag = "fp64" if prec == 0 else "mixed"
+        out[tag] = {"cases_per_s": len(cases) / (t_b * 100.0 / steps), "s_per_step": t_b / steps,
+                    "mpoint_substages_per_s": pts * 5 * steps / t_b / 1e6}
+        if prec == 0:
+            out["sequential_fp64"] = {"cases_per_s": len(cases) / (t_seq * 100.0 / steps),
+                                      "s_per_step": t_seq / steps,
+                                      "mpoint_substages_per_s": pts * 5 * steps / t_seq / 1e6}
+        b.close()
+        for s in sols:
+            s.close()
+        print(f"batched {tag}: {t_b:.3f} s for {steps} steps of {len(cases)} cases", file=sys.stderr, flush=True)
+    out["speedup_fp64_vs_sequential"] = out["fp64"]["cases_per_s"] / out["sequential_fp64"]["cases_per_s"]
+    out["speedup_mixed_vs_sequential_fp64"] = out["mixed"]["cases_per_s"] / out["sequential_fp64"]["cases_per_s"]
+    return out
+
+
+def serve_run(cases, specs, steps):
+    """Poisson request stream through the continuous-batching loop, FP64 and
+    mixed, against sequential FP64 solves of the same stream."""
+    from sengapy import serve
+    reqs = [serve.Request(q, c, steps, sp["arrival"]) for q, (c, sp) in enumerate(zip(cases, specs))]
+    out = {}
+    for tag, prec in (("fp64", 0), ("mixed", 1)):
+        srv = serve.Server(precision=prec, keep=True)
+        res = srv.run(reqs)
+        out[tag] = serve.summary(res)
+        out[tag]["max_points_in_flight"] = srv.max_points_in_flight
+        print(f"served {tag}: {out[tag]}", file=sys.stderr, flush=True)
+    res = serve.run_sequential(reqs, precision=0)
+    out["sequential_fp64"] = serve.summary(res)
+    print(f"sequential: {out['sequential_fp64']}", file=sys.stderr, flush=True)
+    out["speedup_fp64_vs_sequential"] = out["fp64"]["cases_per_s"] / out["sequential_fp64"]["cases_per_s"]
+    out["speedup_mixed_vs_sequential_fp64"] = out["mixed"]["cases_per_s"] / out["sequential_fp64"]["cases_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +250,13 @@ def main():
                     help="also run one 1 GiB device-to-device copy (counter calibration)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision run reported beside the FP64 figure")
+    ap.add_argument("--batch-cases", type=int, default=64,
+                    help="uneven H2-air cases of the batched run (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--serve-requests", type=int, default=16,
+                    help="flame-kernel requests of the served run (0: skip)")
+    ap.add_argument("--serve-steps", type=int, default=40)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (multi-GPU sharding is "
@@ -161,6 +264,10 @@ def main():
         sys.exit(2)
     from sengapy import abi
     from sengapy import case as Cs
+    bspecs = Cs.batch_specs(a.batch_cases) if a.batch_cases > 0 else []
+    sspecs = Cs.kernel_specs(a.serve_requests, rate=a.serve_rate, nsteps=a.serve_steps) if a.serve_requests > 0 else []
+    built = make_cases(bspecs + sspecs) if bspecs or sspecs else []
+    bcases, scases = built[:len(bspecs)], built[len(bspecs):]
     n = a.n
     t0 = time.perf_counter()
     c = Cs.h2_nscbc(n=(n, n, n), length=(5.0e-3, 5.0e-3, 5.0e-3), mixavg=True, nsteps=a.steps)
@@ -237,6 +344,16 @@ def main():
     }
     if not a.no_mixed:
         out["mixed"] = mixed_run(abi, c, a, h, s)
+    if bcases:
+        out["batched"] = batched_run(abi, h, bcases, a.batch_steps)
+        out["batched"]["workload"] = (f"{len(bcases)} H2-air NSCBC cases, n^3 with n from {{48,64,96,128}}, "
+                                      "phi~U(0.5,1.2), u'/S_L~U(1,10), T_u~U(300,700) K; "
+                                      f"{a.batch_steps} RK steps timed, rates per 100-step case")
+    if scases:
+        out["serving"] = serve_run(scases, sspecs, a.serve_steps)
+        out["serving"]["workload"] = (f"{len(scases)} H2-air flame-kernel requests, Poisson {a.serve_rate}/s, "
+                                      f"n^3 with n in 48..160, {a.serve_steps} RK steps each, "
+                                      "240 GB HBM budget, time to solution = arrival to downloaded state")
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -222,6 +222,46 @@ int sg_profile_read(sg_handle *h, int idx, char *name, size_t n, double *total_m
 /* Device bytes held by the handle. */
 int sg_memory(sg_handle *h, size_t *bytes);
 
+/* Device bytes sg_create would allocate for cfg (without allocating), for
+ * admission by memory budget; SG_EINVAL for a configuration sg_create
+ * rejects.  No device access. */
+int sg_footprint(const sg_config *cfg, size_t *bytes);
+
+/* ---- Batches: many cases advanced together ------------------------------
+ * No counterpart in the reference, which runs one case per job; this is the
+ * batch variant of the src/senga2.f:96-194 step loop.  A batch advances its
+ * member handles in lockstep: every substage kernel is ONE launch covering
+ * all members (their grids concatenated), and one full RK step (BOUNDT and
+ * the five substages of src/senga2.f:113-159, then BOUNTT) is captured once
+ * as a HIP graph and replayed.  Each member keeps its own grid size, state,
+ * boundary data and fixed time step, and computes exactly what it computes
+ * alone.  Members share the species count, precision, x boundary kind and
+ * mechanism (chem.dat / diff.dat data), else sg_batch_add returns
+ * SG_EINVAL.  Adaptive stepping is not available inside a batch.
+ *
+ * While a handle is in a batch, sg_upload_state, sg_substage, sg_end_step,
+ * sg_step and sg_rhs return SG_ESTATE; sg_download_state and sg_sync order
+ * themselves after the batch steps queued so far; sg_set_timestep applies
+ * from the next sg_batch_step; sg_destroy removes the handle first.  One host
+ * thread drives a batch and its members. */
+typedef struct sg_batch sg_batch;
+int sg_batch_create(sg_batch **out);
+/* Detaches the members (they stay valid) and frees the batch. */
+int sg_batch_destroy(sg_batch *b);
+/* Adds a handle with uploaded state; its queued work completes first. */
+int sg_batch_add(sg_batch *b, sg_handle *h);
+/* Removes a handle; its later calls run after the queued batch steps. */
+int sg_batch_remove(sg_batch *b, sg_handle *h);
+int sg_batch_size(sg_batch *b, int *n);
+/* Member idx (0-based, in order of addition among current members). */
+int sg_batch_member(sg_batch *b, int idx, sg_handle **h);
+/* Queue nsteps RK steps of every member (asynchronous). */
+int sg_batch_step(sg_batch *b, int nsteps);
+/* Wait for the queued steps; on a temperature failure returns its code and
+ * the failing member's index in *failed (else -1), whose sg_last_error holds
+ * the point. */
+int sg_batch_sync(sg_batch *b, int *failed);
+
 #ifdef __cplusplus
 }
 #endif

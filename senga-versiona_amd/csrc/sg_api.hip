@@ -4,6 +4,7 @@
 // substage sequence mirrors src/senga2.f:113-159; see include/senga_gpu.h for
 // the contract of each entry point.
 #include <math.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -22,6 +23,8 @@ const char *kKernelNames[K_COUNT] = {"tempin", "boundt", "prim", "pass_x", "pass
                                      "final", "bountt", "derivs"};
 
 }  // namespace
+
+struct sg_batch;
 
 struct sg_handle {
   sg_config cfg;       // host copy (species lists 0-based, REAL-rounded falloff constants)
@@ -43,6 +46,12 @@ struct sg_handle {
   int err_code;
   int err_ijk[3];
   std::string err_msg;
+  // this case as the single member of its own launches
+  Member mem;            // host copy of the device member
+  Launch L;
+  void *lbase;           // device member and block maps
+  size_t lbytes;
+  sg_batch *batch;       // batch this handle advances in, or null
   // profiling
   int prof_on;
   hipEvent_t ev[2];
@@ -250,6 +259,7 @@ static int setup_params(sg_handle *h) {
   for (int s = 0; s < c.nspec; s++) rtin = rtin + c.rgspec[s] * c.yrin[s];
   rtin = rtin * c.trin;
   P.drin = c.prin / rtin;
+  for (int q = 0; q < c.nspec; q++) P.yrin[q] = c.yrin[q];
   P.mixavg = c.mixavg;
   P.precision = c.precision;
   P.pofs = c.precision ? c.prin : 0.0;
@@ -261,14 +271,189 @@ static int setup_params(sg_handle *h) {
   return 0;
 }
 
-extern "C" {
+// One-member launch set of a handle: its member record and all-zero block
+// maps (every block belongs to member 0, blk0 = 0).
+static int init_launch(sg_handle *h) {
+  Launch &L = h->L;
+  memset(&L, 0, sizeof L);
+  member_blocks(h->P, L.nblk);
+  size_t bytes = (sizeof(Member) + 255) / 256 * 256;
+  for (int k = 0; k < L_NKIND; k++) bytes += ((size_t)L.nblk[k] * sizeof(uint32_t) + 255) / 256 * 256;
+  if (hipMalloc(&h->lbase, bytes) != hipSuccess) {
+    h->lbase = nullptr;
+    return SG_ENOMEM;
+  }
+  if (hipMemset(h->lbase, 0, bytes) != hipSuccess) return SG_EDEVICE;
+  char *b = (char *)h->lbase;
+  L.mem = (const Member *)b;
+  b += (sizeof(Member) + 255) / 256 * 256;
+  for (int k = 0; k < L_NKIND; k++) {
+    L.map[k] = (const uint32_t *)b;
+    b += ((size_t)L.nblk[k] * sizeof(uint32_t) + 255) / 256 * 256;
+  }
+  L.nmem = 1;
+  L.c = h->dcfg;
+  L.tab = h->F.tab;
+  L.ns = h->P.ns;
+  L.precision = h->P.precision;
+  L.bndx = h->P.bnd[0];
+  L.inflow = h->P.inflow_xl;
+  return 0;
+}
 
-int sg_abi_version(void) { return SG_ABI_VERSION; }
-size_t sg_config_size(void) { return sizeof(sg_config); }
+static void batch_touch(sg_batch *b);
 
-int sg_create(const sg_config *cfg, sg_handle **out) {
-  if (!cfg || !out) return SG_EINVAL;
-  *out = nullptr;
+// Copy the handle's parameters, fields and RK coefficients into its device
+// member (stream-ordered, so launches already queued see the old record).
+static int refresh_member(sg_handle *h) {
+  if (!h->lbase) return 0;
+  Member &M = h->mem;
+  memset(&M, 0, sizeof M);
+  M.P = h->P;
+  M.F = h->F;
+  for (int r = 0; r < 5; r++) {
+    M.rkl[r] = h->rklhs[r];
+    M.rkr[r] = h->rkrhs[r];
+    M.rke[r] = h->rkerr[r];
+  }
+  HIPCHK(h, hipMemcpyAsync((void *)h->L.mem, &M, sizeof M, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (h->batch) batch_touch(h->batch);
+  return 0;
+}
+
+// A batch advances its member handles together: each substage kernel is one
+// launch over all members (see Launch), and one RK step (BOUNDT, five
+// substages, BOUNTT) is captured once as a HIP graph and replayed.  Members
+// share the species count, precision, x boundary kind and mechanism.
+struct sg_batch {
+  std::vector<sg_handle *> members;
+  hipStream_t stream;
+  hipEvent_t ev;
+  int device;
+  void *lbase;          // device member table and block maps
+  size_t lbytes;
+  Launch L;
+  int dirty_layout;     // membership changed: maps and graph are stale
+  int dirty_members;    // a member's record changed: table is stale
+  hipGraphExec_t exec;  // captured RK step, or null
+};
+
+static void batch_touch(sg_batch *b) { b->dirty_members = 1; }
+
+// Mechanism fields the kernels read through the shared configuration (every
+// sg_config member a kernel dereferences, except the per-case inflow state,
+// which travels in Params).
+#define SG_MECH_FIELDS(X)                                                                              \
+  X(nspec) X(nstep) X(mixavg) X(amascp) X(amasct) X(clnten) X(condco) X(cpspec) X(crspec) X(diffco)   \
+  X(diffmu) X(diffmw) X(effy3b) X(mblist) X(mglist) X(mllist) X(mslist) X(mtlist) X(ncenth) X(ncpoly) \
+  X(npclen) X(npcpec) X(npslen) X(npspec) X(nrclen) X(nrcpec) X(nrslen) X(nrspec) X(nsslen) X(nsspec) \
+  X(ntint) X(olewis) X(ovwmol) X(rclind) X(rcsrif) X(rctroe) X(rgspec) X(rparam) X(tinthi) X(viscco)  \
+  X(wilko1) X(wilko2) X(ncovis) X(ncocon) X(ncodif)
+
+static bool same_mechanism(const sg_handle *a, const sg_handle *b) {
+#define CMP(f) if (memcmp(&a->cfg.f, &b->cfg.f, sizeof a->cfg.f) != 0) return false;
+  SG_MECH_FIELDS(CMP)
+#undef CMP
+  // thermodynamic tables (the derivative row weights depend on the grid
+  // spacing and stay per member)
+  const size_t off = offsetof(Tables, cp);
+  return memcmp((const char *)&a->tab + off, (const char *)&b->tab + off, sizeof(Tables) - off) == 0;
+}
+
+// Rebuild the device member table (and, when membership changed, the block
+// maps).  Waits for the queued batch work, which reads the table.
+static int batch_prepare(sg_batch *b) {
+  if (!b->dirty_layout && !b->dirty_members) return 0;
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return SG_EDEVICE;
+  const int nm = (int)b->members.size();
+  std::vector<Member> hm(nm);
+  int tot[L_NKIND] = {0, 0, 0, 0, 0};
+  for (int m = 0; m < nm; m++) {
+    hm[m] = b->members[m]->mem;
+    for (int k = 0; k < L_NKIND; k++) {
+      hm[m].blk0[k] = tot[k];
+      long t = (long)tot[k] + b->members[m]->L.nblk[k];
+      if (t >= (1L << 31)) return SG_EINVAL;
+      tot[k] = (int)t;
+    }
+  }
+  const size_t mbytes = (sizeof(Member) * nm + 255) / 256 * 256;
+  size_t bytes = mbytes;
+  for (int k = 0; k < L_NKIND; k++) bytes += ((size_t)tot[k] * sizeof(uint32_t) + 255) / 256 * 256;
+  if (b->dirty_layout) {
+    if (bytes > b->lbytes) {
+      if (b->lbase) hipFree(b->lbase);
+      b->lbase = nullptr;
+      b->lbytes = 0;
+      if (hipMalloc(&b->lbase, bytes) != hipSuccess) return SG_ENOMEM;
+      b->lbytes = bytes;
+    }
+    char *p = (char *)b->lbase;
+    b->L.mem = (const Member *)p;
+    p += mbytes;
+    std::vector<uint32_t> map;
+    for (int k = 0; k < L_NKIND; k++) {
+      map.assign((size_t)tot[k], 0u);
+      for (int m = 0; m < nm; m++)
+        for (int q = 0; q < b->members[m]->L.nblk[k]; q++) map[(size_t)hm[m].blk0[k] + q] = (uint32_t)m;
+      b->L.map[k] = (const uint32_t *)p;
+      b->L.nblk[k] = tot[k];
+      if (hipMemcpy(p, map.data(), map.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+        return SG_EDEVICE;
+      p += ((size_t)tot[k] * sizeof(uint32_t) + 255) / 256 * 256;
+    }
+    const sg_handle *h0 = b->members[0];
+    b->L.nmem = nm;
+    b->L.c = h0->dcfg;
+    b->L.tab = h0->F.tab;
+    b->L.ns = h0->P.ns;
+    b->L.precision = h0->P.precision;
+    b->L.bndx = h0->P.bnd[0];
+    b->L.inflow = 0;
+    for (int m = 0; m < nm; m++) b->L.inflow |= b->members[m]->P.inflow_xl;
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (hipMemcpy((void *)b->L.mem, hm.data(), sizeof(Member) * nm, hipMemcpyHostToDevice) != hipSuccess)
+    return SG_EDEVICE;
+  b->dirty_layout = b->dirty_members = 0;
+  return 0;
+}
+
+// Capture one RK step of every member: src/senga2.f:113-159 per substage,
+// then BOUNTT.
+static int batch_capture(sg_batch *b) {
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return SG_EDEVICE;
+  for (int r = 1; r <= 5; r++) {
+    if (b->L.inflow) launch_boundt(b->L, b->stream);
+    launch_prim(b->L, b->stream);
+    launch_pass(0, b->L, b->stream);
+    launch_pass(1, b->L, b->stream);
+    launch_pass(2, b->L, b->stream);
+    launch_final(b->L, r, nullptr, b->stream);
+  }
+  if (b->L.inflow) launch_bountt(b->L, b->stream);
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess || !g) return SG_EDEVICE;
+  e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    b->exec = nullptr;
+    return SG_EDEVICE;
+  }
+  return 0;
+}
+
+// Order the handle's stream after every batch step queued so far.
+static int batch_fence(sg_batch *b, sg_handle *h) {
+  HIPCHK(h, hipEventRecord(b->ev, b->stream));
+  HIPCHK(h, hipStreamWaitEvent(h->stream, b->ev, 0));
+  return 0;
+}
+
+static int validate_config(const sg_config *cfg) {
   if (!species_supported(cfg->nspec)) return SG_EINVAL;   // compiled species counts
   if (cfg->nstep < 0 || cfg->nstep > SG_MAX_STEP) return SG_EINVAL;
   if (cfg->nx < 11 || cfg->ny < 11 || cfg->nz < 11) return SG_EINVAL;
@@ -284,6 +469,43 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
           cfg->ncenth[s][it] > SG_NCOF || cfg->ncenpy[s][it] < 1 || cfg->ncenpy[s][it] > SG_NCOF)
         return SG_EINVAL;
   }
+  return 0;
+}
+
+// Device bytes of a case: the register/work block, interval indices, NSCBC
+// face data, configuration and table copies, error word (sg_create's single
+// allocation) plus its member record and block maps.
+static size_t case_bytes(const Params &P, size_t *main_block) {
+  const int ns = P.ns;
+  const long fs = P.fstride;
+  // FP64 registers; primitives, rates, accumulators and gradients in the
+  // working precision (FP32 in the mixed mode)
+  const size_t esz = P.precision ? sizeof(float) : sizeof(double);
+  const long nreg = (5 + ns) * 2, nwork = nprim(ns) + ns + (5 + ns) + 12;
+  size_t bytes = sizeof(double) * (size_t)fs * nreg + esz * (size_t)fs * nwork;
+  bytes = ((bytes + 255) / 256) * 256;
+  const size_t itn = ((sizeof(uint32_t) * (size_t)fs + 255) / 256) * 256;
+  const size_t facesz = sizeof(double) * 2 * (size_t)nface(ns) * P.ny * P.nz;
+  const size_t cfgsz = ((sizeof(sg_config) + 255) / 256) * 256;
+  const size_t tabsz = ((sizeof(Tables) + 255) / 256) * 256;
+  const size_t main = bytes + itn + ((facesz + 255) / 256) * 256 + cfgsz + tabsz + 256;
+  if (main_block) *main_block = main;
+  int nblk[L_NKIND];
+  member_blocks(P, nblk);
+  size_t lb = (sizeof(Member) + 255) / 256 * 256;
+  for (int k = 0; k < L_NKIND; k++) lb += ((size_t)nblk[k] * sizeof(uint32_t) + 255) / 256 * 256;
+  return main + lb;
+}
+
+extern "C" {
+
+int sg_abi_version(void) { return SG_ABI_VERSION; }
+size_t sg_config_size(void) { return sizeof(sg_config); }
+
+int sg_create(const sg_config *cfg, sg_handle **out) {
+  if (!cfg || !out) return SG_EINVAL;
+  *out = nullptr;
+  if (int rc0 = validate_config(cfg)) return rc0;
   sg_handle *h = new sg_handle();
   h->cfg = *cfg;
   build_tables(h->cfg, h->tab);
@@ -312,14 +534,14 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   // FP64 registers; primitives, rates, accumulators and gradients in the
   // working precision (FP32 in the mixed mode)
   const size_t esz = P.precision ? sizeof(float) : sizeof(double);
-  const long nreg = (5 + ns) * 2, nwork = nprim(ns) + ns + (5 + ns) + 12;
-  size_t bytes = sizeof(double) * (size_t)fs * nreg + esz * (size_t)fs * nwork;
+  const size_t total = case_bytes(P, &h->bytes);
+  h->lbytes = total - h->bytes;
+  size_t bytes = sizeof(double) * (size_t)fs * (5 + ns) * 2 + esz * (size_t)fs * (nprim(ns) + ns + (5 + ns) + 12);
   bytes = ((bytes + 255) / 256) * 256;
-  size_t itn = ((sizeof(uint32_t) * (size_t)fs + 255) / 256) * 256;
-  size_t facesz = sizeof(double) * 2 * (size_t)nface(ns) * P.ny * P.nz;
-  size_t cfgsz = ((sizeof(sg_config) + 255) / 256) * 256;
-  size_t tabsz = ((sizeof(Tables) + 255) / 256) * 256;
-  h->bytes = bytes + itn + ((facesz + 255) / 256) * 256 + cfgsz + tabsz + 256;
+  const size_t itn = ((sizeof(uint32_t) * (size_t)fs + 255) / 256) * 256;
+  const size_t facesz = sizeof(double) * 2 * (size_t)nface(ns) * P.ny * P.nz;
+  const size_t cfgsz = ((sizeof(sg_config) + 255) / 256) * 256;
+  const size_t tabsz = ((sizeof(Tables) + 255) / 256) * 256;
   if (hipMalloc(&h->base, h->bytes) != hipSuccess) {
     hipStreamDestroy(h->stream);
     delete h;
@@ -350,6 +572,10 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   hipEventCreate(&h->ev[0]);
   hipEventCreate(&h->ev[1]);
   h->etime = 0.0;
+  if (init_launch(h) != 0) {
+    sg_destroy(h);
+    return SG_ENOMEM;
+  }
   sg_set_timestep(h, 0.0);
   *out = h;
   return 0;
@@ -357,7 +583,9 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
 
 int sg_destroy(sg_handle *h) {
   if (!h) return SG_EINVAL;
+  if (h->batch) sg_batch_remove(h->batch, h);
   hipStreamSynchronize(h->stream);
+  if (h->lbase) hipFree(h->lbase);
   hipEventDestroy(h->ev[0]);
   hipEventDestroy(h->ev[1]);
   hipFree(h->base);
@@ -385,6 +613,7 @@ static int check_device_error(sg_handle *h) {
 
 int sg_upload_state(sg_handle *h, const double *run, double etime) {
   if (!h || !run) return set_err(h, SG_EINVAL, "null argument");
+  if (h->batch) return set_err(h, SG_ESTATE, "handle advances in a batch");
   const Params &P = h->P;
   size_t fb = sizeof(double) * P.npts;
   for (int v = 0; v < 5 + P.ns; v++) {
@@ -405,6 +634,10 @@ int sg_upload_state(sg_handle *h, const double *run, double etime) {
 int sg_download_state(sg_handle *h, double *run, double *temp, double *pres, double *rate) {
   if (!h) return SG_EINVAL;
   if (!h->have_state) return set_err(h, SG_ESTATE, "no state uploaded");
+  if (h->batch) {
+    int rc = batch_fence(h->batch, h);   // after every batch step queued so far
+    if (rc) return rc;
+  }
   const Params &P = h->P;
   size_t fb = sizeof(double) * P.npts;
   if (run)
@@ -461,30 +694,29 @@ int sg_set_timestep(sg_handle *h, double dt) {
     for (int q = 0; q < r - 1; q++) t = t + bcofrk[q];
     h->rktim[r] = t * dt;
   }
-  return 0;
+  return refresh_member(h);
 }
 
 // BOUNDT .. BOUNDS (src/senga2.f:118-131): everything up to the register update.
 static int eval_rhs(sg_handle *h) {
-  const Params &P = h->P;
+  const Launch &L = h->L;
   int rc = 0;
-  if (P.inflow_xl) rc = timed(h, K_BOUNDT, [&] { launch_boundt(P, h->dcfg, h->F, h->stream); });
-  if (!rc) rc = timed(h, K_PRIM, [&] { launch_prim(P, h->dcfg, h->F, h->stream); });
-  if (!rc) rc = timed(h, K_PASSX, [&] { launch_pass(0, P, h->F, h->stream); });
-  if (!rc) rc = timed(h, K_PASSY, [&] { launch_pass(1, P, h->F, h->stream); });
-  if (!rc) rc = timed(h, K_PASSZ, [&] { launch_pass(2, P, h->F, h->stream); });
+  if (L.inflow) rc = timed(h, K_BOUNDT, [&] { launch_boundt(L, h->stream); });
+  if (!rc) rc = timed(h, K_PRIM, [&] { launch_prim(L, h->stream); });
+  if (!rc) rc = timed(h, K_PASSX, [&] { launch_pass(0, L, h->stream); });
+  if (!rc) rc = timed(h, K_PASSY, [&] { launch_pass(1, L, h->stream); });
+  if (!rc) rc = timed(h, K_PASSZ, [&] { launch_pass(2, L, h->stream); });
   return rc;
 }
 
 int sg_substage(sg_handle *h, int irk) {
   if (!h) return SG_EINVAL;
+  if (h->batch) return set_err(h, SG_ESTATE, "handle advances in a batch");
   if (!h->have_state) return set_err(h, SG_ESTATE, "no state uploaded");
   if (irk < 1 || irk > 5) return set_err(h, SG_EINVAL, "irk must be 1..5");
   int rc = eval_rhs(h);
   if (rc) return rc;
-  int mode = irk < 5 ? 0 : 1;
-  double lhs = h->rklhs[irk - 1], rhs = h->rkrhs[irk - 1], erc = h->rkerr[irk - 1];
-  return timed(h, K_FINAL, [&] { launch_final(h->P, h->dcfg, h->F, mode, lhs, rhs, erc, nullptr, h->stream); });
+  return timed(h, K_FINAL, [&] { launch_final(h->L, irk, nullptr, h->stream); });
 }
 
 // ADAPTT (src/adaptt.f:60-330) with the constants of src/dtinit.f:101-170:
@@ -519,20 +751,24 @@ static int adaptt(sg_handle *h) {
     h->rktim[r] = h->rktim[r] * tratio;
   }
   launch_zero(h->F.erk, (long)(5 + h->P.ns) * h->P.fstride, h->stream);
+  int rc = refresh_member(h);
+  if (rc) return rc;
   return check_device_error(h);
 }
 
 int sg_end_step(sg_handle *h) {
   if (!h) return SG_EINVAL;
+  if (h->batch) return set_err(h, SG_ESTATE, "handle advances in a batch");
   h->etime = h->etime + h->tstep;
   int rc = 0;
-  if (h->P.inflow_xl) rc = timed(h, K_BOUNTT, [&] { launch_bountt(h->P, h->dcfg, h->F, h->stream); });
+  if (h->L.inflow) rc = timed(h, K_BOUNTT, [&] { launch_bountt(h->L, h->stream); });
   if (!rc && h->adaptive) rc = adaptt(h);
   return rc;
 }
 
 int sg_set_adaptive(sg_handle *h, int on, double errold, double errldr) {
   if (!h || (on && !(errold > 0.0 && errldr > 0.0))) return SG_EINVAL;
+  if (on && h->batch) return set_err(h, SG_ESTATE, "batched handles use a fixed time step");
   if (on && !h->ebase) {
     const size_t fb = sizeof(double) * (size_t)(5 + h->P.ns) * h->P.fstride;
     h->ebytes = fb + 256;
@@ -550,7 +786,7 @@ int sg_set_adaptive(sg_handle *h, int on, double errold, double errldr) {
     h->errldr = errldr;
     launch_zero(h->F.erk, (long)(5 + h->P.ns) * h->P.fstride, h->stream);
   }
-  return 0;
+  return refresh_member(h);
 }
 
 int sg_get_timestep(sg_handle *h, double *tstep, double *errold, double *errldr) {
@@ -576,6 +812,7 @@ int sg_step(sg_handle *h, int nsteps) {
 
 int sg_rhs(sg_handle *h, int irk, double *rhs) {
   if (!h || !rhs) return SG_EINVAL;
+  if (h->batch) return set_err(h, SG_ESTATE, "handle advances in a batch");
   if (!h->have_state) return set_err(h, SG_ESTATE, "no state uploaded");
   (void)irk;
   const Params &P = h->P;
@@ -586,7 +823,7 @@ int sg_rhs(sg_handle *h, int irk, double *rhs) {
   HIPCHK(h, hipMalloc((void **)&out, qbytes));
   HIPCHK(h, hipMemcpyAsync(save, h->F.q, qbytes, hipMemcpyDeviceToDevice, h->stream));
   int rc = eval_rhs(h);
-  if (!rc) rc = timed(h, K_FINAL, [&] { launch_final(P, h->dcfg, h->F, 2, 0.0, 0.0, 0.0, out, h->stream); });
+  if (!rc) rc = timed(h, K_FINAL, [&] { launch_final(h->L, 0, out, h->stream); });
   if (!rc) {
     size_t fb = sizeof(double) * P.npts;
     for (int v = 0; v < 5 + P.ns; v++)
@@ -619,6 +856,10 @@ int sg_derivatives(sg_handle *h, const double *f, double *out) {
 
 int sg_sync(sg_handle *h) {
   if (!h) return SG_EINVAL;
+  if (h->batch) {
+    int rc = batch_fence(h->batch, h);
+    if (rc) return rc;
+  }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return check_device_error(h);
 }
@@ -661,9 +902,136 @@ int sg_profile_read(sg_handle *h, int idx, char *name, size_t n, double *total_m
   return 0;
 }
 
+int sg_footprint(const sg_config *cfg, size_t *bytes) {
+  if (!cfg || !bytes) return SG_EINVAL;
+  *bytes = 0;
+  if (int rc = validate_config(cfg)) return rc;
+  sg_handle *h = new sg_handle();
+  h->cfg = *cfg;
+  int rc = setup_params(h);
+  if (!rc && h->P.fstride * (long)sizeof(double) >= (1L << 32)) rc = SG_EINVAL;
+  if (!rc) *bytes = case_bytes(h->P, nullptr);
+  delete h;
+  return rc;
+}
+
 int sg_memory(sg_handle *h, size_t *bytes) {
   if (!h || !bytes) return SG_EINVAL;
-  *bytes = h->bytes + (h->ebase ? h->ebytes : 0);
+  *bytes = h->bytes + h->lbytes + (h->ebase ? h->ebytes : 0);
+  return 0;
+}
+
+int sg_batch_create(sg_batch **out) {
+  if (!out) return SG_EINVAL;
+  *out = nullptr;
+  sg_batch *b = new sg_batch();
+  b->lbase = nullptr;
+  b->lbytes = 0;
+  b->exec = nullptr;
+  b->dirty_layout = b->dirty_members = 0;
+  memset(&b->L, 0, sizeof b->L);
+  if (hipGetDevice(&b->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return SG_EDEVICE;
+  }
+  if (hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) != hipSuccess) {
+    hipStreamDestroy(b->stream);
+    delete b;
+    return SG_EDEVICE;
+  }
+  *out = b;
+  return 0;
+}
+
+int sg_batch_destroy(sg_batch *b) {
+  if (!b) return SG_EINVAL;
+  hipStreamSynchronize(b->stream);
+  for (sg_handle *h : b->members) h->batch = nullptr;
+  b->members.clear();
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->lbase) hipFree(b->lbase);
+  hipEventDestroy(b->ev);
+  hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+int sg_batch_add(sg_batch *b, sg_handle *h) {
+  if (!b || !h) return SG_EINVAL;
+  if (h->batch) return set_err(h, SG_ESTATE, "handle already advances in a batch");
+  if (!h->have_state) return set_err(h, SG_ESTATE, "no state uploaded");
+  if (h->adaptive) return set_err(h, SG_EINVAL, "batched handles use a fixed time step");
+  if (h->device != b->device) return set_err(h, SG_EINVAL, "handle and batch are on different devices");
+  if (b->members.size() >= 65535) return set_err(h, SG_EINVAL, "batch is full");
+  if (!b->members.empty()) {
+    const sg_handle *h0 = b->members[0];
+    if (h->P.ns != h0->P.ns || h->P.precision != h0->P.precision || h->P.bnd[0] != h0->P.bnd[0])
+      return set_err(h, SG_EINVAL, "batch members share species count, precision and x boundary kind");
+    if (!same_mechanism(h, h0)) return set_err(h, SG_EINVAL, "batch members share one mechanism");
+  }
+  // the batch stream runs after the handle's queued work (upload, TEMPIN)
+  HIPCHK(h, hipEventRecord(b->ev, h->stream));
+  HIPCHK(h, hipStreamWaitEvent(b->stream, b->ev, 0));
+  b->members.push_back(h);
+  h->batch = b;
+  b->dirty_layout = 1;
+  return 0;
+}
+
+int sg_batch_remove(sg_batch *b, sg_handle *h) {
+  if (!b || !h || h->batch != b) return SG_EINVAL;
+  // the handle's later work runs after the batch steps queued so far
+  int rc = batch_fence(b, h);
+  for (size_t m = 0; m < b->members.size(); m++)
+    if (b->members[m] == h) {
+      b->members.erase(b->members.begin() + (long)m);
+      break;
+    }
+  h->batch = nullptr;
+  b->dirty_layout = 1;
+  return rc;
+}
+
+int sg_batch_size(sg_batch *b, int *n) {
+  if (!b || !n) return SG_EINVAL;
+  *n = (int)b->members.size();
+  return 0;
+}
+
+int sg_batch_step(sg_batch *b, int nsteps) {
+  if (!b || nsteps < 0) return SG_EINVAL;
+  if (b->members.empty() || nsteps == 0) return 0;
+  int rc = batch_prepare(b);
+  if (rc) return rc;
+  if (!b->exec) {
+    rc = batch_capture(b);
+    if (rc) return rc;
+  }
+  for (int n = 0; n < nsteps; n++)
+    if (hipGraphLaunch(b->exec, b->stream) != hipSuccess) return SG_EDEVICE;
+  for (sg_handle *h : b->members)
+    for (int n = 0; n < nsteps; n++) h->etime = h->etime + h->tstep;
+  return 0;
+}
+
+int sg_batch_sync(sg_batch *b, int *failed) {
+  if (!b) return SG_EINVAL;
+  if (failed) *failed = -1;
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return SG_EDEVICE;
+  for (size_t m = 0; m < b->members.size(); m++) {
+    int rc = check_device_error(b->members[m]);
+    if (rc) {
+      if (failed) *failed = (int)m;
+      return rc;
+    }
+  }
+  return 0;
+}
+
+int sg_batch_member(sg_batch *b, int idx, sg_handle **h) {
+  if (!b || !h || idx < 0 || idx >= (int)b->members.size()) return SG_EINVAL;
+  *h = b->members[(size_t)idx];
   return 0;
 }
 

@@ -9,13 +9,11 @@ namespace sg {
 #define SG_NS_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
 template <int NS> void launch_tempin_t(const Params &, const sg_config *, const Fields &, hipStream_t);
-template <int NS> void launch_boundt_t(const Params &, const sg_config *, const Fields &, hipStream_t);
-template <int NS> void launch_bountt_t(const Params &, const sg_config *, const Fields &, hipStream_t);
-template <int NS> void launch_prim_t(const Params &, const sg_config *, const Fields &, hipStream_t);
-template <int NS> void launch_pass_t(int, const Params &, const Fields &, hipStream_t);
-template <int NS>
-void launch_final_t(const Params &, const sg_config *, const Fields &, int, double, double, double, double *,
-                    hipStream_t);
+template <int NS> void launch_boundt_t(const Launch &, hipStream_t);
+template <int NS> void launch_bountt_t(const Launch &, hipStream_t);
+template <int NS> void launch_prim_t(const Launch &, hipStream_t);
+template <int NS> void launch_pass_t(int, const Launch &, hipStream_t);
+template <int NS> void launch_final_t(const Launch &, int, double *, hipStream_t);
 
 bool species_supported(int ns) {
 #define CASE(N) if (ns == N) return true;
@@ -32,42 +30,41 @@ void launch_tempin(const Params &P, const sg_config *c, const Fields &F, hipStre
     default: break;
   }
 }
-void launch_boundt(const Params &P, const sg_config *c, const Fields &F, hipStream_t st) {
-  switch (P.ns) {
-#define CASE(N) case N: launch_boundt_t<N>(P, c, F, st); break;
+void launch_boundt(const Launch &L, hipStream_t st) {
+  switch (L.ns) {
+#define CASE(N) case N: launch_boundt_t<N>(L, st); break;
     SG_NS_LIST(CASE)
 #undef CASE
     default: break;
   }
 }
-void launch_bountt(const Params &P, const sg_config *c, const Fields &F, hipStream_t st) {
-  switch (P.ns) {
-#define CASE(N) case N: launch_bountt_t<N>(P, c, F, st); break;
+void launch_bountt(const Launch &L, hipStream_t st) {
+  switch (L.ns) {
+#define CASE(N) case N: launch_bountt_t<N>(L, st); break;
     SG_NS_LIST(CASE)
 #undef CASE
     default: break;
   }
 }
-void launch_prim(const Params &P, const sg_config *c, const Fields &F, hipStream_t st) {
-  switch (P.ns) {
-#define CASE(N) case N: launch_prim_t<N>(P, c, F, st); break;
+void launch_prim(const Launch &L, hipStream_t st) {
+  switch (L.ns) {
+#define CASE(N) case N: launch_prim_t<N>(L, st); break;
     SG_NS_LIST(CASE)
 #undef CASE
     default: break;
   }
 }
-void launch_pass(int dir, const Params &P, const Fields &F, hipStream_t st) {
-  switch (P.ns) {
-#define CASE(N) case N: launch_pass_t<N>(dir, P, F, st); break;
+void launch_pass(int dir, const Launch &L, hipStream_t st) {
+  switch (L.ns) {
+#define CASE(N) case N: launch_pass_t<N>(dir, L, st); break;
     SG_NS_LIST(CASE)
 #undef CASE
     default: break;
   }
 }
-void launch_final(const Params &P, const sg_config *c, const Fields &F, int mode, double lhs, double rhs,
-                  double erc, double *out, hipStream_t st) {
-  switch (P.ns) {
-#define CASE(N) case N: launch_final_t<N>(P, c, F, mode, lhs, rhs, erc, out, st); break;
+void launch_final(const Launch &L, int irk, double *out, hipStream_t st) {
+  switch (L.ns) {
+#define CASE(N) case N: launch_final_t<N>(L, irk, out, st); break;
     SG_NS_LIST(CASE)
 #undef CASE
     default: break;

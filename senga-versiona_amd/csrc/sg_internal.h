@@ -28,6 +28,7 @@ struct Params {
   int inflow_xl, outflow_xr;   // NSBCI3 at x-left, NSBCO1 at x-right
   int rklo[6];                 // first x index of the RK update for D,U,V,W,E,Y
   double drin, uin, cobcxr, pinfxr;
+  double yrin[SG_MAX_SPEC];    // inflow mass fractions (NSBCI3)
   int ychunk;                  // rows per y-chunk of the final kernel's point order (divides ny)
   int precision;               // 0: FP64; 1: mixed (FP32 primitives, rates, RHS; FP64 registers, T)
   double pofs;                 // stored pressure is p - pofs (mixed: the default pressure, else 0)
@@ -97,18 +98,47 @@ inline __host__ __device__ long prim_h(const Params &P, int s) { return (long)(P
 inline __host__ __device__ int nprim(int ns) { return P_NSCAL + 3 * ns; }
 inline __host__ __device__ int nface(int ns) { return F_NSCAL + ns; }
 
+// Substage launches cover any number of cases ("members") with one grid per
+// kernel kind.  The grid of a kind is the concatenation of every member's own
+// grid of that kind: flattened block b belongs to member map[kind][b] and is
+// that member's block b - blk0[kind].  A single case is a launch of one member.
+enum LaunchKind { L_FACE = 0, L_POINT, L_PASSX, L_PASSY, L_PASSZ, L_NKIND };
+
+struct Member {
+  Params P;
+  Fields F;
+  double rkl[5], rkr[5], rke[5];   // RKLHS, RKRHS, RKERR of the member's time step (src/dtinit.f:196-223)
+  int blk0[L_NKIND];               // first flattened block of the member per kind
+};
+
+// Host-side view of one launch set (device pointers).  Every member shares
+// the mechanism (c, tab), the species count, the precision and the x
+// boundary kind, which select the kernel instantiation.
+struct Launch {
+  const Member *mem;
+  const uint32_t *map[L_NKIND];
+  int nblk[L_NKIND];
+  int nmem;
+  const sg_config *c;
+  const Tables *tab;
+  int ns, precision, bndx, inflow;
+};
+
+// member block counts of each kind (sg_kernels.hip)
+void member_blocks(const Params &P, int nblk[L_NKIND]);
+
 // launchers (sg_kernels.hip)
 void launch_tempin(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st);
-void launch_boundt(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st);
-void launch_prim(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st);
-void launch_pass(int dir, const Params &P, const Fields &F, hipStream_t st);
-// mode: 0 = LINCOM (coefficients lhs, rhs), 1 = FINCOM, 2 = write RHS to out
-void launch_final(const Params &P, const sg_config *dcfg, const Fields &F, int mode, double lhs,
-                  double rhs, double erc, double *out, hipStream_t st);
+void launch_boundt(const Launch &L, hipStream_t st);
+void launch_prim(const Launch &L, hipStream_t st);
+void launch_pass(int dir, const Launch &L, hipStream_t st);
+// substage irk (1..5): LINCOM for irk < 5, FINCOM for irk = 5, with each
+// member's coefficients; mode 2 (irk = 0): write the RHS to out instead
+void launch_final(const Launch &L, int irk, double *out, hipStream_t st);
 // adaptive step: max over points and variables of |err|/(|run| + norm) into F.emax
 void launch_errmax(const Params &P, const Fields &F, hipStream_t st);
 void launch_zero(double *p, long n, hipStream_t st);
-void launch_bountt(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st);
+void launch_bountt(const Launch &L, hipStream_t st);
 void launch_derivs(const Params &P, const double *f, double *out, hipStream_t st);
 bool species_supported(int ns);
 

@@ -77,6 +77,43 @@ __device__ void set_error(int *err, int code, int i, int j, int k) {
   }
 }
 
+// Member owning flattened block gb of a launch of kind KIND, and the block's
+// index lb within that member's own grid.  The map and table are read-only
+// for the kernel's lifetime, so these are uniform scalar loads.
+// Field pointers read from the member table are generic pointers to the
+// compiler; stating that they point to global memory (a no-op conversion on
+// this target) lets every access through them use global instead of flat
+// instructions, which would otherwise also count against the LDS wait counter
+// and stall the LDS-staged stencils.
+template <class T>
+__device__ __forceinline__ T *as_global(T *p) {
+  return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
+}
+__device__ __forceinline__ Fields global_fields(const Fields &G) {
+  Fields F;
+  F.tab = as_global(G.tab);
+  F.q = as_global(G.q);
+  F.run = as_global(G.run);
+  F.prim = as_global(G.prim);
+  F.rate = as_global(G.rate);
+  F.acc = as_global(G.acc);
+  F.grad = as_global(G.grad);
+  F.itndex = as_global(G.itndex);
+  F.face = as_global(G.face);
+  F.err = as_global(G.err);
+  F.erk = as_global(G.erk);
+  F.emax = as_global(G.emax);
+  return F;
+}
+
+template <int KIND>
+__device__ __forceinline__ const Member &member_at(const Member *__restrict__ mem, const uint32_t *__restrict__ map,
+                                                   int gb, int &lb) {
+  const Member &M = mem[map[gb]];
+  lb = gb - M.blk0[KIND];
+  return M;
+}
+
 // ---------------------------------------------------------------------------
 // TEMPIN: bisection from TRIN in 50 K steps, src/tempin.f:55-512
 template <int NS, class R>
@@ -173,8 +210,13 @@ __global__ void __launch_bounds__(BLK) k_tempin(Params P, const sg_config *__res
 // ---------------------------------------------------------------------------
 // BOUNDT for NSBCI3 at x-left (src/boundt.f:177-218; BCDTXL, BCUTXL type 1, BCYTXL)
 template <int NS>
-__global__ void k_boundt(Params P, const sg_config *__restrict__ c, Fields F) {
-  long t = (long)blockIdx.x * BLK + threadIdx.x;
+__global__ void __launch_bounds__(BLK) k_boundt(const Member *__restrict__ mem, const uint32_t *__restrict__ map) {
+  int lb;
+  const Member &M = member_at<L_FACE>(mem, map, blockIdx.x, lb);
+  const Params P = M.P;
+  const Fields F = global_fields(M.F);
+  if (!P.inflow_xl) return;
+  long t = (long)lb * BLK + threadIdx.x;
   if (t >= (long)P.ny * P.nz) return;
   long p = t * P.nx;  // i = 0
   const long fs = P.fstride;
@@ -184,13 +226,18 @@ __global__ void k_boundt(Params P, const sg_config *__restrict__ c, Fields F) {
   F.q[2 * fs + p] = strd * 0.0;
   F.q[3 * fs + p] = strd * 0.0;
 #pragma unroll
-  for (int s = 0; s < NS; s++) F.q[(5 + s) * fs + p] = strd * c->yrin[s];
+  for (int s = 0; s < NS; s++) F.q[(5 + s) * fs + p] = strd * P.yrin[s];
 }
 
 // BOUNTT for NSBCI3 at x-left (src/bountt.f:195-262)
 template <int NS>
-__global__ void k_bountt(Params P, const sg_config *__restrict__ c, Fields F) {
-  long t = (long)blockIdx.x * BLK + threadIdx.x;
+__global__ void __launch_bounds__(BLK) k_bountt(const Member *__restrict__ mem, const uint32_t *__restrict__ map) {
+  int lb;
+  const Member &M = member_at<L_FACE>(mem, map, blockIdx.x, lb);
+  const Params P = M.P;
+  const Fields F = global_fields(M.F);
+  if (!P.inflow_xl) return;
+  long t = (long)lb * BLK + threadIdx.x;
   if (t >= (long)P.ny * P.nz) return;
   long p = t * P.nx;
   const long fs = P.fstride;
@@ -203,7 +250,7 @@ __global__ void k_bountt(Params P, const sg_config *__restrict__ c, Fields F) {
   }
 #pragma unroll
   for (int s = 0; s < NS; s++) {
-    double y = strd * c->yrin[s];
+    double y = strd * P.yrin[s];
     F.q[(5 + s) * fs + p] = y;
     F.run[(5 + s) * fs + p] = y;
   }
@@ -263,9 +310,13 @@ __device__ __forceinline__ double trfit(const double *a, double x) {
 }
 
 template <int NS>
-__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k_prim(Params P, const sg_config *__restrict__ c,
-                                              const Tables *__restrict__ tb, Fields F) {
-  long idx = (long)blockIdx.x * BLK + threadIdx.x;
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k_prim(const Member *__restrict__ mem,
+    const uint32_t *__restrict__ map, const sg_config *__restrict__ c, const Tables *__restrict__ tb) {
+  int lb;
+  const Member &M = member_at<L_POINT>(mem, map, blockIdx.x, lb);
+  const Params P = M.P;
+  const Fields F = global_fields(M.F);
+  long idx = (long)lb * BLK + threadIdx.x;
   if (idx >= P.npts) return;
   const uint32_t bo = (uint32_t)idx * 8u;
   const long fs = P.fstride;
@@ -644,9 +695,13 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
 // n ln T - E/RT, Gibbs sums) are assembled in FP64 before the FP32
 // exponential, since they are differences of O(10..100) terms.
 template <int NS>
-__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_prim_mx(Params P, const sg_config *__restrict__ c,
-                                              const Tables *__restrict__ tb, Fields F) {
-  long idx = (long)blockIdx.x * BLK + threadIdx.x;
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_prim_mx(const Member *__restrict__ mem,
+    const uint32_t *__restrict__ map, const sg_config *__restrict__ c, const Tables *__restrict__ tb) {
+  int lb;
+  const Member &M = member_at<L_POINT>(mem, map, blockIdx.x, lb);
+  const Params P = M.P;
+  const Fields F = global_fields(M.F);
+  long idx = (long)lb * BLK + threadIdx.x;
   if (idx >= P.npts) return;
   const uint32_t bo = (uint32_t)idx * 8u, b4 = (uint32_t)idx * 4u;
   const long fs = P.fstride;
@@ -1038,7 +1093,8 @@ struct PassTile {
 };
 
 template <int DIR, bool BND, int NS, class R>
-__global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves_per_eu(4))) k_pass(Params P, Fields F) {
+__global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves_per_eu(4))) k_pass(const Member *__restrict__ mem,
+                                                                                                   const uint32_t *__restrict__ map) {
   using T_ = PassTile<DIR>;
   constexpr int TX = T_::TX, TY = T_::TY, LX = T_::LX, LY = T_::LY, LN = T_::LN, LSTR = T_::LSTR;
   constexpr int NTH = T_::NT;
@@ -1048,8 +1104,11 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   __shared__ R tile[2][NIT * NTH];
   const int t = threadIdx.x;
   const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
-  const int nbx = gridDim.x, nby = gridDim.y;
-  const int lb = xcd_block(blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z), nbx * nby * gridDim.z);
+  int lb;
+  const Member &M = member_at<L_PASSX + DIR>(mem, map, xcd_block(blockIdx.x, gridDim.x), lb);
+  const Params P = M.P;
+  const Fields F = global_fields(M.F);
+  const int nbx = (P.nx + TX - 1) / TX, nby = (P.ny + TY - 1) / TY;
   const int x0 = (lb % nbx) * TX, y0 = ((lb / nbx) % nby) * TY, z0 = (lb / (nbx * nby)) * T_::TZ;
   const int i = x0 + lx, j = y0 + ly, k = z0 + lz;
   const bool active = i < P.nx && j < P.ny && k < P.nz;
@@ -1389,8 +1448,17 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
 // face corrections (src/bounds.f:115-206, 610-682, 1196-1458) and the RK
 // register update (src/lincom.f:53-169, src/fincom.f:54-168).
 template <bool BND, int NS, class R>
-__global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__restrict__ c, Fields F, int mode,
-                                               double lhs, double rhsc, double erc, double *__restrict__ out) {
+__global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, const uint32_t *__restrict__ map,
+                                               const sg_config *__restrict__ c, int irk, double *__restrict__ out) {
+  int lb;
+  const Member &M = member_at<L_POINT>(mem, map, xcd_block(blockIdx.x, gridDim.x), lb);
+  const Params P = M.P;
+  const Fields F = global_fields(M.F);
+  // substage irk: LINCOM (mode 0) or FINCOM at the last substage (mode 1);
+  // irk = 0 writes the right-hand side to out (mode 2)
+  const int mode = irk == 0 ? 2 : (irk < 5 ? 0 : 1);
+  const double lhs = irk ? M.rkl[irk - 1] : 0.0, rhsc = irk ? M.rkr[irk - 1] : 0.0,
+               erc = irk ? M.rke[irk - 1] : 0.0;
   // bounded x: mixed-derivative row weights in LDS (uniform row form)
   __shared__ double mtab[BND ? SG_NROW * 7 : 1];
   const int tl = threadIdx.x;
@@ -1401,7 +1469,7 @@ __global__ void __launch_bounds__(BLK) k_final(Params P, const sg_config *__rest
   // Point order: x fastest, then z, then y within chunks of P.ychunk rows,
   // so that the +-5-plane z neighbours of the mixed derivatives stay in the
   // XCD's L2 while its blocks sweep a chunk.
-  const long lp = (long)xcd_block(blockIdx.x, gridDim.x) * BLK + tl;
+  const long lp = (long)lb * BLK + tl;
   if (lp >= P.npts) return;
   int i, j, k;
   {
@@ -1668,64 +1736,57 @@ void launch_tempin_t(const Params &P, const sg_config *dcfg, const Fields &F, hi
   else hipLaunchKernelGGL((k_tempin<NS, double>), dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F);
 }
 template <int NS>
-void launch_boundt_t(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st) {
-  hipLaunchKernelGGL(k_boundt<NS>, dim3(grid_for((long)P.ny * P.nz)), dim3(BLK), 0, st, P, dcfg, F);
+void launch_boundt_t(const Launch &L, hipStream_t st) {
+  hipLaunchKernelGGL(k_boundt<NS>, dim3(L.nblk[L_FACE]), dim3(BLK), 0, st, L.mem, L.map[L_FACE]);
 }
 template <int NS>
-void launch_bountt_t(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st) {
-  hipLaunchKernelGGL(k_bountt<NS>, dim3(grid_for((long)P.ny * P.nz)), dim3(BLK), 0, st, P, dcfg, F);
+void launch_bountt_t(const Launch &L, hipStream_t st) {
+  hipLaunchKernelGGL(k_bountt<NS>, dim3(L.nblk[L_FACE]), dim3(BLK), 0, st, L.mem, L.map[L_FACE]);
 }
 template <int NS>
-void launch_prim_t(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st) {
-  if (P.precision) hipLaunchKernelGGL(k_prim_mx<NS>, dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F.tab, F);
-  else hipLaunchKernelGGL(k_prim<NS>, dim3(grid_for(P.npts)), dim3(BLK), 0, st, P, dcfg, F.tab, F);
+void launch_prim_t(const Launch &L, hipStream_t st) {
+  const dim3 g(L.nblk[L_POINT]), b(BLK);
+  if (L.precision) hipLaunchKernelGGL(k_prim_mx<NS>, g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
+  else hipLaunchKernelGGL(k_prim<NS>, g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
 }
-template <int DIR>
-static dim3 pass_grid(const Params &P) {
-  using T_ = PassTile<DIR>;
-  const unsigned gx = (P.nx + T_::TX - 1) / T_::TX, gy = (P.ny + T_::TY - 1) / T_::TY,
-                 gz = (P.nz + T_::TZ - 1) / T_::TZ;
-  return dim3(gx, gy, gz);
-}
-
 template <int NS, class R>
-static void launch_pass_r(int dir, const Params &P, const Fields &F, hipStream_t st) {
+static void launch_pass_r(int dir, const Launch &L, hipStream_t st) {
+  const int k = L_PASSX + dir;
   if (dir == 0) {
-    dim3 b(PassTile<0>::NT);
-    if (P.bnd[0]) hipLaunchKernelGGL((k_pass<0, true, NS, R>), pass_grid<0>(P), b, 0, st, P, F);
-    else hipLaunchKernelGGL((k_pass<0, false, NS, R>), pass_grid<0>(P), b, 0, st, P, F);
+    const dim3 b(PassTile<0>::NT);
+    if (L.bndx) hipLaunchKernelGGL((k_pass<0, true, NS, R>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
+    else hipLaunchKernelGGL((k_pass<0, false, NS, R>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
   } else if (dir == 1) {
-    hipLaunchKernelGGL((k_pass<1, false, NS, R>), pass_grid<1>(P), dim3(PassTile<1>::NT), 0, st, P, F);
+    hipLaunchKernelGGL((k_pass<1, false, NS, R>), dim3(L.nblk[k]), dim3(PassTile<1>::NT), 0, st, L.mem, L.map[k]);
   } else {
-    hipLaunchKernelGGL((k_pass<2, false, NS, R>), pass_grid<2>(P), dim3(PassTile<2>::NT), 0, st, P, F);
+    hipLaunchKernelGGL((k_pass<2, false, NS, R>), dim3(L.nblk[k]), dim3(PassTile<2>::NT), 0, st, L.mem, L.map[k]);
   }
 }
 template <int NS>
-void launch_pass_t(int dir, const Params &P, const Fields &F, hipStream_t st) {
-  if (P.precision) launch_pass_r<NS, float>(dir, P, F, st);
-  else launch_pass_r<NS, double>(dir, P, F, st);
+void launch_pass_t(int dir, const Launch &L, hipStream_t st) {
+  if (L.precision) launch_pass_r<NS, float>(dir, L, st);
+  else launch_pass_r<NS, double>(dir, L, st);
 }
 template <int NS>
-void launch_final_t(const Params &P, const sg_config *dcfg, const Fields &F, int mode, double lhs,
-                    double rhs, double erc, double *out, hipStream_t st) {
-  dim3 g(grid_for(P.npts)), b(BLK);
-  if (P.precision) {
-    if (P.bnd[0]) hipLaunchKernelGGL((k_final<true, NS, float>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, erc, out);
-    else hipLaunchKernelGGL((k_final<false, NS, float>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, erc, out);
+void launch_final_t(const Launch &L, int irk, double *out, hipStream_t st) {
+  const dim3 g(L.nblk[L_POINT]), b(BLK);
+  const uint32_t *map = L.map[L_POINT];
+  if (L.precision) {
+    if (L.bndx) hipLaunchKernelGGL((k_final<true, NS, float>), g, b, 0, st, L.mem, map, L.c, irk, out);
+    else hipLaunchKernelGGL((k_final<false, NS, float>), g, b, 0, st, L.mem, map, L.c, irk, out);
   } else {
-    if (P.bnd[0]) hipLaunchKernelGGL((k_final<true, NS, double>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, erc, out);
-    else hipLaunchKernelGGL((k_final<false, NS, double>), g, b, 0, st, P, dcfg, F, mode, lhs, rhs, erc, out);
+    if (L.bndx) hipLaunchKernelGGL((k_final<true, NS, double>), g, b, 0, st, L.mem, map, L.c, irk, out);
+    else hipLaunchKernelGGL((k_final<false, NS, double>), g, b, 0, st, L.mem, map, L.c, irk, out);
   }
 }
 
 #ifdef SG_NS
 template void launch_tempin_t<SG_NS>(const Params &, const sg_config *, const Fields &, hipStream_t);
-template void launch_boundt_t<SG_NS>(const Params &, const sg_config *, const Fields &, hipStream_t);
-template void launch_bountt_t<SG_NS>(const Params &, const sg_config *, const Fields &, hipStream_t);
-template void launch_prim_t<SG_NS>(const Params &, const sg_config *, const Fields &, hipStream_t);
-template void launch_pass_t<SG_NS>(int, const Params &, const Fields &, hipStream_t);
-template void launch_final_t<SG_NS>(const Params &, const sg_config *, const Fields &, int, double, double,
-                                    double, double *, hipStream_t);
+template void launch_boundt_t<SG_NS>(const Launch &, hipStream_t);
+template void launch_bountt_t<SG_NS>(const Launch &, hipStream_t);
+template void launch_prim_t<SG_NS>(const Launch &, hipStream_t);
+template void launch_pass_t<SG_NS>(int, const Launch &, hipStream_t);
+template void launch_final_t<SG_NS>(const Launch &, int, double *, hipStream_t);
 #endif
 
 #ifdef SG_DERIVS
@@ -1761,6 +1822,17 @@ __global__ void __launch_bounds__(BLK) k_errmax(Params P, Fields F) {
 __global__ void __launch_bounds__(BLK) k_zero(double *p, long n) {
   const long idx = (long)blockIdx.x * BLK + threadIdx.x;
   if (idx < n) p[idx] = 0.0;
+}
+
+void member_blocks(const Params &P, int nblk[L_NKIND]) {
+  nblk[L_FACE] = (int)grid_for((long)P.ny * P.nz);
+  nblk[L_POINT] = (int)grid_for(P.npts);
+  auto tiles = [&](int tx, int ty, int tz) {
+    return ((P.nx + tx - 1) / tx) * ((P.ny + ty - 1) / ty) * ((P.nz + tz - 1) / tz);
+  };
+  nblk[L_PASSX] = tiles(PassTile<0>::TX, PassTile<0>::TY, PassTile<0>::TZ);
+  nblk[L_PASSY] = tiles(PassTile<1>::TX, PassTile<1>::TY, PassTile<1>::TZ);
+  nblk[L_PASSZ] = tiles(PassTile<2>::TX, PassTile<2>::TY, PassTile<2>::TZ);
 }
 
 void launch_errmax(const Params &P, const Fields &F, hipStream_t st) {

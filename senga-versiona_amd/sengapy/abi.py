@@ -167,7 +167,9 @@ def _dp(a):
 ABI_FUNCS = ["sg_abi_version", "sg_config_size", "sg_create", "sg_destroy", "sg_upload_state",
              "sg_download_state", "sg_set_timestep", "sg_substage", "sg_end_step", "sg_step",
              "sg_rhs", "sg_derivatives", "sg_sync", "sg_last_error", "sg_get_stream",
-             "sg_profile", "sg_profile_read", "sg_memory", "sg_set_adaptive", "sg_get_timestep"]
+             "sg_profile", "sg_profile_read", "sg_memory", "sg_set_adaptive", "sg_get_timestep",
+             "sg_batch_create", "sg_batch_destroy", "sg_batch_add", "sg_batch_remove", "sg_batch_size",
+             "sg_batch_member", "sg_batch_step", "sg_batch_sync", "sg_footprint"]
 
 
 def load_library(path=None):
@@ -199,10 +201,31 @@ def load_library(path=None):
     lib.sg_memory.argtypes = [vp, P(C.c_size_t)]
     lib.sg_set_adaptive.argtypes = [vp, C.c_int, C.c_double, C.c_double]
     lib.sg_get_timestep.argtypes = [vp, dp, dp, dp]
+    lib.sg_footprint.argtypes = [P(Config), P(C.c_size_t)]
+    lib.sg_batch_create.argtypes = [P(vp)]
+    lib.sg_batch_destroy.argtypes = [vp]
+    lib.sg_batch_add.argtypes = [vp, vp]
+    lib.sg_batch_remove.argtypes = [vp, vp]
+    lib.sg_batch_size.argtypes = [vp, P(C.c_int)]
+    lib.sg_batch_member.argtypes = [vp, C.c_int, P(vp)]
+    lib.sg_batch_step.argtypes = [vp, C.c_int]
+    lib.sg_batch_sync.argtypes = [vp, P(C.c_int)]
     for f in ABI_FUNCS:
         if f not in ("sg_config_size",):
             getattr(lib, f).restype = C.c_int
     return lib
+
+
+def footprint(case, precision=0, lib=None):
+    """Device bytes a Solver for this case allocates (sg_footprint; no device
+    access)."""
+    lib = lib or load_library()
+    cfg = config_from_case(case, precision)
+    b = C.c_size_t(0)
+    rc = lib.sg_footprint(C.byref(cfg), C.byref(b))
+    if rc:
+        raise SgError(rc, "sg_footprint: configuration rejected")
+    return b.value
 
 
 class SgError(RuntimeError):
@@ -314,3 +337,65 @@ class Solver:
         b = C.c_size_t(0)
         self._chk(self.lib.sg_memory(self.h, C.byref(b)))
         return b.value
+
+
+class BatchError(SgError):
+    def __init__(self, code, msg, member=-1, ijk=(0, 0, 0)):
+        super().__init__(code, msg, ijk)
+        self.member = member
+
+
+class Batch:
+    """Cases advanced together (sg_batch_*): one launch per substage kernel
+    over all members, one captured graph per RK step."""
+
+    def __init__(self, lib=None):
+        self.lib = lib or load_library()
+        b = C.c_void_p()
+        rc = self.lib.sg_batch_create(C.byref(b))
+        if rc != 0:
+            raise SgError(rc, "sg_batch_create failed")
+        self.b = b
+        self.members = []
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.sg_batch_destroy(self.b)
+            self.b = None
+            self.members = []
+
+    def __del__(self):
+        self.close()
+
+    def add(self, solver):
+        solver._chk(self.lib.sg_batch_add(self.b, solver.h))
+        self.members.append(solver)
+
+    def remove(self, solver):
+        solver._chk(self.lib.sg_batch_remove(self.b, solver.h))
+        self.members.remove(solver)
+
+    def size(self):
+        n = C.c_int(0)
+        rc = self.lib.sg_batch_size(self.b, C.byref(n))
+        if rc:
+            raise SgError(rc, "sg_batch_size failed")
+        return n.value
+
+    def step(self, n=1):
+        rc = self.lib.sg_batch_step(self.b, n)
+        if rc:
+            raise SgError(rc, "sg_batch_step failed")
+
+    def sync(self):
+        failed = C.c_int(-1)
+        rc = self.lib.sg_batch_sync(self.b, C.byref(failed))
+        if rc:
+            m = failed.value
+            if 0 <= m < len(self.members):
+                try:
+                    self.members[m]._chk(rc)
+                except SgError as e:
+                    raise BatchError(rc, str(e), m, e.ijk) from None
+            raise BatchError(rc, "sg_batch_sync failed", m)
+

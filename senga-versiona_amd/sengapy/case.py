@@ -281,3 +281,53 @@ def h2_kernel(n=(64, 64, 64), length=(4.0e-3, 4.0e-3, 4.0e-3), seed=11, phi=0.8,
     return Case(f"h2_kernel_{nx}x{ny}x{nz}", ctl, mech, tab, dif, Dump(st, 0.0, dt),
                 os.path.join(DATA, "h2_9sp", "chem.dat"),
                 os.path.join(DATA, "h2_9sp", "diff.dat") if mixavg else None)
+
+
+# ---------------------------------------------------------------------------
+# Case sets for batched and served runs.  Each spec is a small dict drawn from
+# a seeded generator; make_case(spec) builds the initial dump on demand.
+H2_SPACING = 5.0e-3 / 127   # grid spacing of the 128^3, 5 mm box
+
+
+def batch_specs(count=64, seed=2024, sizes=(48, 64, 96, 128)):
+    """Uneven H2-air inflow/outflow flames: n^3 with n drawn from sizes,
+    phi ~ U(0.5, 1.2), u'/S_L ~ U(1, 10), T_u ~ U(300, 700) K, each with its
+    own turbulence seed; the box is (n - 1) grid spacings of the 128^3 case."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(count):
+        n = int(rng.choice(sizes))
+        out.append(dict(kind="nscbc", n=(n, n, n), phi=float(rng.uniform(0.5, 1.2)),
+                        uprime_over_sl=float(rng.uniform(1.0, 10.0)), Tu=float(rng.uniform(300.0, 700.0)),
+                        seed=int(rng.integers(1, 2 ** 31 - 1))))
+    return out
+
+
+def kernel_specs(count=32, seed=4242, rate=4.0, sizes=(48, 64, 80, 96, 112, 128, 144, 160), nsteps=200):
+    """Flame-kernel requests: Poisson arrivals of mean rate per second (arrival
+    times in seconds from 0), n^3 with n drawn from sizes, T_peak ~ U(1500,
+    2000) K, radius ~ U(0.5, 2) mm, u'/S_L ~ U(1, 10), phi ~ U(0.5, 1.2)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for q in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        n = int(rng.choice(sizes))
+        out.append(dict(kind="kernel", n=(n, n, n), arrival=t, t_peak=float(rng.uniform(1500.0, 2000.0)),
+                        radius=float(rng.uniform(0.5e-3, 2.0e-3)), phi=float(rng.uniform(0.5, 1.2)),
+                        uprime_over_sl=float(rng.uniform(1.0, 10.0)), seed=int(rng.integers(1, 2 ** 31 - 1)),
+                        nsteps=nsteps))
+    return out
+
+
+def make_case(spec) -> Case:
+    n = tuple(spec["n"])
+    length = tuple((m - 1) * H2_SPACING for m in n)
+    if spec["kind"] == "nscbc":
+        return h2_nscbc(n=n, length=length, seed=spec["seed"], phi=spec["phi"],
+                        uprime_over_sl=spec["uprime_over_sl"], Tu=spec["Tu"], nsteps=spec.get("nsteps", 100))
+    if spec["kind"] == "kernel":
+        return h2_kernel(n=n, length=length, seed=spec["seed"], phi=spec["phi"], t_peak=spec["t_peak"],
+                         radius=spec["radius"], uprime_over_sl=spec["uprime_over_sl"],
+                         nsteps=spec.get("nsteps", 200))
+    raise ValueError(spec["kind"])
