@@ -238,6 +238,59 @@ def serve_run(cases, specs, steps):
     return out
 
 
+def precision_study(abi, h, c, steps):
+    """FP64 and mixed runs of the same case for `steps` RK steps: throughput of
+    each and, per field, the max and L2 relative error of the mixed state
+    against the FP64 state (normalised by the field's max and L2 norm)."""
+    import numpy as np
+    n3 = c.shape[0] * c.shape[1] * c.shape[2]
+    res = {}
+    states = {}
+    for tag, prec in (("fp64", 0), ("mixed", 1)):
+        s = abi.Solver(c, precision=prec)
+        s.set_timestep(c.dump.tstep)
+        s.upload(c.dump.state, 0.0)
+        s.sync()
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:   # progress lines every 10 steps
+            k = min(10, steps - done)
+            s.step(k)
+            s.sync()
+            done += k
+            print(f"{tag}: {done}/{steps} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        dt = time.perf_counter() - t0
+        res[tag] = {"mpoint_substages_per_s": n3 * 5 * steps / dt / 1e6, "s_per_step": dt / steps,
+                    "device_bytes": s.memory()}
+        states[tag] = s.download()[:3]
+        s.close()
+    names = ["rho", "rho_u", "rho_v", "rho_w", "rho_E"] + [f"rho_Y_{k}" for k in range(c.nspec)]
+    err = {}
+    r64, T64, P64 = states["fp64"]
+    r32, T32, P32 = states["mixed"]
+
+    def both(x, y):
+        d = x - y
+        return {"max_rel": float(np.max(np.abs(d)) / max(float(np.max(np.abs(y))), 1e-300)),
+                "l2_rel": float(np.sqrt(np.sum(d * d)) / max(float(np.sqrt(np.sum(y * y))), 1e-300))}
+
+    for v, nm in enumerate(names):
+        err[nm] = both(r32[v], r64[v])
+    err["T"] = both(T32, T64)
+    err["p"] = both(P32, P64)
+    # mass fractions Y_k = rho Y_k / rho
+    ymax = 0.0
+    for k in range(c.nspec):
+        e = both(r32[5 + k] / r32[0], r64[5 + k] / r64[0])
+        err[f"Y_{k}"] = e
+        ymax = max(ymax, e["max_rel"])
+    res["speedup_mixed"] = res["mixed"]["mpoint_substages_per_s"] / res["fp64"]["mpoint_substages_per_s"]
+    res["errors_mixed_vs_fp64"] = err
+    res["max_rel_T"] = err["T"]["max_rel"]
+    res["max_rel_Y"] = ymax
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,6 +310,9 @@ def main():
                     help="flame-kernel requests of the served run (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--study-steps", type=int, default=0,
+                    help="instead of the throughput run: FP64 vs mixed on the --n case for this many "
+                         "steps, per-field errors")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (multi-GPU sharding is "
@@ -264,6 +320,17 @@ def main():
         sys.exit(2)
     from sengapy import abi
     from sengapy import case as Cs
+    if a.study_steps > 0:
+        n = a.n
+        t0 = time.perf_counter()
+        c = Cs.h2_nscbc(n=(n, n, n), length=(5.0e-3 * (n - 1) / 127,) * 3, mixavg=True, nsteps=a.study_steps)
+        print(f"case built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        out = {"study": "precision", "workload": f"H2-air 9-species NSCBC mixture-averaged {n}^3, grid spacing of "
+                                                 f"the 128^3 / 5 mm case, {a.study_steps} RK steps",
+               "grid": [n, n, n], "steps": a.study_steps, "tstep": c.dump.tstep}
+        out.update(precision_study(abi, hip(), c, a.study_steps))
+        print(json.dumps(out), flush=True)
+        return
     bspecs = Cs.batch_specs(a.batch_cases) if a.batch_cases > 0 else []
     sspecs = Cs.kernel_specs(a.serve_requests, rate=a.serve_rate, nsteps=a.serve_steps) if a.serve_requests > 0 else []
     built = make_cases(bspecs + sspecs) if bspecs or sspecs else []
