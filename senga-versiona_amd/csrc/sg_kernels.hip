@@ -1101,7 +1101,16 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   // LN staged values per field, padded to whole block-sized rows so that
   // every thread loads and stores unconditionally
   constexpr int NIT = (LN + NTH - 1) / NTH;
-  __shared__ R tile[2][NIT * NTH];
+  // Two buffers of up to STW fields each: a stage stages one field, or the
+  // three fields of a species (Y_k, rho D_k, h_k) under one barrier.  The
+  // FP64 x pass stages one field at a time: the extra prefetch registers
+  // would spill there.
+  constexpr bool SPW = !(DIR == 0 && sizeof(R) == 8);
+  // The velocity, temperature and flux stages are grouped the same way where
+  // that does not spill (measured: the y pass, and the FP32 z pass).
+  constexpr bool EMW = DIR == 1 || (DIR == 2 && sizeof(R) == 4);
+  constexpr int STW = SPW ? 3 : 1;
+  __shared__ R tile[2][STW][NIT * NTH];
   const int t = threadIdx.x;
   const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
   int lb;
@@ -1164,32 +1173,42 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   // field into registers; stage() stores them into the current buffer and
   // synchronises.  Each stage's loads are thus in flight while the previous
   // stage's stencils are evaluated.
-  R pf[NIT];
-  auto prefetch = [&](auto fn) {
+  R pf[STW][NIT];
+  auto prefetch_to = [&](int f, auto fn) {
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
       const int l = t + it * NTH;
-      pf[it] = (R)fn(gsrc(l < LN ? l : LN - 1));
+      pf[f][it] = (R)fn(gsrc(l < LN ? l : LN - 1));
     }
   };
-  auto stage = [&]() {
-    R *tb = tile[buf];
+  auto prefetch = [&](auto fn) { prefetch_to(0, fn); };
+  auto stage_n = [&](int nf) {
 #pragma unroll
-    for (int it = 0; it < NIT; it++) tb[t + it * NTH] = pf[it];
+    for (int f = 0; f < STW; f++)
+      if (f < nf) {
+#pragma unroll
+        for (int it = 0; it < NIT; it++) tile[buf][f][t + it * NTH] = pf[f][it];
+      }
     __syncthreads();
   };
-  auto at = [&](int kk) { return tile[buf][lbase + (kk - 8) * LSTR]; };
-  auto ctr = [&]() { return tile[buf][lbase]; };
-  auto fq = [&](int q) { return tile[buf][lbase + q * LSTR]; };
+  auto stage = [&]() { stage_n(1); };
   auto wq = [&](int q) { return wtab[BND ? row : 0][q]; };
-  auto D1 = [&]() {
+  auto D1f = [&](int f) {
+    auto at = [&](int kk) { return tile[buf][f][lbase + (kk - 8) * LSTR]; };
+    auto fq = [&](int q) { return tile[buf][f][lbase + q * LSTR]; };
     if (BND && edge) return d1_rows<R>(P, pk, DIR, wq, fq);
     return d1<false, R>(P, S, DIR, at);
   };
-  auto D2 = [&]() {
+  auto D2f = [&](int f) {
+    auto at = [&](int kk) { return tile[buf][f][lbase + (kk - 8) * LSTR]; };
+    auto fq = [&](int q) { return tile[buf][f][lbase + q * LSTR]; };
     if (BND && edge) return d2_rows<R>(P, pk, DIR, wq, fq);
     return d2<false, R>(P, S, DIR, at);
   };
+  auto ctrf = [&](int f) { return tile[buf][f][lbase]; };
+  auto D1 = [&]() { return D1f(0); };
+  auto D2 = [&]() { return D2f(0); };
+  auto ctr = [&]() { return ctrf(0); };
   auto next = [&]() { buf ^= 1; };
   const long fs = P.fstride;
   const double *Q = F.q;
@@ -1218,19 +1237,11 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   __shared__ R tmd[NIT * NTH];
   auto stage_md = [&]() {
 #pragma unroll
-    for (int it = 0; it < NIT; it++) tmd[t + it * NTH] = pf[it];
+    for (int it = 0; it < NIT; it++) tmd[t + it * NTH] = pf[0][it];
     __syncthreads();
   };
   auto atm = [&](int kk) { return tmd[lbase + (kk - 8) * LSTR]; };
   auto fqm = [&](int q) { return tmd[lbase + q * LSTR]; };
-  auto atp = [&](int kk) {
-    const int l = lbase + (kk - 8) * LSTR;
-    return tile[buf][l] * tmd[l];
-  };
-  auto fqp = [&](int q) {
-    const int l = lbase + q * LSTR;
-    return tile[buf][l] * tmd[l];
-  };
   auto D1M = [&]() {
     if (BND && edge) return d1_rows<R>(P, pk, DIR, wq, fqm);
     return d1<false, R>(P, S, DIR, atm);
@@ -1238,7 +1249,15 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   // The product stencil reads twice the operands of a plain one; its loads
   // are kept from being hoisted over the preceding stencils so that they do
   // not hold registers alongside them.
-  auto D1P = [&]() {
+  auto D1Pf = [&](int f) {
+    auto atp = [&](int kk) {
+      const int l = lbase + (kk - 8) * LSTR;
+      return tile[buf][f][l] * tmd[l];
+    };
+    auto fqp = [&](int q) {
+      const int l = lbase + q * LSTR;
+      return tile[buf][f][l] * tmd[l];
+    };
     SG_PHASE();
     R r;
     if (BND && edge) r = d1_rows<R>(P, pk, DIR, wq, fqp);
@@ -1247,80 +1266,182 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
     SG_PHASE();
     return r;
   };
+  auto D1P = [&]() { return D1Pf(0); };
   // Momentum fluxes as the reference forms them: rho u*u, rho u*v, rho u*w,
   // rho v*v, rho v*w, rho w*w.  Those of the form rho u_d * u_b come from the
   // resident tile in the u_b stage; the others (rho u*v in y; rho u*w and
   // rho v*w in z) are staged as products.
-  auto f_y = [&](int sp) { const R *Y_ = pr + prim_y(P, sp); return [Y_](long g) { return Y_[g]; }; };
   R flux[3];
-  // ---------------- continuity and energy convection (src/rhscal.f:141-415)
+  R dm, md0, aD, aE, dp, p0, dmu, mu0;
+  R du[3], d2u[3], u0[3];
+  auto prefetch_species = [&](int sp) {
+    const R *Ys = pr + prim_y(P, sp), *Rs = pr + prim_rd(P, sp), *Hs = pr + prim_h(P, sp);
+    prefetch_to(0, [Ys](long g) { return Ys[g]; });
+    if constexpr (SPW) {
+      prefetch_to(1, [Rs](long g) { return Rs[g]; });
+      prefetch_to(2, [Hs](long g) { return Hs[g]; });
+    }
+  };
+  auto f_pp = [&](long g) { return pp[g] - pref; };
+  auto f_mu = [&](long g) { return mu[g]; };
   prefetch([&](long g) { return md[g]; });
   stage_md();
-  prefetch([&](long g) { return ee[g]; });
-  R dm = D1M(), md0 = tmd[lbase];
-  pin(dm); pin(md0);
-  R aD = -dm;
-  stage();
-  prefetch([&](long g) { return TT[g]; });
-  R dem = D1P();
-  const R de = D1();
-  const R e0 = ctr();
-  next();
-  R aE = -R(0.5) * (e0 * dm + dem + md0 * de);
-  pin(aE);
-  // ---------------- heat conduction (src/rhscal.f:427-797)
-  {
+  if constexpr (EMW) {
+    // e, T and lambda under one barrier, u, v and w under the next, then the
+    // fluxes without rho u_d, p - p_ref and mu (z: mu after them)
+    prefetch_to(0, [&](long g) { return ee[g]; });
+    prefetch_to(1, [&](long g) { return TT[g]; });
+    prefetch_to(2, [&](long g) { return la[g]; });
+    dm = D1M(); md0 = tmd[lbase];
+    pin(dm); pin(md0);
+    aD = -dm;
+    stage_n(3);
+    prefetch_to(0, [&](long g) { return uu[g]; });
+    prefetch_to(1, [&](long g) { return vv[g]; });
+    prefetch_to(2, [&](long g) { return ww[g]; });
+    // ---------------- continuity and energy convection (src/rhscal.f:141-415)
+    {
+      R dem = D1Pf(0);
+      const R de = D1f(0);
+      const R e0 = ctrf(0);
+      aE = -R(0.5) * (e0 * dm + dem + md0 * de);
+      pin(aE);
+    }
+    SG_PHASE();
+    // ---------------- heat conduction (src/rhscal.f:427-797)
+    {
+      R dT = D1f(1), d2T = D2f(1);
+      pin(dT); pin(d2T);
+      SG_PHASE();
+      const R dl = D1f(2);
+      const R lam0 = ctrf(2);
+      aE = aE + (zcon ? R(0) : dl) * dT;
+      aE = aE + lam0 * (zcon ? R(0) : d2T);
+      pin(aE);
+    }
+    next();
+    // ---------------- velocity derivatives and rho u_d u_b fluxes
+    stage_n(3);
+    if (DIR == 0) {
+      prefetch_to(0, f_pp);
+      prefetch_to(1, f_mu);
+    } else if (DIR == 1) {
+      prefetch_to(0, [&](long g) { return mx[g] * vv[g]; });
+      prefetch_to(1, f_pp);
+      prefetch_to(2, f_mu);
+    } else {
+      prefetch_to(0, [&](long g) { return mx[g] * ww[g]; });
+      prefetch_to(1, [&](long g) { return my[g] * ww[g]; });
+      prefetch_to(2, f_pp);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      du[a] = D1f(a); d2u[a] = D2f(a); u0[a] = ctrf(a);
+      pin(du[a]); pin(d2u[a]); pin(u0[a]);
+      if (a >= DIR) {
+        flux[a] = D1Pf(a);
+        pin(flux[a]);
+      }
+      SG_PHASE();
+    }
+    next();
+    stage_n(DIR == 0 ? 2 : 3);
+    if (DIR == 2) prefetch(f_mu);
+    else prefetch_species(0);
+    if (DIR == 0) {
+      dp = D1f(0); p0 = ctrf(0);
+      dmu = D1f(1); mu0 = ctrf(1);
+    } else if (DIR == 1) {
+      flux[0] = D1f(0);
+      dp = D1f(1); p0 = ctrf(1);
+      dmu = D1f(2); mu0 = ctrf(2);
+    } else {
+      flux[0] = D1f(0); flux[1] = D1f(1);
+      dp = D1f(2); p0 = ctrf(2);
+    }
+    next();
+    if (DIR == 2) {
+      stage();
+      prefetch_species(0);
+      dmu = D1(); mu0 = ctr();
+      next();
+    }
+  } else {
+    prefetch([&](long g) { return ee[g]; });
+    dm = D1M(); md0 = tmd[lbase];
+    pin(dm); pin(md0);
+    aD = -dm;
     stage();
-    prefetch([&](long g) { return la[g]; });
-    R dT = D1(), d2T = D2();
-    pin(dT); pin(d2T);
+    prefetch([&](long g) { return TT[g]; });
+    {
+      R dem = D1P();
+      const R de = D1();
+      const R e0 = ctr();
+      next();
+      aE = -R(0.5) * (e0 * dm + dem + md0 * de);
+      pin(aE);
+    }
+    {
+      stage();
+      prefetch([&](long g) { return la[g]; });
+      R dT = D1(), d2T = D2();
+      pin(dT); pin(d2T);
+      next();
+      stage();
+      prefetch([&](long g) { return uu[g]; });
+      const R dl = D1();
+      const R lam0 = ctr();
+      next();
+      aE = aE + (zcon ? R(0) : dl) * dT;
+      aE = aE + lam0 * (zcon ? R(0) : d2T);
+      pin(aE);
+    }
+    stage();
+    prefetch([&](long g) { return vv[g]; });
+    du[0] = D1(); d2u[0] = D2(); u0[0] = ctr();
+    if (DIR == 0) flux[0] = D1P();
+    pin(du[0]); pin(d2u[0]); pin(u0[0]);
     next();
     stage();
-    prefetch([&](long g) { return uu[g]; });
-    const R dl = D1();
-    const R lam0 = ctr();
+    prefetch([&](long g) { return ww[g]; });
+    du[1] = D1(); d2u[1] = D2(); u0[1] = ctr();
+    if (DIR <= 1) flux[1] = D1P();
+    pin(du[1]); pin(d2u[1]); pin(u0[1]);
     next();
-    aE = aE + (zcon ? R(0) : dl) * dT;
-    aE = aE + lam0 * (zcon ? R(0) : d2T);
-    pin(aE);
+    stage();
+    if (DIR == 0) prefetch(f_pp);
+    else if (DIR == 1) prefetch([&](long g) { return mx[g] * vv[g]; });
+    else prefetch([&](long g) { return mx[g] * ww[g]; });
+    du[2] = D1(); d2u[2] = D2(); u0[2] = ctr();
+    flux[2] = D1P();
+    pin(du[2]); pin(d2u[2]); pin(u0[2]);
+    next();
+    if (DIR == 1) {
+      stage();
+      prefetch(f_pp);
+      flux[0] = D1();
+      next();
+    } else if (DIR == 2) {
+      stage();
+      prefetch([&](long g) { return my[g] * ww[g]; });
+      flux[0] = D1();
+      next();
+      stage();
+      prefetch(f_pp);
+      flux[1] = D1();
+      next();
+    }
+    stage();
+    prefetch(f_mu);
+    dp = D1(); p0 = ctr();
+    pin(dp); pin(p0);
+    next();
+    stage();
+    prefetch_species(0);
+    dmu = D1(); mu0 = ctr();
+    next();
   }
-  // ---------------- velocity derivatives and rho u_d u_b fluxes
-  R du[3], d2u[3], u0[3];
-  stage();
-  prefetch([&](long g) { return vv[g]; });
-  du[0] = D1(); d2u[0] = D2(); u0[0] = ctr();
-  if (DIR == 0) flux[0] = D1P();
-  pin(du[0]); pin(d2u[0]); pin(u0[0]);
-  next();
-  stage();
-  prefetch([&](long g) { return ww[g]; });
-  du[1] = D1(); d2u[1] = D2(); u0[1] = ctr();
-  if (DIR <= 1) flux[1] = D1P();
-  pin(du[1]); pin(d2u[1]); pin(u0[1]);
-  next();
-  stage();
-  if (DIR == 0) prefetch([&](long g) { return pp[g] - pref; });
-  else if (DIR == 1) prefetch([&](long g) { return mx[g] * vv[g]; });
-  else prefetch([&](long g) { return mx[g] * ww[g]; });
-  du[2] = D1(); d2u[2] = D2(); u0[2] = ctr();
-  flux[2] = D1P();
-  pin(du[2]); pin(d2u[2]); pin(u0[2]);
-  next();
-  if (DIR == 1) {
-    stage();
-    prefetch([&](long g) { return pp[g] - pref; });
-    flux[0] = D1();
-    next();
-  } else if (DIR == 2) {
-    stage();
-    prefetch([&](long g) { return my[g] * ww[g]; });
-    flux[0] = D1();
-    next();
-    stage();
-    prefetch([&](long g) { return pp[g] - pref; });
-    flux[1] = D1();
-    next();
-  }
+  pin(dmu); pin(mu0);
   // ---------------- momentum convection (src/rhsvel.f:348-729, 1312-2110)
   R aM[3];
   {
@@ -1334,11 +1455,7 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   }
   // ---------------- pressure gradient and pressure work (src/rhsvel.f:741-917)
   {
-    stage();
-    prefetch([&](long g) { return mu[g]; });
-    const R dp = D1();
-    const R p0 = ctr() + pref + (R)P.pofs;   // stored pressure is p - pofs
-    next();
+    p0 = p0 + pref + (R)P.pofs;   // stored pressure is p - pofs
     aM[DIR] = aM[DIR] - dp;
     aE = aE - u0[DIR] * dp - p0 * du[DIR];
     pin(aM[DIR]); pin(aE);
@@ -1349,11 +1466,6 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
     }
   }
   // ---------------- separable viscous terms (src/rhsvel.f:1462-2305)
-  stage();
-  prefetch(f_y(0));
-  R dmu = D1(), mu0 = ctr();
-  pin(dmu); pin(mu0);
-  next();
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     R tv;
@@ -1376,24 +1488,42 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   R aY[NS], dYz[NS], yv[NS];
 #pragma unroll
   for (int sp = 0; sp < NS; sp++) {
-    const R *Ys = pr + prim_y(P, sp), *Rs = pr + prim_rd(P, sp), *Hs = pr + prim_h(P, sp);
-    stage();
-    prefetch([&](long g) { return Rs[g]; });
-    R dY = D1(), d2Y = D2(), yc = ctr();
-    R dYm = D1P();
-    pin(dY); pin(d2Y); pin(yc); pin(dYm);
-    next();
-    stage();
-    prefetch([&](long g) { return Hs[g]; });
-    R drD = D1(), rdc = ctr();
-    pin(drD); pin(rdc);
-    next();
-    stage();
-    if (sp + 1 < NS) prefetch(f_y(sp + 1));
-    else prefetch([&](long g) { return hm[g]; });
-    const R dh = D1();
-    const R hc = ctr();
-    next();
+    R dY, d2Y, yc, dYm, drD, rdc, dh, hc;
+    if constexpr (SPW) {
+      stage_n(3);
+      if (sp + 1 < NS) prefetch_species(sp + 1);
+      else prefetch([&](long g) { return hm[g]; });
+      dY = D1f(0); d2Y = D2f(0); yc = ctrf(0);
+      pin(dY); pin(d2Y); pin(yc);
+      dYm = D1P();
+      pin(dYm);
+      SG_PHASE();
+      drD = D1f(1); rdc = ctrf(1);
+      pin(drD); pin(rdc);
+      SG_PHASE();
+      dh = D1f(2);
+      hc = ctrf(2);
+      next();
+    } else {
+      const R *Rs = pr + prim_rd(P, sp), *Hs = pr + prim_h(P, sp);
+      stage();
+      prefetch([&](long g) { return Rs[g]; });
+      dY = D1(); d2Y = D2(); yc = ctr();
+      dYm = D1P();
+      pin(dY); pin(d2Y); pin(yc); pin(dYm);
+      next();
+      stage();
+      prefetch([&](long g) { return Hs[g]; });
+      drD = D1(); rdc = ctr();
+      pin(drD); pin(rdc);
+      next();
+      stage();
+      if (sp + 1 < NS) prefetch_species(sp + 1);
+      else prefetch([&](long g) { return hm[g]; });
+      dh = D1();
+      hc = ctr();
+      next();
+    }
     R ys = -R(0.5) * yc * dm - R(0.5) * dYm - R(0.5) * md0 * dY;
     const R g1 = (zdif ? R(0) : drD) * dY;
     const R g2 = rdc * (zdif ? R(0) : d2Y);
