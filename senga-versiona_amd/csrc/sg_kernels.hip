@@ -1149,12 +1149,12 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
   // Bounded direction: waves holding any closure row evaluate all their
   // stencils in the uniform 5-term row form (weights from an LDS table,
   // offsets packed per lane); other waves take the interior stencil.
-  __shared__ double wtab[BND ? SG_NROW : 1][10];
+  __shared__ R wtab[BND ? SG_NROW : 1][10];   // in the working precision
   int pk[5] = {0, 0, 0, 0, 0};
   int row = 0;
   bool edge = false;
   if (BND) {
-    if (t < SG_NROW * 10) (&wtab[0][0])[t] = (&F.tab->roww[0][0])[t];
+    if (t < SG_NROW * 10) (&wtab[0][0])[t] = (R)(&F.tab->roww[0][0])[t];
     row = row_of_mode(S.mode);
 #pragma unroll
     for (int q = 0; q < 5; q++) pk[q] = c_row_offsets.w[row][q];
@@ -1605,10 +1605,10 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
   const double lhs = irk ? M.rkl[irk - 1] : 0.0, rhsc = irk ? M.rkr[irk - 1] : 0.0,
                erc = irk ? M.rke[irk - 1] : 0.0;
   // bounded x: mixed-derivative row weights in LDS (uniform row form)
-  __shared__ double mtab[BND ? SG_NROW * 7 : 1];
+  __shared__ R mtab[BND ? SG_NROW * 7 : 1];   // in the working precision
   const int tl = threadIdx.x;
   if (BND) {
-    if (tl < SG_NROW * 7) mtab[tl] = (&F.tab->mixw[0][0])[tl];
+    if (tl < SG_NROW * 7) mtab[tl] = (R)(&F.tab->mixw[0][0])[tl];
     __syncthreads();
   }
   // Point order: x fastest, then z, then y within chunks of P.ychunk rows,

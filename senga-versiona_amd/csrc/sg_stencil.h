@@ -49,40 +49,40 @@ template <bool BND, class R = double, class Fn>
 __device__ __forceinline__ R d1(const Params &P, const Sten &S, int dir, Fn f) {
   R r;
   if (!BND || S.mode == 0) {
-    double fa = FQ(1) - FQ(-1), fb = FQ(2) - FQ(-2), fc = FQ(3) - FQ(-3), fd = FQ(4) - FQ(-4),
+    R fa = FQ(1) - FQ(-1), fb = FQ(2) - FQ(-2), fc = FQ(3) - FQ(-3), fd = FQ(4) - FQ(-4),
            fe = FQ(5) - FQ(-5);
     r = (R)P.ac[0] * fa + (R)P.ac[1] * fb + (R)P.ac[2] * fc + (R)P.ac[3] * fd + (R)P.ac[4] * fe;
   } else {
     switch (S.mode) {
       case 1: {
-        double fa = FQ(1) - FQ(0), fb = FQ(2) - FQ(0), fc = FQ(3) - FQ(0), fd = FQ(4) - FQ(0);
+        R fa = FQ(1) - FQ(0), fb = FQ(2) - FQ(0), fc = FQ(3) - FQ(0), fd = FQ(4) - FQ(0);
         r = (R)P.c1[0] * fa + (R)P.c1[1] * fb + (R)P.c1[2] * fc + (R)P.c1[3] * fd;
       } break;
       case 2: {
-        double fa = FQ(-1) - FQ(0), fb = FQ(1) - FQ(0), fc = FQ(2) - FQ(0), fd = FQ(3) - FQ(0);
+        R fa = FQ(-1) - FQ(0), fb = FQ(1) - FQ(0), fc = FQ(2) - FQ(0), fd = FQ(3) - FQ(0);
         r = (R)P.c2[0] * fa + (R)P.c2[1] * fb + (R)P.c2[2] * fc + (R)P.c2[3] * fd;
       } break;
       case 3:
       case 13: {
-        double fa = FQ(1) - FQ(-1), fb = FQ(2) - FQ(-2);
+        R fa = FQ(1) - FQ(-1), fb = FQ(2) - FQ(-2);
         r = (R)P.c3[0] * fa + (R)P.c3[1] * fb;
       } break;
       case 4:
       case 14: {
-        double fa = FQ(1) - FQ(-1), fb = FQ(2) - FQ(-2), fc = FQ(3) - FQ(-3);
+        R fa = FQ(1) - FQ(-1), fb = FQ(2) - FQ(-2), fc = FQ(3) - FQ(-3);
         r = (R)P.c4[0] * fa + (R)P.c4[1] * fb + (R)P.c4[2] * fc;
       } break;
       case 5:
       case 15: {
-        double fa = FQ(1) - FQ(-1), fb = FQ(2) - FQ(-2), fc = FQ(3) - FQ(-3), fd = FQ(4) - FQ(-4);
+        R fa = FQ(1) - FQ(-1), fb = FQ(2) - FQ(-2), fc = FQ(3) - FQ(-3), fd = FQ(4) - FQ(-4);
         r = (R)P.c5[0] * fa + (R)P.c5[1] * fb + (R)P.c5[2] * fc + (R)P.c5[3] * fd;
       } break;
       case 12: {
-        double fa = FQ(0) - FQ(1), fb = FQ(0) - FQ(-1), fc = FQ(0) - FQ(-2), fd = FQ(0) - FQ(-3);
+        R fa = FQ(0) - FQ(1), fb = FQ(0) - FQ(-1), fc = FQ(0) - FQ(-2), fd = FQ(0) - FQ(-3);
         r = (R)P.c2[0] * fa + (R)P.c2[1] * fb + (R)P.c2[2] * fc + (R)P.c2[3] * fd;
       } break;
       default: {  // 11: last point
-        double fa = FQ(0) - FQ(-1), fb = FQ(0) - FQ(-2), fc = FQ(0) - FQ(-3), fd = FQ(0) - FQ(-4);
+        R fa = FQ(0) - FQ(-1), fb = FQ(0) - FQ(-2), fc = FQ(0) - FQ(-3), fd = FQ(0) - FQ(-4);
         r = (R)P.c1[0] * fa + (R)P.c1[1] * fb + (R)P.c1[2] * fc + (R)P.c1[3] * fd;
       } break;
     }
@@ -94,45 +94,45 @@ template <bool BND, class R = double, class Fn>
 __device__ __forceinline__ R d2(const Params &P, const Sten &S, int dir, Fn f) {
   R r;
   if (!BND || S.mode == 0) {
-    double c = FQ(0);
-    double ap = FQ(1) - c, am = c - FQ(-1), bp = FQ(2) - c, bm = c - FQ(-2), cp = FQ(3) - c,
+    R c = FQ(0);
+    R ap = FQ(1) - c, am = c - FQ(-1), bp = FQ(2) - c, bm = c - FQ(-2), cp = FQ(3) - c,
            cm = c - FQ(-3), dp = FQ(4) - c, dm = c - FQ(-4), ep = FQ(5) - c, em = c - FQ(-5);
     r = (R)P.sc[0] * (ap - am) + (R)P.sc[1] * (bp - bm) + (R)P.sc[2] * (cp - cm) + (R)P.sc[3] * (dp - dm) +
         (R)P.sc[4] * (ep - em);
   } else {
-    double c = FQ(0);
+    R c = FQ(0);
     switch (S.mode) {
       case 1: {
-        double ap = FQ(1) - c, bp = FQ(2) - c, cp = FQ(3) - c, dp = FQ(4) - c, ep = FQ(5) - c;
+        R ap = FQ(1) - c, bp = FQ(2) - c, cp = FQ(3) - c, dp = FQ(4) - c, ep = FQ(5) - c;
         r = (R)P.s1[0] * ap + (R)P.s1[1] * bp + (R)P.s1[2] * cp + (R)P.s1[3] * dp + (R)P.s1[4] * ep;
       } break;
       case 2: {
-        double ap = FQ(-1) - c, bp = FQ(1) - c, cp = FQ(2) - c, dp = FQ(3) - c, ep = FQ(4) - c;
+        R ap = FQ(-1) - c, bp = FQ(1) - c, cp = FQ(2) - c, dp = FQ(3) - c, ep = FQ(4) - c;
         r = (R)P.s2[0] * ap + (R)P.s2[1] * bp + (R)P.s2[2] * cp + (R)P.s2[3] * dp + (R)P.s2[4] * ep;
       } break;
       case 3:
       case 13: {
-        double ap = FQ(1) - c, am = c - FQ(-1), bp = FQ(2) - c, bm = c - FQ(-2);
+        R ap = FQ(1) - c, am = c - FQ(-1), bp = FQ(2) - c, bm = c - FQ(-2);
         r = (R)P.s3[0] * (ap - am) + (R)P.s3[1] * (bp - bm);
       } break;
       case 4:
       case 14: {
-        double ap = FQ(1) - c, am = c - FQ(-1), bp = FQ(2) - c, bm = c - FQ(-2), cp = FQ(3) - c,
+        R ap = FQ(1) - c, am = c - FQ(-1), bp = FQ(2) - c, bm = c - FQ(-2), cp = FQ(3) - c,
                cm = c - FQ(-3);
         r = (R)P.s4[0] * (ap - am) + (R)P.s4[1] * (bp - bm) + (R)P.s4[2] * (cp - cm);
       } break;
       case 5:
       case 15: {
-        double ap = FQ(1) - c, am = c - FQ(-1), bp = FQ(2) - c, bm = c - FQ(-2), cp = FQ(3) - c,
+        R ap = FQ(1) - c, am = c - FQ(-1), bp = FQ(2) - c, bm = c - FQ(-2), cp = FQ(3) - c,
                cm = c - FQ(-3), dp = FQ(4) - c, dm = c - FQ(-4);
         r = (R)P.s5[0] * (ap - am) + (R)P.s5[1] * (bp - bm) + (R)P.s5[2] * (cp - cm) + (R)P.s5[3] * (dp - dm);
       } break;
       case 12: {
-        double ap = FQ(1) - c, bp = FQ(-1) - c, cp = FQ(-2) - c, dp = FQ(-3) - c, ep = FQ(-4) - c;
+        R ap = FQ(1) - c, bp = FQ(-1) - c, cp = FQ(-2) - c, dp = FQ(-3) - c, ep = FQ(-4) - c;
         r = (R)P.s2[0] * ap + (R)P.s2[1] * bp + (R)P.s2[2] * cp + (R)P.s2[3] * dp + (R)P.s2[4] * ep;
       } break;
       default: {
-        double ap = FQ(-1) - c, bp = FQ(-2) - c, cp = FQ(-3) - c, dp = FQ(-4) - c, ep = FQ(-5) - c;
+        R ap = FQ(-1) - c, bp = FQ(-2) - c, cp = FQ(-3) - c, dp = FQ(-4) - c, ep = FQ(-5) - c;
         r = (R)P.s1[0] * ap + (R)P.s1[1] * bp + (R)P.s1[2] * cp + (R)P.s1[3] * dp + (R)P.s1[4] * ep;
       } break;
     }
@@ -309,39 +309,39 @@ template <bool BND, class R = double, class Gn>
 __device__ __forceinline__ R dmix(const Params &P, const Sten &S1, int d1, int d2, Gn g) {
   R r;
   if (!BND || S1.mode == 0) {
-    double fa = DIAG(1, 1), fb = DIAG(2, 2), fc = DIAG(3, 3), fd = DIAG(4, 4), fe = DIAG(5, 5);
+    R fa = DIAG(1, 1), fb = DIAG(2, 2), fc = DIAG(3, 3), fd = DIAG(4, 4), fe = DIAG(5, 5);
     r = (R)P.mc[0] * fa + (R)P.mc[1] * fb + (R)P.mc[2] * fc + (R)P.mc[3] * fd + (R)P.mc[4] * fe;
   } else {
     switch (S1.mode) {
       case 1: {
-        double fa = TR(1, 0), fb = TR(2, 0), fc = TR(3, 0), fd = TR(4, 0);
+        R fa = TR(1, 0), fb = TR(2, 0), fc = TR(3, 0), fd = TR(4, 0);
         r = (R)P.m1[0] * fa + (R)P.m1[1] * fb + (R)P.m1[2] * fc + (R)P.m1[3] * fd;
       } break;
       case 2: {
-        double fa = TR(-1, 0), fb = TR(1, 0), fc = TR(2, 0), fd = TR(3, 0);
+        R fa = TR(-1, 0), fb = TR(1, 0), fc = TR(2, 0), fd = TR(3, 0);
         r = (R)P.m2[0] * fa + (R)P.m2[1] * fb + (R)P.m2[2] * fc + (R)P.m2[3] * fd;
       } break;
       case 3:
       case 13: {
-        double fa = DIAG(1, 1), fb = DIAG(2, 2);
+        R fa = DIAG(1, 1), fb = DIAG(2, 2);
         r = (R)P.m3[0] * fa + (R)P.m3[1] * fb;
       } break;
       case 4:
       case 14: {
-        double fa = DIAG(1, 1), fb = DIAG(2, 2), fc = DIAG(3, 3);
+        R fa = DIAG(1, 1), fb = DIAG(2, 2), fc = DIAG(3, 3);
         r = (R)P.m4[0] * fa + (R)P.m4[1] * fb + (R)P.m4[2] * fc;
       } break;
       case 5:
       case 15: {
-        double fa = DIAG(1, 1), fb = DIAG(2, 2), fc = DIAG(3, 3), fd = DIAG(4, 4);
+        R fa = DIAG(1, 1), fb = DIAG(2, 2), fc = DIAG(3, 3), fd = DIAG(4, 4);
         r = (R)P.m5[0] * fa + (R)P.m5[1] * fb + (R)P.m5[2] * fc + (R)P.m5[3] * fd;
       } break;
       case 12: {
-        double fa = TR(0, 1), fb = TR(0, -1), fc = TR(0, -2), fd = TR(0, -3);
+        R fa = TR(0, 1), fb = TR(0, -1), fc = TR(0, -2), fd = TR(0, -3);
         r = (R)P.m2[0] * fa + (R)P.m2[1] * fb + (R)P.m2[2] * fc + (R)P.m2[3] * fd;
       } break;
       default: {
-        double fa = TR(0, -1), fb = TR(0, -2), fc = TR(0, -3), fd = TR(0, -4);
+        R fa = TR(0, -1), fb = TR(0, -2), fc = TR(0, -3), fd = TR(0, -4);
         r = (R)P.m1[0] * fa + (R)P.m1[1] * fb + (R)P.m1[2] * fc + (R)P.m1[3] * fd;
       } break;
     }
