@@ -18,9 +18,9 @@ using namespace sg;
 
 namespace {
 
-enum KernelId { K_TEMPIN = 0, K_BOUNDT, K_PRIM, K_PASSX, K_PASSY, K_PASSZ, K_FINAL, K_BOUNTT, K_DERIVS, K_COUNT };
+enum KernelId { K_TEMPIN = 0, K_BOUNDT, K_PRIM, K_PASSX, K_PASSY, K_PASSZ, K_FINAL, K_BOUNTT, K_DERIVS, K_NSCBC, K_COUNT };
 const char *kKernelNames[K_COUNT] = {"tempin", "boundt", "prim", "pass_x", "pass_y", "pass_z",
-                                     "final", "bountt", "derivs"};
+                                     "final", "bountt", "derivs", "nscbc"};
 
 }  // namespace
 
@@ -368,7 +368,7 @@ static int batch_prepare(sg_batch *b) {
   if (hipStreamSynchronize(b->stream) != hipSuccess) return SG_EDEVICE;
   const int nm = (int)b->members.size();
   std::vector<Member> hm(nm);
-  int tot[L_NKIND] = {0, 0, 0, 0, 0};
+  int tot[L_NKIND] = {};
   for (int m = 0; m < nm; m++) {
     hm[m] = b->members[m]->mem;
     for (int k = 0; k < L_NKIND; k++) {
@@ -432,6 +432,7 @@ static int batch_capture(sg_batch *b) {
     launch_pass(0, b->L, b->stream);
     launch_pass(1, b->L, b->stream);
     launch_pass(2, b->L, b->stream);
+    if (b->L.bndx) launch_nscbc(b->L, b->stream);
     launch_final(b->L, r, nullptr, b->stream);
   }
   if (b->L.inflow) launch_bountt(b->L, b->stream);
@@ -706,6 +707,7 @@ static int eval_rhs(sg_handle *h) {
   if (!rc) rc = timed(h, K_PASSX, [&] { launch_pass(0, L, h->stream); });
   if (!rc) rc = timed(h, K_PASSY, [&] { launch_pass(1, L, h->stream); });
   if (!rc) rc = timed(h, K_PASSZ, [&] { launch_pass(2, L, h->stream); });
+  if (!rc && L.bndx) rc = timed(h, K_NSCBC, [&] { launch_nscbc(L, h->stream); });
   return rc;
 }
 

@@ -90,19 +90,21 @@ struct Fields {
 enum PrimSlot { P_U = 0, P_V, P_W, P_E, P_T, P_P, P_LAM, P_MU, P_HMIX, P_NSCAL };
 // face slots: scalars then per-species
 enum FaceSlot { F_CP = 0, F_RR, F_DPDX, F_DPDY, F_DPDZ, F_DRDX, F_NSCAL };
-// per species: F_DYDX at F_NSCAL + s
+// per species: F_DYDX at F_NSCAL + s; then the NSCBC corrections of the
+// 5 + ns right-hand sides at F_NSCAL + ns + v (written by k_nscbc)
 
 inline __host__ __device__ long prim_y(const Params &P, int s) { return (long)(P_NSCAL + s) * P.fstride; }
 inline __host__ __device__ long prim_rd(const Params &P, int s) { return (long)(P_NSCAL + P.ns + s) * P.fstride; }
 inline __host__ __device__ long prim_h(const Params &P, int s) { return (long)(P_NSCAL + 2 * P.ns + s) * P.fstride; }
 inline __host__ __device__ int nprim(int ns) { return P_NSCAL + 3 * ns; }
-inline __host__ __device__ int nface(int ns) { return F_NSCAL + ns; }
+inline __host__ __device__ int nface(int ns) { return F_NSCAL + ns + 5 + ns; }
+inline __host__ __device__ int face_corr(int ns) { return F_NSCAL + ns; }
 
 // Substage launches cover any number of cases ("members") with one grid per
 // kernel kind.  The grid of a kind is the concatenation of every member's own
 // grid of that kind: flattened block b belongs to member map[kind][b] and is
 // that member's block b - blk0[kind].  A single case is a launch of one member.
-enum LaunchKind { L_FACE = 0, L_POINT, L_PASSX, L_PASSY, L_PASSZ, L_NKIND };
+enum LaunchKind { L_FACE = 0, L_FACE2, L_POINT, L_PASSX, L_PASSY, L_PASSZ, L_NKIND };   // L_FACE2: both x faces
 
 struct Member {
   Params P;
@@ -132,6 +134,8 @@ void launch_tempin(const Params &P, const sg_config *dcfg, const Fields &F, hipS
 void launch_boundt(const Launch &L, hipStream_t st);
 void launch_prim(const Launch &L, hipStream_t st);
 void launch_pass(int dir, const Launch &L, hipStream_t st);
+// NSCBC face corrections (bounded x only), before launch_final
+void launch_nscbc(const Launch &L, hipStream_t st);
 // substage irk (1..5): LINCOM for irk < 5, FINCOM for irk = 5, with each
 // member's coefficients; mode 2 (irk = 0): write the RHS to out instead
 void launch_final(const Launch &L, int irk, double *out, hipStream_t st);

@@ -1589,6 +1589,114 @@ __global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves
 }
 
 // ---------------------------------------------------------------------------
+// NSCBC face corrections (src/bounds.f:115-206 preparation, NSBCI3 at x-left
+// 610-682, NSBCO1 at x-right 1296-1458), one thread per face point.  They are
+// additive in the right-hand sides and depend only on fields the passes have
+// written, so they are formed here for the 2 ny nz face points and the final
+// kernel adds them.  Inside the final kernel they made every wave (each
+// holds a face lane when a row is one or two waves long) run the FP64 face
+// code, and raised its register count.
+#define NSCBC_BLK 64   // one wave per block: the face points spread over every CU
+template <int NS, class R>
+__global__ void __launch_bounds__(NSCBC_BLK) k_nscbc(const Member *__restrict__ mem, const uint32_t *__restrict__ map,
+                                                     const sg_config *__restrict__ c) {
+  int lb;
+  const Member &M = member_at<L_FACE2>(mem, map, blockIdx.x, lb);
+  const Params P = M.P;
+  const Fields F = global_fields(M.F);
+  const long nf = (long)P.ny * P.nz;
+  const long t = (long)lb * NSCBC_BLK + threadIdx.x;
+  if (t >= 2 * nf) return;
+  const int side = t >= nf ? 1 : 0;
+  const long fq = t - side * nf;
+  if (side == 0 ? !P.inflow_xl : !P.outflow_xr) return;
+  const int k = (int)(fq / P.ny), j = (int)(fq - (long)k * P.ny);
+  const int i = side ? P.nx - 1 : 0;
+  const long idx = ((long)k * P.ny + j) * P.nx + i;
+  const long fs = P.fstride;
+  const R *pr = (const R *)F.prim;
+  const R u = pr[P_U * fs + idx], v = pr[P_V * fs + idx], w = pr[P_W * fs + idx];
+  const R *G = (const R *)F.grad;
+  auto gr = [&](int q) { return G[q * fs + idx]; };
+  const R dux = gr(0), dvy = gr(5), dwz = gr(10);
+  R rD = R(0), rU = R(0), rV = R(0), rW = R(0), rE = R(0);
+  R rY[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) rY[s] = R(0);
+  constexpr bool BND = true;
+  {
+    const long nf = (long)P.ny * P.nz, fq = (long)k * P.ny + j;
+    const double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
+    const double strg = fc[F_CP * nf + fq], strr = fc[F_RR * nf + fq];
+    const double strt = pr[P_T * fs + idx];
+    const double strd = F.q[idx];
+    double gam1 = strg - strr;
+    const double stre = pr[P_E * fs + idx] - gam1 * strt;
+    gam1 = strr / gam1;
+    const double ovgm = 1.0 / gam1;
+    double fornow = strg * gam1 * strt;
+    const double acou = sqrt(fornow), ova2 = 1.0 / fornow;
+    double strh[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) strh[s] = pr[prim_h(P, s) + idx] - strg * strt * c->rgspec[s] / strr;
+    if (i == 0 && P.inflow_xl) {
+      // NSBCI3, src/bounds.f:610-682 (imposed values have zero time derivative)
+      // drho/dx at the face (src/rhscal.f:3966-3990), one-sided closure row
+      const Sten Sx = make_sten<BND>(i, P.nx, 1);
+      const double drdx = d1<BND>(P, Sx, 0, [&](int kk) { return F.q[idx + Sx.o[kk]]; });
+      double bcl1 = dux, bcl2 = drdx, bcl3 = gr(1), bcl4 = gr(2), bcl5 = fc[F_DPDX * nf + fq];
+      fornow = strd * acou * bcl1;
+      bcl1 = 0.5 * (u - acou) * (bcl5 - fornow);
+      bcl2 = u * (bcl2 - bcl5 * ova2);
+      bcl3 = u * bcl3;
+      bcl4 = u * bcl4;
+      bcl5 = 0.5 * (u + acou) * (bcl5 + fornow);
+      fornow = bcl1 - strd * acou * 0.0;
+      bcl2 = -0.0 - ova2 * (bcl1 + fornow) - bcl2;
+      bcl3 = -0.0 - bcl3;
+      bcl4 = -0.0 - bcl4;
+      bcl5 = fornow - bcl5;
+      rE = rE - bcl2 * stre - bcl3 * strd * v - bcl4 * strd * w - bcl5 * (ova2 * stre + u / acou + ovgm);
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        double bcly = ((const R *)F.rate)[s * fs + idx] / strd - 0.0 - u * fc[(F_NSCAL + s) * nf + fq];
+        rE = rE - bcly * strd * strh[s];
+      }
+    }
+    if (i == P.nx - 1 && P.outflow_xr) {
+      // NSBCO1, src/bounds.f:1296-1458 (pointwise in/outflow switch and local-Mach beta off)
+      const double bet = 0.050;
+      const double strp = (double)pr[P_P * fs + idx] + P.pofs;
+      const double t51 = -v * fc[F_DPDY * nf + fq] - w * fc[F_DPDZ * nf + fq];
+      const double t52 = -strp * (dvy + dwz);
+      const double t2b = -v * gr(4) - w * gr(8);
+      const double tt1 = t51 + t52 * (gam1 + 1.0) - strd * acou * t2b;
+      double sorp = 0.0;
+#pragma unroll
+      for (int s = 0; s < NS; s++) sorp = sorp + strh[s] * ((const R *)F.rate)[s * fs + idx];
+      sorp = -sorp * gam1;
+      double bcl1 = 0.5 * (u - acou) * (fc[F_DPDX * nf + fq] - strd * acou * dux);
+      bcl1 = 0.5 * sorp + P.cobcxr * acou * (strp - P.pinfxr) + 0.5 * (1.0 - bet) * tt1 - bcl1;
+      rD = rD - bcl1 * ova2;
+      rU = rU - bcl1 * ova2 * (u - acou);
+      rV = rV - bcl1 * ova2 * v;
+      rW = rW - bcl1 * ova2 * w;
+      rE = rE - bcl1 * (ova2 * stre - u / acou + ovgm);
+#pragma unroll
+      for (int s = 0; s < NS; s++) rY[s] = rY[s] - bcl1 * ova2 * pr[prim_y(P, s) + idx];
+    }
+  }
+  double *cr = F.face + (long)side * nface(P.ns) * nf + (long)face_corr(P.ns) * nf + fq;
+  cr[0] = rD;
+  cr[nf] = rU;
+  cr[2 * nf] = rV;
+  cr[3 * nf] = rW;
+  cr[4 * nf] = rE;
+#pragma unroll
+  for (int s = 0; s < NS; s++) cr[(5 + s) * nf] = rY[s];
+}
+
+// ---------------------------------------------------------------------------
 // Final kernel: cross-direction viscous terms (src/rhsvel.f:992-2185), NSCBC
 // face corrections (src/bounds.f:115-206, 610-682, 1196-1458) and the RK
 // register update (src/lincom.f:53-169, src/fincom.f:54-168).
@@ -1752,68 +1860,17 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
   R rY[NS];
 #pragma unroll
   for (int s = 0; s < NS; s++) rY[s] = fld(A + (5 + s) * fs, bo);
-  // ---------------- NSCBC faces
+  // ---------------- NSCBC faces: corrections formed by k_nscbc
   if (BND && (i == 0 || i == P.nx - 1)) {
     const long nf = (long)P.ny * P.nz, fq = (long)k * P.ny + j;
-    const double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
-    const double strg = fc[F_CP * nf + fq], strr = fc[F_RR * nf + fq];
-    const double strt = pr[P_T * fs + idx];
-    const double strd = F.q[idx];
-    double gam1 = strg - strr;
-    const double stre = pr[P_E * fs + idx] - gam1 * strt;
-    gam1 = strr / gam1;
-    const double ovgm = 1.0 / gam1;
-    double fornow = strg * gam1 * strt;
-    const double acou = sqrt(fornow), ova2 = 1.0 / fornow;
-    double strh[NS];
+    const double *cr = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf) + (long)face_corr(P.ns) * nf + fq;
+    rD = rD + (R)cr[0];
+    rU = rU + (R)cr[nf];
+    rV = rV + (R)cr[2 * nf];
+    rW = rW + (R)cr[3 * nf];
+    rE = rE + (R)cr[4 * nf];
 #pragma unroll
-    for (int s = 0; s < NS; s++) strh[s] = pr[prim_h(P, s) + idx] - strg * strt * c->rgspec[s] / strr;
-    if (i == 0 && P.inflow_xl) {
-      // NSBCI3, src/bounds.f:610-682 (imposed values have zero time derivative)
-      // drho/dx at the face (src/rhscal.f:3966-3990), one-sided closure row
-      const Sten Sx = make_sten<BND>(i, P.nx, 1);
-      const double drdx = d1<BND>(P, Sx, 0, [&](int kk) { return F.q[idx + Sx.o[kk]]; });
-      double bcl1 = dux, bcl2 = drdx, bcl3 = gr(1), bcl4 = gr(2), bcl5 = fc[F_DPDX * nf + fq];
-      fornow = strd * acou * bcl1;
-      bcl1 = 0.5 * (u - acou) * (bcl5 - fornow);
-      bcl2 = u * (bcl2 - bcl5 * ova2);
-      bcl3 = u * bcl3;
-      bcl4 = u * bcl4;
-      bcl5 = 0.5 * (u + acou) * (bcl5 + fornow);
-      fornow = bcl1 - strd * acou * 0.0;
-      bcl2 = -0.0 - ova2 * (bcl1 + fornow) - bcl2;
-      bcl3 = -0.0 - bcl3;
-      bcl4 = -0.0 - bcl4;
-      bcl5 = fornow - bcl5;
-      rE = rE - bcl2 * stre - bcl3 * strd * v - bcl4 * strd * w - bcl5 * (ova2 * stre + u / acou + ovgm);
-#pragma unroll
-      for (int s = 0; s < NS; s++) {
-        double bcly = ((const R *)F.rate)[s * fs + idx] / strd - 0.0 - u * fc[(F_NSCAL + s) * nf + fq];
-        rE = rE - bcly * strd * strh[s];
-      }
-    }
-    if (i == P.nx - 1 && P.outflow_xr) {
-      // NSBCO1, src/bounds.f:1296-1458 (pointwise in/outflow switch and local-Mach beta off)
-      const double bet = 0.050;
-      const double strp = (double)pr[P_P * fs + idx] + P.pofs;
-      const double t51 = -v * fc[F_DPDY * nf + fq] - w * fc[F_DPDZ * nf + fq];
-      const double t52 = -strp * (dvy + dwz);
-      const double t2b = -v * gr(4) - w * gr(8);
-      const double tt1 = t51 + t52 * (gam1 + 1.0) - strd * acou * t2b;
-      double sorp = 0.0;
-#pragma unroll
-      for (int s = 0; s < NS; s++) sorp = sorp + strh[s] * ((const R *)F.rate)[s * fs + idx];
-      sorp = -sorp * gam1;
-      double bcl1 = 0.5 * (u - acou) * (fc[F_DPDX * nf + fq] - strd * acou * dux);
-      bcl1 = 0.5 * sorp + P.cobcxr * acou * (strp - P.pinfxr) + 0.5 * (1.0 - bet) * tt1 - bcl1;
-      rD = rD - bcl1 * ova2;
-      rU = rU - bcl1 * ova2 * (u - acou);
-      rV = rV - bcl1 * ova2 * v;
-      rW = rW - bcl1 * ova2 * w;
-      rE = rE - bcl1 * (ova2 * stre - u / acou + ovgm);
-#pragma unroll
-      for (int s = 0; s < NS; s++) rY[s] = rY[s] - bcl1 * ova2 * pr[prim_y(P, s) + idx];
-    }
+    for (int s = 0; s < NS; s++) rY[s] = rY[s] + (R)cr[(5 + s) * nf];
   }
   // ---------------- register update
   R Rv[5 + NS];
@@ -1913,6 +1970,12 @@ void launch_pass_t(int dir, const Launch &L, hipStream_t st) {
   else launch_pass_r<NS, double>(dir, L, st);
 }
 template <int NS>
+void launch_nscbc_t(const Launch &L, hipStream_t st) {
+  const dim3 g(L.nblk[L_FACE2]), b(NSCBC_BLK);
+  if (L.precision) hipLaunchKernelGGL((k_nscbc<NS, float>), g, b, 0, st, L.mem, L.map[L_FACE2], L.c);
+  else hipLaunchKernelGGL((k_nscbc<NS, double>), g, b, 0, st, L.mem, L.map[L_FACE2], L.c);
+}
+template <int NS>
 void launch_final_t(const Launch &L, int irk, double *out, hipStream_t st) {
   const dim3 g(L.nblk[L_POINT]), b(BLK);
   const uint32_t *map = L.map[L_POINT];
@@ -1931,6 +1994,7 @@ template void launch_boundt_t<SG_NS>(const Launch &, hipStream_t);
 template void launch_bountt_t<SG_NS>(const Launch &, hipStream_t);
 template void launch_prim_t<SG_NS>(const Launch &, hipStream_t);
 template void launch_pass_t<SG_NS>(int, const Launch &, hipStream_t);
+template void launch_nscbc_t<SG_NS>(const Launch &, hipStream_t);
 template void launch_final_t<SG_NS>(const Launch &, int, double *, hipStream_t);
 #endif
 
@@ -1971,6 +2035,7 @@ __global__ void __launch_bounds__(BLK) k_zero(double *p, long n) {
 
 void member_blocks(const Params &P, int nblk[L_NKIND]) {
   nblk[L_FACE] = (int)grid_for((long)P.ny * P.nz);
+  nblk[L_FACE2] = (int)((2L * P.ny * P.nz + NSCBC_BLK - 1) / NSCBC_BLK);
   nblk[L_POINT] = (int)grid_for(P.npts);
   auto tiles = [&](int tx, int ty, int tz) {
     return ((P.nx + tx - 1) / tx) * ((P.ny + ty - 1) / ty) * ((P.nz + tz - 1) / tz);
