@@ -95,3 +95,67 @@ def test_derived_tables_match_reference_recipe():
             cp = sum(t.amascp[m, it, s] * T ** m for m in range(5))
             dh = (h(T + 1e-3) - h(T - 1e-3)) / 2e-3
             assert abs(dh - cp) < 1e-6 * cp
+
+
+def test_footprint_without_device(lib):
+    """sg_footprint sizes a case on the host: registers (2(5+N) FP64 fields)
+    plus working fields (FP64, or FP32 in the mixed form), and rejects what
+    sg_create rejects."""
+    c = CS.periodic_1step(n=(16, 12, 11))
+    ns, npts = c.nspec, 16 * 12 * 11
+    b64 = abi.footprint(c, 0, lib)
+    b32 = abi.footprint(c, 1, lib)
+    work = (9 + 3 * ns) + ns + (5 + ns) + 12
+    assert b64 >= npts * (8 * 2 * (5 + ns) + 8 * work)
+    assert b32 >= npts * (8 * 2 * (5 + ns) + 4 * work)
+    assert b64 - b32 >= npts * 4 * work
+    big = CS.periodic_1step(n=(32, 24, 22))        # 8x the points: the per-point part grows 8x
+    assert abi.footprint(big, 0, lib) - b64 >= 7 * npts * (8 * 2 * (5 + ns) + 8 * work)
+    cfg = abi.config_from_case(c)
+    out = C.c_size_t(0)
+    bad = abi.Config.from_buffer_copy(cfg)
+    bad.nx = 8
+    assert lib.sg_footprint(C.byref(bad), C.byref(out)) == 1
+    bad = abi.Config.from_buffer_copy(cfg)
+    bad.nspec = 1
+    assert lib.sg_footprint(C.byref(bad), C.byref(out)) == 1
+
+
+def test_batch_entry_points_reject_null(lib):
+    h = C.c_void_p()
+    n = C.c_int(0)
+    assert lib.sg_batch_create(None) == 1
+    assert lib.sg_batch_destroy(None) == 1
+    assert lib.sg_batch_add(None, None) == 1
+    assert lib.sg_batch_remove(None, None) == 1
+    assert lib.sg_batch_size(None, C.byref(n)) == 1
+    assert lib.sg_batch_member(None, 0, C.byref(h)) == 1
+    assert lib.sg_batch_step(None, 1) == 1
+    assert lib.sg_batch_sync(None, None) == 1
+
+
+def test_case_sets_follow_their_distributions():
+    b = CS.batch_specs(200, seed=3)
+    assert {s["n"][0] for s in b} == {48, 64, 96, 128}
+    assert all(0.5 <= s["phi"] <= 1.2 and 1.0 <= s["uprime_over_sl"] <= 10.0 and 300.0 <= s["Tu"] <= 700.0
+               for s in b)
+    assert CS.batch_specs(5, seed=3) == b[:5]          # seeded
+    k = CS.kernel_specs(400, seed=5, rate=4.0)
+    t = np.array([s["arrival"] for s in k])
+    assert np.all(np.diff(t) > 0)
+    assert abs(len(k) / t[-1] - 4.0) < 0.5              # Poisson rate
+    assert all(1500.0 <= s["t_peak"] <= 2000.0 and 0.5e-3 <= s["radius"] <= 2e-3 for s in k)
+    assert all(48 <= s["n"][0] <= 160 for s in k)
+
+
+def test_serving_summary():
+    from sengapy.serve import Result, summary
+    rs = [Result(q, arrival=0.5 * q, admitted=0.5 * q, finished=0.5 * q + 1.0 + 0.1 * q, nsteps=10,
+                 points=1000) for q in range(10)]
+    s = summary(rs)
+    lat = np.array([1.0 + 0.1 * q for q in range(10)])
+    assert s["cases"] == 10
+    assert abs(s["makespan_s"] - (4.5 + 1.9)) < 1e-12
+    assert abs(s["p50_s"] - np.percentile(lat, 50)) < 1e-12
+    assert abs(s["p99_s"] - np.percentile(lat, 99)) < 1e-12
+    assert abs(s["mpoint_substages_per_s"] - 10 * 1000 * 50 / 6.4 / 1e6) < 1e-9
