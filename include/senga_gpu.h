@@ -222,6 +222,20 @@ int sg_profile_read(sg_handle *h, int idx, char *name, size_t n, double *total_m
 /* Device bytes held by the handle. */
 int sg_memory(sg_handle *h, size_t *bytes);
 
+/* ---- Tiles and their autotuner (no counterpart in the reference) ---------
+ * narrow[d] selects the tile of the direction-d derivative pass: 0 = rows of
+ * 32 points (default), 1 = rows of 16 points and twice the rows (no idle
+ * lanes when n_x is an odd multiple of 16, a shorter halo along y and z).
+ * ychunk (> 0, dividing ny; 0 keeps the current one) is the number of y rows
+ * the final kernel sweeps together.  Neither changes any result.
+ * sg_autotune times each choice on the handle's current state (reps launches
+ * each, 5 if reps <= 0), keeps the fastest, and returns in *ms the tuned
+ * passes' and final kernel's time per substage; the state it leaves is the
+ * one it found, so tuned and untuned runs compute identical results. */
+int sg_set_tiles(sg_handle *h, const int narrow[3], int ychunk);
+int sg_get_tiles(sg_handle *h, int narrow[3], int *ychunk);
+int sg_autotune(sg_handle *h, int reps, double *ms);
+
 /* Device bytes sg_create would allocate for cfg (without allocating), for
  * admission by memory budget; SG_EINVAL for a configuration sg_create
  * rejects.  No device access. */
@@ -261,6 +275,11 @@ int sg_batch_step(sg_batch *b, int nsteps);
  * the failing member's index in *failed (else -1), whose sg_last_error holds
  * the point. */
 int sg_batch_sync(sg_batch *b, int *failed);
+/* Pass tiles of the batch's launches (one choice for all members), set or
+ * tuned on the members' current states as sg_autotune does for a handle. */
+int sg_batch_set_tiles(sg_batch *b, const int narrow[3]);
+int sg_batch_get_tiles(sg_batch *b, int narrow[3]);
+int sg_batch_autotune(sg_batch *b, int reps, double *ms);
 
 #ifdef __cplusplus
 }

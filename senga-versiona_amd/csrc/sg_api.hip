@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -273,10 +274,16 @@ static int setup_params(sg_handle *h) {
 
 // One-member launch set of a handle: its member record and all-zero block
 // maps (every block belongs to member 0, blk0 = 0).
-static int init_launch(sg_handle *h) {
+static int init_launch(sg_handle *h, const int nw[3]) {
+  if (h->lbase) {
+    hipStreamSynchronize(h->stream);
+    hipFree(h->lbase);
+    h->lbase = nullptr;
+  }
   Launch &L = h->L;
   memset(&L, 0, sizeof L);
-  member_blocks(h->P, L.nblk);
+  for (int d = 0; d < 3; d++) L.pass_nw[d] = nw[d] ? 1 : 0;
+  member_blocks(h->P, L.pass_nw, L.nblk);
   size_t bytes = (sizeof(Member) + 255) / 256 * 256;
   for (int k = 0; k < L_NKIND; k++) bytes += ((size_t)L.nblk[k] * sizeof(uint32_t) + 255) / 256 * 256;
   if (hipMalloc(&h->lbase, bytes) != hipSuccess) {
@@ -369,11 +376,13 @@ static int batch_prepare(sg_batch *b) {
   const int nm = (int)b->members.size();
   std::vector<Member> hm(nm);
   int tot[L_NKIND] = {};
+  std::vector<std::array<int, L_NKIND>> nb(nm);
   for (int m = 0; m < nm; m++) {
     hm[m] = b->members[m]->mem;
+    member_blocks(b->members[m]->P, b->L.pass_nw, nb[m].data());
     for (int k = 0; k < L_NKIND; k++) {
       hm[m].blk0[k] = tot[k];
-      long t = (long)tot[k] + b->members[m]->L.nblk[k];
+      long t = (long)tot[k] + nb[m][k];
       if (t >= (1L << 31)) return SG_EINVAL;
       tot[k] = (int)t;
     }
@@ -396,7 +405,7 @@ static int batch_prepare(sg_batch *b) {
     for (int k = 0; k < L_NKIND; k++) {
       map.assign((size_t)tot[k], 0u);
       for (int m = 0; m < nm; m++)
-        for (int q = 0; q < b->members[m]->L.nblk[k]; q++) map[(size_t)hm[m].blk0[k] + q] = (uint32_t)m;
+        for (int q = 0; q < nb[m][k]; q++) map[(size_t)hm[m].blk0[k] + q] = (uint32_t)m;
       b->L.map[k] = (const uint32_t *)p;
       b->L.nblk[k] = tot[k];
       if (hipMemcpy(p, map.data(), map.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
@@ -491,10 +500,14 @@ static size_t case_bytes(const Params &P, size_t *main_block) {
   const size_t tabsz = ((sizeof(Tables) + 255) / 256) * 256;
   const size_t main = bytes + itn + ((facesz + 255) / 256) * 256 + cfgsz + tabsz + 256;
   if (main_block) *main_block = main;
-  int nblk[L_NKIND];
-  member_blocks(P, nblk);
+  // block maps sized for the larger of the wide and narrow pass tiles
+  int nblk[L_NKIND], nb1[L_NKIND];
+  const int wide[3] = {0, 0, 0}, narrow[3] = {1, 1, 1};
+  member_blocks(P, wide, nblk);
+  member_blocks(P, narrow, nb1);
   size_t lb = (sizeof(Member) + 255) / 256 * 256;
-  for (int k = 0; k < L_NKIND; k++) lb += ((size_t)nblk[k] * sizeof(uint32_t) + 255) / 256 * 256;
+  for (int k = 0; k < L_NKIND; k++)
+    lb += ((size_t)(nblk[k] > nb1[k] ? nblk[k] : nb1[k]) * sizeof(uint32_t) + 255) / 256 * 256;
   return main + lb;
 }
 
@@ -573,7 +586,8 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   hipEventCreate(&h->ev[0]);
   hipEventCreate(&h->ev[1]);
   h->etime = 0.0;
-  if (init_launch(h) != 0) {
+  const int wide[3] = {0, 0, 0};
+  if (init_launch(h, wide) != 0) {
     sg_destroy(h);
     return SG_ENOMEM;
   }
@@ -1035,6 +1049,197 @@ int sg_batch_member(sg_batch *b, int idx, sg_handle **h) {
   if (!b || !h || idx < 0 || idx >= (int)b->members.size()) return SG_EINVAL;
   *h = b->members[(size_t)idx];
   return 0;
+}
+
+}  // extern "C"
+
+// ---- Tile tuning ------------------------------------------------------------
+// A tuning run times launches on the live state.  It changes nothing a later
+// step reads except the stored temperature (the Newton initial guess) and the
+// interval indices, which prim rewrites; those are saved and put back, so a
+// tuned run computes exactly what an untuned one does.
+struct TempSnap {
+  void *buf = nullptr;
+  size_t tb = 0, ib = 0;
+};
+
+static int snap_save(sg_handle *h, TempSnap &sn) {
+  const Params &P = h->P;
+  const size_t esz = P.precision ? sizeof(float) : sizeof(double);
+  sn.tb = esz * (size_t)P.fstride;
+  sn.ib = sizeof(uint32_t) * (size_t)P.fstride;
+  HIPCHK(h, hipMalloc(&sn.buf, sn.tb + sn.ib));
+  const char *pt = (const char *)h->F.prim + esz * (size_t)P_T * P.fstride;
+  HIPCHK(h, hipMemcpyAsync(sn.buf, pt, sn.tb, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync((char *)sn.buf + sn.tb, h->F.itndex, sn.ib, hipMemcpyDeviceToDevice, h->stream));
+  return 0;
+}
+
+static int snap_restore(sg_handle *h, TempSnap &sn, hipStream_t st) {
+  const Params &P = h->P;
+  const size_t esz = P.precision ? sizeof(float) : sizeof(double);
+  char *pt = (char *)h->F.prim + esz * (size_t)P_T * P.fstride;
+  HIPCHK(h, hipStreamSynchronize(st));
+  HIPCHK(h, hipMemcpyAsync(pt, sn.buf, sn.tb, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->F.itndex, (char *)sn.buf + sn.tb, sn.ib, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipFree(sn.buf));
+  sn.buf = nullptr;
+  return 0;
+}
+
+// Mean duration in ms of reps launches made by fn on stream st (after one
+// untimed launch).
+template <class Fn>
+static double time_launches(hipStream_t st, int reps, Fn fn) {
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  fn();
+  hipEventRecord(e0, st);
+  for (int r = 0; r < reps; r++) fn();
+  hipEventRecord(e1, st);
+  hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return (double)ms / reps;
+}
+
+extern "C" {
+
+int sg_set_tiles(sg_handle *h, const int narrow[3], int ychunk) {
+  if (!h || !narrow) return SG_EINVAL;
+  if (h->batch) return set_err(h, SG_ESTATE, "handle advances in a batch");
+  if (ychunk < 0 || (ychunk > 0 && h->P.ny % ychunk != 0)) return set_err(h, SG_EINVAL, "ychunk must divide ny");
+  if (ychunk > 0) h->P.ychunk = ychunk;
+  if (init_launch(h, narrow) != 0) return set_err(h, SG_ENOMEM, "launch table allocation failed");
+  return refresh_member(h);
+}
+
+int sg_get_tiles(sg_handle *h, int narrow[3], int *ychunk) {
+  if (!h) return SG_EINVAL;
+  if (narrow)
+    for (int d = 0; d < 3; d++) narrow[d] = h->L.pass_nw[d];
+  if (ychunk) *ychunk = h->P.ychunk;
+  return 0;
+}
+
+int sg_autotune(sg_handle *h, int reps, double *ms) {
+  if (!h) return SG_EINVAL;
+  if (h->batch) return set_err(h, SG_ESTATE, "handle advances in a batch");
+  if (!h->have_state) return set_err(h, SG_ESTATE, "no state uploaded");
+  if (reps <= 0) reps = 5;
+  TempSnap sn;
+  int rc = snap_save(h, sn);
+  if (rc) return rc;
+  if (h->L.inflow) launch_boundt(h->L, h->stream);
+  launch_prim(h->L, h->stream);
+  int nw[3] = {h->L.pass_nw[0], h->L.pass_nw[1], h->L.pass_nw[2]};
+  double total = 0.0;
+  for (int d = 0; d < 3 && !rc; d++) {
+    double tb = 1e30;
+    int bw = nw[d];
+    for (int w = 0; w < 2 && !rc; w++) {
+      int cur[3] = {nw[0], nw[1], nw[2]};
+      cur[d] = w;
+      if (init_launch(h, cur) != 0) rc = set_err(h, SG_ENOMEM, "launch table allocation failed");
+      if (!rc) rc = refresh_member(h);
+      if (rc) break;
+      const double t = time_launches(h->stream, reps, [&] { launch_pass(d, h->L, h->stream); });
+      if (t < tb) { tb = t; bw = w; }
+    }
+    nw[d] = bw;
+    total += tb;
+  }
+  if (!rc && init_launch(h, nw) != 0) rc = set_err(h, SG_ENOMEM, "launch table allocation failed");
+  if (!rc) rc = refresh_member(h);
+  // point order of the final kernel: rows per y-chunk
+  double *out = nullptr;
+  if (!rc && hipMalloc((void **)&out, sizeof(double) * (size_t)h->P.fstride * (5 + h->P.ns)) != hipSuccess)
+    rc = set_err(h, SG_ENOMEM, "scratch allocation failed");
+  if (!rc) {
+    for (int d = 0; d < 3; d++) launch_pass(d, h->L, h->stream);
+    if (h->L.bndx) launch_nscbc(h->L, h->stream);
+    double tb = 1e30;
+    int bc = h->P.ychunk;
+    const int cand[7] = {1, 2, 4, 8, 16, 32, 64};
+    for (int q = 0; q < 7 && !rc; q++) {
+      if (h->P.ny % cand[q] != 0) continue;
+      h->P.ychunk = cand[q];
+      rc = refresh_member(h);
+      if (rc) break;
+      const double t = time_launches(h->stream, reps, [&] { launch_final(h->L, 0, out, h->stream); });
+      if (t < tb) { tb = t; bc = cand[q]; }
+    }
+    h->P.ychunk = bc;
+    total += tb;
+  }
+  if (out) {
+    hipStreamSynchronize(h->stream);
+    hipFree(out);
+  }
+  int rc2 = snap_restore(h, sn, h->stream);
+  if (!rc) rc = rc2;
+  if (!rc) rc = refresh_member(h);
+  if (!rc && ms) *ms = total;
+  return rc;
+}
+
+int sg_batch_set_tiles(sg_batch *b, const int narrow[3]) {
+  if (!b || !narrow) return SG_EINVAL;
+  for (int d = 0; d < 3; d++) b->L.pass_nw[d] = narrow[d] ? 1 : 0;
+  b->dirty_layout = 1;
+  return 0;
+}
+
+int sg_batch_get_tiles(sg_batch *b, int narrow[3]) {
+  if (!b || !narrow) return SG_EINVAL;
+  for (int d = 0; d < 3; d++) narrow[d] = b->L.pass_nw[d];
+  return 0;
+}
+
+int sg_batch_autotune(sg_batch *b, int reps, double *ms) {
+  if (!b) return SG_EINVAL;
+  if (b->members.empty()) return 0;
+  if (reps <= 0) reps = 5;
+  b->dirty_layout = 1;
+  int rc = batch_prepare(b);
+  if (rc) return rc;
+  std::vector<TempSnap> sn(b->members.size());
+  for (size_t m = 0; m < b->members.size() && !rc; m++) rc = snap_save(b->members[m], sn[m]);
+  for (sg_handle *h : b->members)
+    if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = SG_EDEVICE;
+  if (!rc) {
+    if (b->L.inflow) launch_boundt(b->L, b->stream);
+    launch_prim(b->L, b->stream);
+  }
+  double total = 0.0;
+  for (int d = 0; d < 3 && !rc; d++) {
+    double tb = 1e30;
+    int bw = b->L.pass_nw[d];
+    for (int w = 0; w < 2 && !rc; w++) {
+      b->L.pass_nw[d] = w;
+      b->dirty_layout = 1;
+      rc = batch_prepare(b);
+      if (rc) break;
+      const double t = time_launches(b->stream, reps, [&] { launch_pass(d, b->L, b->stream); });
+      if (t < tb) { tb = t; bw = w; }
+    }
+    b->L.pass_nw[d] = bw;
+    total += tb;
+  }
+  b->dirty_layout = 1;
+  int rc2 = batch_prepare(b);
+  if (!rc) rc = rc2;
+  for (size_t m = 0; m < b->members.size(); m++)
+    if (sn[m].buf) {
+      rc2 = snap_restore(b->members[m], sn[m], b->stream);
+      if (!rc) rc = rc2;
+    }
+  if (!rc && ms) *ms = total;
+  return rc;
 }
 
 }  // extern "C"

@@ -124,10 +124,11 @@ struct Launch {
   const sg_config *c;
   const Tables *tab;
   int ns, precision, bndx, inflow;
+  int pass_nw[3];   // pass tile per direction: 0 wide (rows of 32), 1 narrow (rows of 16)
 };
 
 // member block counts of each kind (sg_kernels.hip)
-void member_blocks(const Params &P, int nblk[L_NKIND]);
+void member_blocks(const Params &P, const int nw[3], int nblk[L_NKIND]);
 
 // launchers (sg_kernels.hip)
 void launch_tempin(const Params &P, const sg_config *dcfg, const Fields &F, hipStream_t st);

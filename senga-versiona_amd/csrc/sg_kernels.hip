@@ -1092,13 +1092,17 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
 // evaluates its stencils from LDS at compile-time offsets.  Fields staged:
 // rho u_d, e*rho u_d, e, T, lambda, u, v, w, the three momentum fluxes,
 // p - p_ref, mu, h_mix and per species Y, Y*rho u_d, rho D, h.
-template <int DIR>
+template <int DIR, int NW = 0>
 struct PassTile {
-  // x: 32x8 points (halo 42/32 along x).  y, z: 32x16 points, 512 threads,
-  // so that the 10-point halo along the pass direction costs 26/16 re-reads
-  static constexpr int TX = 32;
-  static constexpr int TY = DIR == 0 ? 8 : (DIR == 1 ? 16 : 1);
-  static constexpr int TZ = DIR == 2 ? 16 : 1;
+  // Wide tiles (NW = 0), x: 32x8 points (halo 42/32 along x); y, z: 32x16
+  // points, 512 threads, so that the 10-point halo along the pass direction
+  // costs 26/16 re-reads.  Narrow tiles (NW = 1) have rows of 16 points and
+  // twice the rows: no idle lanes when nx is an odd multiple of 16, and a
+  // cheaper y/z halo (42/32), at half-length coalesced rows.  The autotuner
+  // picks one per direction.
+  static constexpr int TX = NW ? 16 : 32;
+  static constexpr int TY = DIR == 0 ? (NW ? 16 : 8) : (DIR == 1 ? (NW ? 32 : 16) : 1);
+  static constexpr int TZ = DIR == 2 ? (NW ? 32 : 16) : 1;
   static constexpr int NT = TX * TY * TZ;   // threads per block
   static constexpr int LX = TX + (DIR == 0 ? 10 : 0);
   static constexpr int LY = TY + (DIR == 1 ? 10 : 0);
@@ -1107,10 +1111,10 @@ struct PassTile {
   static constexpr int LSTR = DIR == 0 ? 1 : (DIR == 1 ? LX : LX * LY);   // LDS stride along DIR
 };
 
-template <int DIR, bool BND, int NS, class R>
-__global__ void __launch_bounds__(PassTile<DIR>::NT) __attribute__((amdgpu_waves_per_eu(4))) k_pass(const Member *__restrict__ mem,
+template <int DIR, bool BND, int NS, class R, int NW>
+__global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu_waves_per_eu(4))) k_pass(const Member *__restrict__ mem,
                                                                                                    const uint32_t *__restrict__ map) {
-  using T_ = PassTile<DIR>;
+  using T_ = PassTile<DIR, NW>;
   constexpr int TX = T_::TX, TY = T_::TY, LX = T_::LX, LY = T_::LY, LN = T_::LN, LSTR = T_::LSTR;
   constexpr int NTH = T_::NT;
   // LN staged values per field, padded to whole block-sized rows so that
@@ -1951,18 +1955,25 @@ void launch_prim_t(const Launch &L, hipStream_t st) {
   if (L.precision) hipLaunchKernelGGL(k_prim_mx<NS>, g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
   else hipLaunchKernelGGL(k_prim<NS>, g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
 }
-template <int NS, class R>
-static void launch_pass_r(int dir, const Launch &L, hipStream_t st) {
+template <int NS, class R, int NW>
+static void launch_pass_w(int dir, const Launch &L, hipStream_t st) {
   const int k = L_PASSX + dir;
   if (dir == 0) {
-    const dim3 b(PassTile<0>::NT);
-    if (L.bndx) hipLaunchKernelGGL((k_pass<0, true, NS, R>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
-    else hipLaunchKernelGGL((k_pass<0, false, NS, R>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
+    const dim3 b(PassTile<0, NW>::NT);
+    if (L.bndx) hipLaunchKernelGGL((k_pass<0, true, NS, R, NW>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
+    else hipLaunchKernelGGL((k_pass<0, false, NS, R, NW>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
   } else if (dir == 1) {
-    hipLaunchKernelGGL((k_pass<1, false, NS, R>), dim3(L.nblk[k]), dim3(PassTile<1>::NT), 0, st, L.mem, L.map[k]);
+    hipLaunchKernelGGL((k_pass<1, false, NS, R, NW>), dim3(L.nblk[k]), dim3(PassTile<1, NW>::NT), 0, st, L.mem,
+                       L.map[k]);
   } else {
-    hipLaunchKernelGGL((k_pass<2, false, NS, R>), dim3(L.nblk[k]), dim3(PassTile<2>::NT), 0, st, L.mem, L.map[k]);
+    hipLaunchKernelGGL((k_pass<2, false, NS, R, NW>), dim3(L.nblk[k]), dim3(PassTile<2, NW>::NT), 0, st, L.mem,
+                       L.map[k]);
   }
+}
+template <int NS, class R>
+static void launch_pass_r(int dir, const Launch &L, hipStream_t st) {
+  if (L.pass_nw[dir]) launch_pass_w<NS, R, 1>(dir, L, st);
+  else launch_pass_w<NS, R, 0>(dir, L, st);
 }
 template <int NS>
 void launch_pass_t(int dir, const Launch &L, hipStream_t st) {
@@ -2033,16 +2044,19 @@ __global__ void __launch_bounds__(BLK) k_zero(double *p, long n) {
   if (idx < n) p[idx] = 0.0;
 }
 
-void member_blocks(const Params &P, int nblk[L_NKIND]) {
+void member_blocks(const Params &P, const int nw[3], int nblk[L_NKIND]) {
   nblk[L_FACE] = (int)grid_for((long)P.ny * P.nz);
   nblk[L_FACE2] = (int)((2L * P.ny * P.nz + NSCBC_BLK - 1) / NSCBC_BLK);
   nblk[L_POINT] = (int)grid_for(P.npts);
   auto tiles = [&](int tx, int ty, int tz) {
     return ((P.nx + tx - 1) / tx) * ((P.ny + ty - 1) / ty) * ((P.nz + tz - 1) / tz);
   };
-  nblk[L_PASSX] = tiles(PassTile<0>::TX, PassTile<0>::TY, PassTile<0>::TZ);
-  nblk[L_PASSY] = tiles(PassTile<1>::TX, PassTile<1>::TY, PassTile<1>::TZ);
-  nblk[L_PASSZ] = tiles(PassTile<2>::TX, PassTile<2>::TY, PassTile<2>::TZ);
+  nblk[L_PASSX] = nw[0] ? tiles(PassTile<0, 1>::TX, PassTile<0, 1>::TY, PassTile<0, 1>::TZ)
+                        : tiles(PassTile<0, 0>::TX, PassTile<0, 0>::TY, PassTile<0, 0>::TZ);
+  nblk[L_PASSY] = nw[1] ? tiles(PassTile<1, 1>::TX, PassTile<1, 1>::TY, PassTile<1, 1>::TZ)
+                        : tiles(PassTile<1, 0>::TX, PassTile<1, 0>::TY, PassTile<1, 0>::TZ);
+  nblk[L_PASSZ] = nw[2] ? tiles(PassTile<2, 1>::TX, PassTile<2, 1>::TY, PassTile<2, 1>::TZ)
+                        : tiles(PassTile<2, 0>::TX, PassTile<2, 0>::TY, PassTile<2, 0>::TZ);
 }
 
 void launch_errmax(const Params &P, const Fields &F, hipStream_t st) {

@@ -169,7 +169,8 @@ ABI_FUNCS = ["sg_abi_version", "sg_config_size", "sg_create", "sg_destroy", "sg_
              "sg_rhs", "sg_derivatives", "sg_sync", "sg_last_error", "sg_get_stream",
              "sg_profile", "sg_profile_read", "sg_memory", "sg_set_adaptive", "sg_get_timestep",
              "sg_batch_create", "sg_batch_destroy", "sg_batch_add", "sg_batch_remove", "sg_batch_size",
-             "sg_batch_member", "sg_batch_step", "sg_batch_sync", "sg_footprint"]
+             "sg_batch_member", "sg_batch_step", "sg_batch_sync", "sg_footprint", "sg_set_tiles",
+             "sg_get_tiles", "sg_autotune", "sg_batch_set_tiles", "sg_batch_get_tiles", "sg_batch_autotune"]
 
 
 def load_library(path=None):
@@ -202,6 +203,13 @@ def load_library(path=None):
     lib.sg_set_adaptive.argtypes = [vp, C.c_int, C.c_double, C.c_double]
     lib.sg_get_timestep.argtypes = [vp, dp, dp, dp]
     lib.sg_footprint.argtypes = [P(Config), P(C.c_size_t)]
+    i3 = C.c_int * 3
+    lib.sg_set_tiles.argtypes = [vp, i3, C.c_int]
+    lib.sg_get_tiles.argtypes = [vp, i3, P(C.c_int)]
+    lib.sg_autotune.argtypes = [vp, C.c_int, dp]
+    lib.sg_batch_set_tiles.argtypes = [vp, i3]
+    lib.sg_batch_get_tiles.argtypes = [vp, i3]
+    lib.sg_batch_autotune.argtypes = [vp, C.c_int, dp]
     lib.sg_batch_create.argtypes = [P(vp)]
     lib.sg_batch_destroy.argtypes = [vp]
     lib.sg_batch_add.argtypes = [vp, vp]
@@ -338,6 +346,25 @@ class Solver:
         self._chk(self.lib.sg_memory(self.h, C.byref(b)))
         return b.value
 
+    def set_tiles(self, narrow=(0, 0, 0), ychunk=0):
+        """Pass tiles per direction (0: rows of 32, 1: rows of 16) and the
+        final kernel's y-chunk (0 keeps it); results do not change."""
+        self._chk(self.lib.sg_set_tiles(self.h, (C.c_int * 3)(*narrow), int(ychunk)))
+
+    def tiles(self):
+        nw = (C.c_int * 3)()
+        yc = C.c_int(0)
+        self._chk(self.lib.sg_get_tiles(self.h, nw, C.byref(yc)))
+        return tuple(nw), yc.value
+
+    def autotune(self, reps=5):
+        """Time the tile choices on the current state and keep the fastest;
+        returns (narrow, ychunk, ms per substage of the tuned kernels)."""
+        ms = C.c_double(0)
+        self._chk(self.lib.sg_autotune(self.h, int(reps), C.byref(ms)))
+        nw, yc = self.tiles()
+        return nw, yc, ms.value
+
 
 class BatchError(SgError):
     def __init__(self, code, msg, member=-1, ijk=(0, 0, 0)):
@@ -386,6 +413,25 @@ class Batch:
         rc = self.lib.sg_batch_step(self.b, n)
         if rc:
             raise SgError(rc, "sg_batch_step failed")
+
+    def set_tiles(self, narrow=(0, 0, 0)):
+        rc = self.lib.sg_batch_set_tiles(self.b, (C.c_int * 3)(*narrow))
+        if rc:
+            raise SgError(rc, "sg_batch_set_tiles failed")
+
+    def tiles(self):
+        nw = (C.c_int * 3)()
+        rc = self.lib.sg_batch_get_tiles(self.b, nw)
+        if rc:
+            raise SgError(rc, "sg_batch_get_tiles failed")
+        return tuple(nw)
+
+    def autotune(self, reps=5):
+        ms = C.c_double(0)
+        rc = self.lib.sg_batch_autotune(self.b, int(reps), C.byref(ms))
+        if rc:
+            raise SgError(rc, "sg_batch_autotune failed")
+        return self.tiles(), ms.value
 
     def sync(self):
         failed = C.c_int(-1)

@@ -128,3 +128,49 @@ def test_batch_rejects_mismatched_members():
     b.step(1)
     b.sync()
     b.close()
+
+
+@pytest.mark.parametrize("kind,precision", [("nscbc", 0), ("kernel", 1)])
+def test_tiles_and_autotune_leave_results_unchanged(kind, precision):
+    """Every pass-tile choice and y-chunk computes bit-identical results, and
+    the autotuner restores the state it times on."""
+    specs = _specs(kind, [(48, 20, 16)], seed=21)
+    spec = specs[0]
+    print("tiles", kind, precision, spec["n"], flush=True)
+    ref = _alone(spec, 2, precision)
+    for narrow, yc in (((1, 1, 1), 4), ((1, 0, 1), 0), ((0, 1, 0), 2)):
+        s = _solver(spec, precision)
+        s.set_tiles(narrow, yc)
+        assert s.tiles()[0] == narrow
+        s.step(2)
+        run, T, P, _ = s.download()
+        s.close()
+        assert np.array_equal(run, ref[0]), f"tiles {narrow} ychunk {yc}"
+        assert np.array_equal(T, ref[1])
+    s = _solver(spec, precision)
+    nw, yc, ms = s.autotune(3)
+    print("autotuned", nw, yc, round(ms, 4), flush=True)
+    assert ms > 0 and spec["n"][1] % yc == 0
+    s.step(2)
+    run, T, _, _ = s.download()
+    s.close()
+    assert np.array_equal(run, ref[0]) and np.array_equal(T, ref[1])
+
+
+def test_batch_autotune_keeps_results():
+    from sengapy.abi import Batch
+    specs = _specs("nscbc", [(48, 16, 16), (32, 20, 16), (16, 16, 24)], seed=31)
+    ref = [_alone(s, 2) for s in specs]
+    b = Batch()
+    sol = [_solver(s) for s in specs]
+    for s in sol:
+        b.add(s)
+    nw, ms = b.autotune(3)
+    print("batch autotuned", nw, round(ms, 4), flush=True)
+    b.step(2)
+    b.sync()
+    for q, s in enumerate(sol):
+        assert np.array_equal(s.download()[0], ref[q][0]), f"member {q}"
+    b.set_tiles((1, 1, 1))
+    assert b.tiles() == (1, 1, 1)
+    b.close()
