@@ -112,6 +112,7 @@ def mixed_run(abi, c, a, h, s64):
     s = abi.Solver(c, precision=1)
     s.set_timestep(c.dump.tstep)
     s.upload(c.dump.state, 0.0)
+    tuned = s.autotune(5)
     stream = C.c_void_p(s.stream())
     ev0, ev1 = C.c_void_p(), C.c_void_p()
     h.hipEventCreate(C.byref(ev0))
@@ -146,6 +147,7 @@ def mixed_run(abi, c, a, h, s64):
             "max_rel_err_T": rel(T32, T64), "max_rel_err_Y": max(rel(r32[5 + k], r64[5 + k]) for k in range(ns)),
             "max_rel_err_conserved": max(rel(r32[v], r64[v]) for v in range(5 + ns)),
             "steps_compared": a.warmup + a.steps + a.profile_steps,
+            "autotuned": {"narrow_pass_tiles": list(tuned[0]), "final_ychunk": tuned[1]},
             "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in prof.items() if v[1] > 0}}
 
 
@@ -183,9 +185,10 @@ def batched_run(abi, h, cases, steps):
             s.set_timestep(c.dump.tstep)
             s.upload(c.dump.state, 0.0)
             sols.append(s)
-        # sequential: each case on its own, one after the other
+        # sequential: each case on its own (tiles autotuned per case), one after the other
         if prec == 0:
             for s in sols:
+                s.autotune(3)
                 s.step(1)
             h.hipDeviceSynchronize()
             t0 = time.perf_counter()
@@ -196,6 +199,7 @@ def batched_run(abi, h, cases, steps):
         b = abi.Batch()
         for s in sols:
             b.add(s)
+        tiles, _ = b.autotune(3)
         b.step(1)
         b.sync()
         t0 = time.perf_counter()
@@ -204,7 +208,7 @@ def batched_run(abi, h, cases, steps):
         t_b = time.perf_counter() - t0
         tag = "fp64" if prec == 0 else "mixed"
         out[tag] = {"cases_per_s": len(cases) / (t_b * 100.0 / steps), "s_per_step": t_b / steps,
-                    "mpoint_substages_per_s": pts * 5 * steps / t_b / 1e6}
+                    "mpoint_substages_per_s": pts * 5 * steps / t_b / 1e6, "autotuned_narrow_tiles": list(tiles)}
         if prec == 0:
             out["sequential_fp64"] = {"cases_per_s": len(cases) / (t_seq * 100.0 / steps),
                                       "s_per_step": t_seq / steps,
@@ -342,6 +346,8 @@ def main():
     s = abi.Solver(c)
     s.set_timestep(c.dump.tstep)
     s.upload(c.dump.state, 0.0)
+    tuned = s.autotune(5)
+    print(f"autotuned tiles {tuned}", file=sys.stderr, flush=True)
     h = hip()
     if a.copy_calibration:
         copy_calibration(h)
@@ -408,6 +414,7 @@ def main():
                      "mean_launch_ms": dom_ms},
         "substage_roofline_frac": substage_bytes * value * 1e6 / HBM_PEAK,
         "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kern.items()},
+        "autotuned": {"narrow_pass_tiles": list(tuned[0]), "final_ychunk": tuned[1]},
     }
     if not a.no_mixed:
         out["mixed"] = mixed_run(abi, c, a, h, s)

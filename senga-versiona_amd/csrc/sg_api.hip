@@ -290,7 +290,12 @@ static int init_launch(sg_handle *h, const int nw[3]) {
     h->lbase = nullptr;
     return SG_ENOMEM;
   }
-  if (hipMemset(h->lbase, 0, bytes) != hipSuccess) return SG_EDEVICE;
+  // zeroed on the handle's own stream: the handle's streams do not wait for
+  // the null stream, so a plain hipMemset could land after the member record
+  // that refresh_member copies in next
+  if (hipMemsetAsync(h->lbase, 0, bytes, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    return SG_EDEVICE;
   char *b = (char *)h->lbase;
   L.mem = (const Member *)b;
   b += (sizeof(Member) + 255) / 256 * 256;
@@ -408,7 +413,9 @@ static int batch_prepare(sg_batch *b) {
         for (int q = 0; q < nb[m][k]; q++) map[(size_t)hm[m].blk0[k] + q] = (uint32_t)m;
       b->L.map[k] = (const uint32_t *)p;
       b->L.nblk[k] = tot[k];
-      if (hipMemcpy(p, map.data(), map.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+      if (hipMemcpyAsync(p, map.data(), map.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream) !=
+              hipSuccess ||
+          hipStreamSynchronize(b->stream) != hipSuccess)
         return SG_EDEVICE;
       p += ((size_t)tot[k] * sizeof(uint32_t) + 255) / 256 * 256;
     }
@@ -424,7 +431,9 @@ static int batch_prepare(sg_batch *b) {
     if (b->exec) hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
   }
-  if (hipMemcpy((void *)b->L.mem, hm.data(), sizeof(Member) * nm, hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpyAsync((void *)b->L.mem, hm.data(), sizeof(Member) * nm, hipMemcpyHostToDevice, b->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
     return SG_EDEVICE;
   b->dirty_layout = b->dirty_members = 0;
   return 0;
@@ -575,9 +584,11 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   h->dcfg = (sg_config *)b; b += cfgsz;
   F.tab = (const Tables *)b; b += tabsz;
   F.err = (int *)b;
-  if (hipMemset(h->base, 0, h->bytes) != hipSuccess ||
-      hipMemcpy(h->dcfg, &h->cfg, sizeof(sg_config), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy((void *)F.tab, &h->tab, sizeof(Tables), hipMemcpyHostToDevice) != hipSuccess) {
+  // all on the handle's stream, which does not wait for the null stream
+  if (hipMemsetAsync(h->base, 0, h->bytes, h->stream) != hipSuccess ||
+      hipMemcpyAsync(h->dcfg, &h->cfg, sizeof(sg_config), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+      hipMemcpyAsync((void *)F.tab, &h->tab, sizeof(Tables), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess) {
     hipFree(h->base);
     hipStreamDestroy(h->stream);
     delete h;
