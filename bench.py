@@ -222,14 +222,14 @@ def batched_run(abi, h, cases, steps):
     return out
 
 
-def serve_run(cases, specs, steps):
+def serve_run(cases, specs, steps, point_budget=6e6, chunk=5):
     """Poisson request stream through the continuous-batching loop, FP64 and
     mixed, against sequential FP64 solves of the same stream."""
     from sengapy import serve
     reqs = [serve.Request(q, c, steps, sp["arrival"]) for q, (c, sp) in enumerate(zip(cases, specs))]
     out = {}
     for tag, prec in (("fp64", 0), ("mixed", 1)):
-        srv = serve.Server(precision=prec, keep=True)
+        srv = serve.Server(precision=prec, keep=True, point_budget=point_budget, chunk=chunk)
         res = srv.run(reqs)
         out[tag] = serve.summary(res)
         out[tag]["max_points_in_flight"] = srv.max_points_in_flight
@@ -314,6 +314,9 @@ def main():
                     help="flame-kernel requests of the served run (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-point-budget", type=float, default=6e6,
+                    help="grid points in flight beyond which the serving loop queues requests")
+    ap.add_argument("--serve-chunk", type=int, default=5, help="RK steps between admission points")
     ap.add_argument("--study-steps", type=int, default=0,
                     help="instead of the throughput run: FP64 vs mixed on the --n case for this many "
                          "steps, per-field errors")
@@ -424,10 +427,11 @@ def main():
                                       "phi~U(0.5,1.2), u'/S_L~U(1,10), T_u~U(300,700) K; "
                                       f"{a.batch_steps} RK steps timed, rates per 100-step case")
     if scases:
-        out["serving"] = serve_run(scases, sspecs, a.serve_steps)
+        out["serving"] = serve_run(scases, sspecs, a.serve_steps, a.serve_point_budget, a.serve_chunk)
         out["serving"]["workload"] = (f"{len(scases)} H2-air flame-kernel requests, Poisson {a.serve_rate}/s, "
                                       f"n^3 with n in 48..160, {a.serve_steps} RK steps each, "
-                                      "240 GB HBM budget, time to solution = arrival to downloaded state")
+                                      f"240 GB HBM budget, {a.serve_point_budget:.3g} points in flight, "
+                                      f"{a.serve_chunk}-step chunks, time to solution = arrival to downloaded state")
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
