@@ -77,8 +77,9 @@ class Server:
         queued (a few million points already fill the GPU; admitting more
         only stretches every admitted case's time to solution).  Measured on
         a Poisson stream of 48-160^3 cases: 3-6 M points keep the throughput
-        of larger budgets at the sequential time to solution or better.  A request larger
-        than the budget still runs when nothing else is in flight.
+        of larger budgets at the sequential time to solution or better.  A
+        request larger than the budget still runs when nothing else is in
+        flight.
     chunk: RK steps between admission/retirement points.
     precision: 0 = FP64, 1 = mixed (every case of a server shares it).
     keep: download the final state of each case into its Result.
