@@ -267,26 +267,8 @@ __global__ void __launch_bounds__(BLK) k_bountt(const Member *__restrict__ mem, 
 // (src/rhscal.f:515-573, 3854-3891, src/rhsvel.f:926-980), CHRATE
 // (src/chrate.f:146-760), primitive variables, diffusivities
 // (src/rhscal.f:1292-1346) and species enthalpies (src/rhscal.f:1444-1470).
-// Coefficient m of species s's interval it (0 or 1) from a two-interval table:
-// both candidates are uniform loads, the choice is per point.
-// The empty asm pins each candidate in scalar registers; without it the
-// compiler turns the choice of two loaded values into a per-point load from a
-// chosen address, i.e. a vector load and live registers for every coefficient.
-__device__ __forceinline__ double uni(double x) {
-  asm("" : "+s"(x));
-  return x;
-}
-template <int N>
-__device__ __forceinline__ double pick(const double (&a)[SG_NTINT][N], int it, int m) {
-  static_assert(SG_NTINT == 2, "two temperature intervals");
-  const double a0 = uni(a[0][m]), a1 = uni(a[1][m]);
-  return it ? a1 : a0;
-}
-__device__ __forceinline__ double pick1(const double (&a)[SG_NTINT], int it) {
-  const double a0 = uni(a[0]), a1 = uni(a[1]);
-  return it ? a1 : a0;
-}
-
+// Two-interval polynomials are evaluated for both intervals with uniform
+// (scalar) coefficients and the result is chosen per point.
 // a * b + c for a uniform c, as one VOP3 FMA that reads c from scalar
 // registers.  Left to itself the compiler copies c into vector registers and
 // uses the two-address FMAC form (two moves and an FMA in FP64, one move and
@@ -316,6 +298,35 @@ __device__ __forceinline__ double horner(const double (&a)[SG_NTINT][SG_NCPMAX],
   return it ? f1 : f0;
 }
 
+// c(it) + x * poly(it)(x) for a two-interval table: formed for both intervals
+// with scalar coefficients and the result chosen per point (a choice of the
+// coefficients instead costs register copies and selects for every one).
+__device__ __forceinline__ double horner_c(const double (&a)[SG_NTINT][SG_NCPMAX], const double (&c0)[SG_NTINT],
+                                           int it, double x) {
+  double f0 = a[0][SG_NCPMAX - 1], f1 = a[1][SG_NCPMAX - 1];
+#pragma unroll
+  for (int m = SG_NCPMAX - 2; m >= 0; m--) {
+    f0 = fma_u(f0, x, a[0][m]);
+    f1 = fma_u(f1, x, a[1][m]);
+  }
+  f0 = fma_u(f0, x, c0[0]);
+  f1 = fma_u(f1, x, c0[1]);
+  return it ? f1 : f0;
+}
+// Gibbs function gbh(it)/T - gbs(it) ln T - poly(it)(T), both intervals formed,
+// one chosen (ovt = 1/T).
+__device__ __forceinline__ double gibbs_of(const Tables *tb, int s, int it, double t, double ovt, double lt) {
+  double f0 = tb->gb[s][0][SG_NCPMAX - 1], f1 = tb->gb[s][1][SG_NCPMAX - 1];
+#pragma unroll
+  for (int m = SG_NCPMAX - 2; m >= 0; m--) {
+    f0 = fma_u(f0, t, tb->gb[s][0][m]);
+    f1 = fma_u(f1, t, tb->gb[s][1][m]);
+  }
+  const double g0 = tb->gbh[s][0] * ovt - tb->gbs[s][0] * lt - f0;
+  const double g1 = tb->gbh[s][1] * ovt - tb->gbs[s][1] * lt - f1;
+  return it ? g1 : g0;
+}
+
 // Transport fits, zero-padded to SG_NTRC terms.
 __device__ __forceinline__ double trfit(const double *a, double x) {
   double f = a[SG_NTRC - 1];
@@ -325,7 +336,7 @@ __device__ __forceinline__ double trfit(const double *a, double x) {
 }
 
 template <int NS>
-__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k_prim(const Member *__restrict__ mem,
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_prim(const Member *__restrict__ mem,
     const uint32_t *__restrict__ map, const sg_config *__restrict__ c, const Tables *__restrict__ tb) {
   int lb;
   const Member &M = member_at<L_POINT>(mem, map, blockIdx.x, lb);
@@ -343,7 +354,8 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
 #pragma unroll
   for (int s = 0; s < NS; s++) ry[s] = fld(q + (5 + s) * fs, bo);
   // --- TEMPER (src/temper.f:58-160) ---
-  double ukuk = (ru * ru + rv * rv + rw * rw) / d;
+  const double ovd = 1.0 / d;   // one reciprocal for every division by rho
+  double ukuk = (ru * ru + rv * rv + rw * rw) * ovd;
   double tc[6], td[6];
   tc[0] = 0.5 * ukuk - re;
 #pragma unroll
@@ -410,25 +422,24 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
     nidx += (uint32_t)it << (2 * s);
     cp = cp + ry[s] * horner(tb->cp[s], it, t);
   }
-  cp = cp / d;
+  cp = cp * ovd;
   fld(F.itndex, bo / 2u) = nidx;
   SG_PHASE();
   double pres = t * s7;
   fld(pr + P_T * fs, bo) = t;
   fld(pr + P_P * fs, bo) = pres;
   // primitives
-  fld(pr + P_U * fs, bo) = ru / d;
-  fld(pr + P_V * fs, bo) = rv / d;
-  fld(pr + P_W * fs, bo) = rw / d;
-  fld(pr + P_E * fs, bo) = re / d;
+  fld(pr + P_U * fs, bo) = ru * ovd;
+  fld(pr + P_V * fs, bo) = rv * ovd;
+  fld(pr + P_W * fs, bo) = rw * ovd;
+  fld(pr + P_E * fs, bo) = re * ovd;
   // --- conductivity (src/rhscal.f:515-573) ---
   double lam, transp, wmomix = 0.0;
-  {
+  if (!P.mixavg) {
     double fornow = P.alamda * exp(P.rlamda * log(t));
     lam = fornow * cp;
     transp = fornow;
-  }
-  if (P.mixavg) {
+  } else {
     transp = log(t / P.tdifgb);
     double c1 = 0.0, c2 = 0.0, c3 = 0.0;
 #pragma unroll
@@ -453,15 +464,14 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
     double rate[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) rate[s] = 0.0;
-    double lt = log(t);
+    const double lt = log(t), ovt = 1.0 / t;
     // Gibbs functions of all species at this temperature, each formed as
     // src/chrate.f:612-632 forms it for every step it enters
     double gibbs[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       int it = itn[s];
-      double g = horner(tb->gb[s], it, t);
-      gibbs[s] = pick1(tb->gbh[s], it) / t - pick1(tb->gbs[s], it) * lt - g;
+      gibbs[s] = gibbs_of(tb, s, it, t, ovt, lt);
     }
     // Molar concentrations and Gibbs functions of this point in LDS, so that
     // a step's species lists (uniform, known only at run time) index them
@@ -483,12 +493,12 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
 #pragma unroll
         for (int s = 0; s < NS; s++) s3 = s3 + ry[s] * (c->ovwmol[s] * c->effy3b[mb - 1][s]);
       }
-      double s2 = c->rparam[st][0] + c->rparam[st][1] * lt - c->rparam[st][2] / t;
+      double s2 = c->rparam[st][0] + c->rparam[st][1] * lt - c->rparam[st][2] * ovt;
       double s1 = exp(s2);
       int ml = c->mllist[st];
       if (ml != 0) {
         const double *rl = c->rclind[ml - 1];
-        double s4 = rl[0] + rl[1] * lt - rl[2] / t;
+        double s4 = rl[0] + rl[1] * lt - rl[2] * ovt;
         s4 = exp(s4);
         double preduc = s3 * s4 / s1;
         double fornow = rl[3] * preduc / (1.0 + preduc);  // rl[3] pre-rounded to REAL
@@ -500,10 +510,10 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
       if (mt != 0) {
         const double *rt = c->rctroe[mt - 1];
         double omalph = 1.0 - rt[3];
-        double s4 = rt[0] + rt[1] * lt - rt[2] / t;
+        double s4 = rt[0] + rt[1] * lt - rt[2] * ovt;
         s4 = exp(s4);
         double preduc = s3 * s4 / s1;
-        double trats1 = t * rt[4], trats2 = -rt[5] / t, trats3 = t * rt[6];
+        double trats1 = t * rt[4], trats2 = -rt[5] * ovt, trats3 = t * rt[6];
         double ftcent = omalph * exp(trats3) + rt[3] * exp(trats1) + exp(trats2);
         ftcent = log10(ftcent);
         double cfactr = rt[7] + rt[8] * ftcent;
@@ -521,10 +531,10 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
       int ms = c->mslist[st];
       if (ms != 0) {
         const double *rs = c->rcsrif[ms - 1];
-        double s4 = rs[0] + rs[1] * lt - rs[2] / t;
+        double s4 = rs[0] + rs[1] * lt - rs[2] * ovt;
         s4 = exp(s4);
         double preduc = s3 * s4 / s1;
-        double fornow = rs[4] / t;
+        double fornow = rs[4] * ovt;
         double fbroad = rs[3] * exp(fornow);
         fornow = t * rs[5];
         fbroad = fbroad + exp(fornow);
@@ -586,13 +596,13 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
   // --- mass fractions and enthalpies (src/rhscal.f:1444-1470) ---
   double y[NS];
 #pragma unroll
-  for (int s = 0; s < NS; s++) y[s] = ry[s] / d;
+  for (int s = 0; s < NS; s++) y[s] = ry[s] * ovd;
   double hmix = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     fld(pr + prim_y(P, s), bo) = y[s];
     int it = itn[s];
-    double h = pick1(tb->hch[s], it) + horner(tb->ch[s], it, t) * t;
+    double h = horner_c(tb->ch[s], tb->hch[s], it, t);
     fld(pr + prim_h(P, s), bo) = h;
     hmix = hmix + h * y[s];
   }
@@ -698,7 +708,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
       double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
       long fq = (long)k * P.ny + j;
       fc[F_CP * nf + fq] = cp;
-      fc[F_RR * nf + fq] = s7 / d;
+      fc[F_RR * nf + fq] = s7 * ovd;
     }
   }
 }
@@ -728,7 +738,8 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
   for (int s = 0; s < NS; s++) ry[s] = fld(q + (5 + s) * fs, bo);
   // --- TEMPER (src/temper.f:58-160) ---
-  double ukuk = (ru * ru + rv * rv + rw * rw) / d;
+  const double ovd = 1.0 / d;   // one reciprocal for every division by rho
+  double ukuk = (ru * ru + rv * rv + rw * rw) * ovd;
   double tc[6], td[6];
   tc[0] = 0.5 * ukuk - re;
 #pragma unroll
@@ -795,21 +806,21 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
     nidx += (uint32_t)it << (2 * s);
     cp = cp + ry[s] * horner(tb->cp[s], it, t);
   }
-  cp = cp / d;
+  cp = cp * ovd;
   fld(F.itndex, bo / 2u) = nidx;
   SG_PHASE();
   double pres = t * s7;
   fld(pr + P_T * fs, b4) = (float)(t);
   fld(pr + P_P * fs, b4) = (float)(pres - P.pofs);
   // primitives
-  fld(pr + P_U * fs, b4) = (float)(ru / d);
-  fld(pr + P_V * fs, b4) = (float)(rv / d);
-  fld(pr + P_W * fs, b4) = (float)(rw / d);
-  fld(pr + P_E * fs, b4) = (float)(re / d);
+  fld(pr + P_U * fs, b4) = (float)(ru * ovd);
+  fld(pr + P_V * fs, b4) = (float)(rv * ovd);
+  fld(pr + P_W * fs, b4) = (float)(rw * ovd);
+  fld(pr + P_E * fs, b4) = (float)(re * ovd);
   // --- conductivity (src/rhscal.f:515-573), FP32 ---
   const float tf = (float)t;
-  float lam, transp, wmomix = 0.0f;
-  {
+  float lam = 0.0f, transp = 0.0f, wmomix = 0.0f;
+  if (!P.mixavg) {
     float fornow = (float)P.alamda * __expf((float)P.rlamda * __logf(tf));
     lam = fornow * (float)cp;
     transp = fornow;
@@ -853,8 +864,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       int it = itn[s];
-      double g = horner(tb->gb[s], it, t);
-      gibbs[s] = pick1(tb->gbh[s], it) * ovt - pick1(tb->gbs[s], it) * lt - g;
+      gibbs[s] = gibbs_of(tb, s, it, t, ovt, lt);
     }
     __shared__ float lcy[NS][BLK];
     __shared__ double lgb[NS][BLK];
@@ -975,11 +985,11 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   double hmix = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; s++) {
-    const double yd = ry[s] / d;
+    const double yd = ry[s] * ovd;
     y[s] = (float)yd;
     fld(pr + prim_y(P, s), b4) = y[s];
     int it = itn[s];
-    double h = pick1(tb->hch[s], it) + horner(tb->ch[s], it, t) * t;
+    double h = horner_c(tb->ch[s], tb->hch[s], it, t);
     fld(pr + prim_h(P, s), b4) = (float)h;
     hmix = hmix + h * yd;
   }
@@ -1080,7 +1090,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
       long fq = (long)k * P.ny + j;
       fc[F_CP * nf + fq] = cp;
-      fc[F_RR * nf + fq] = s7 / d;
+      fc[F_RR * nf + fq] = s7 * ovd;
     }
   }
 }
