@@ -17,7 +17,8 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "lib", "libsenga_gpu.so")
 HIPCC = "/opt/rocm/bin/hipcc"
 SPECIES = list(range(2, 13))
-# Packed FP32 instructions are disabled for all device code.
+# Packed FP32 instructions are disabled for all device code: some of their forms have
+# given wrong results on this hardware, and unpacked FP32 keeps the arithmetic exact.
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I", os.path.join(ROOT, "include"), "-I", CSRC]

@@ -11,7 +11,7 @@ for p in (os.path.join(ROOT, "senga-versiona_amd"), os.path.join(ROOT, "oracle")
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X GPU (run on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
 
 
 @pytest.fixture(scope="session")

@@ -1,6 +1,6 @@
 """HIP path against the reference fixtures and the C oracle (needs the GPU).
 
-Tolerances (FP64 path): the north-star bar is <= 1e-10 max-norm relative error
+Tolerances (FP64 path): the bar is <= 1e-10 max-norm relative error
 per field after N steps against the reference.  The GPU evaluates the same
 formulas but sums direction-split terms in a different order and contracts
 multiply-adds, and ROCm's exp/log differ from glibc's by an ulp; single RHS
