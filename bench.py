@@ -396,6 +396,18 @@ def main():
         t = json.load(open(tfile))
         if t.get("workload") == f"h2_nscbc_mix_{n}^3" and dom in t.get("kernels", {}):
             traffic = t["kernels"][dom]
+    # The point-physics kernel is bound by the vector ALU, not HBM: its FP64
+    # rate and VALU issue use from the per-type instruction counters of the
+    # same workload (committed rocprofv3 summary), beside the HBM figure.
+    valu = None
+    vfile = os.path.join(ROOT, "profiles", "valu.json")
+    if os.path.exists(vfile):
+        vj = json.load(open(vfile))
+        k = vj.get("kernels", {}).get(f"k_prim<{ns}>")
+        if vj.get("workload") == f"h2_nscbc_mix_{n}^3" and k:
+            valu = {"kernel": "prim", "bound": "fp64 valu", "achieved": k["fp64_tflops"], "peak": 78.6,
+                    "unit": "TFLOP/s", "frac": k["fp64_tflops"] / 78.6, "valu_issue_use": k["valu_issue_util"],
+                    "source": "profiles/valu_h2_nscbc_mix_128.md (rocprofv3 --pmc)"}
     substage_bytes = 32 * (5 + ns) + 16      # compulsory state traffic of one fused substage
     out = {
         "metric": "Mpoint-RK-substages/s",
@@ -417,6 +429,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "algorithmic_bytes_per_point": kernel_bytes(dom, ns, True),
                      "mean_launch_ms": dom_ms},
+        "valu_roofline": valu,
         "substage_roofline_frac": substage_bytes * value * 1e6 / HBM_PEAK,
         "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kern.items()},
         "tiles": {"narrow_pass_tiles": list(tuned[0]), "final_ychunk": tuned[1], "autotuned": not a.no_autotune},
