@@ -17,8 +17,9 @@ time to solution.
 
 Output: one JSON line.  roofline: the dominant kernel's algorithmic HBM bytes
 per launch over its mean duration (HIP events on the library's stream);
-cpu_baseline: the C oracle (a scalar restatement of the reference, one core)
-on a bounded sample of the same physics.
+cpu_baseline: the reference SENGA2 compiled from its own sources (one rank,
+one core; the C oracle where that binary is absent) on a bounded 32^3 sample
+of the same physics.
 """
 import argparse
 import ctypes as C
@@ -79,12 +80,16 @@ def kernel_bytes(name, ns, bounded):
     return 0
 
 
-def cpu_baseline(seconds_target=15.0):
+def _baseline_case():
+    from sengapy import case as Cs
+    return Cs.h2_nscbc(n=(32, 32, 32), length=(1.25e-3, 1.25e-3, 1.25e-3), mixavg=True, nsteps=1)
+
+
+def cpu_baseline_port(seconds_target=10.0):
     """Oracle (scalar C restatement, one thread) on a 32^3 sample of the same
     H2 NSCBC mixture-averaged physics; whole RK steps."""
     from pyoracle import Oracle
-    from sengapy import case as Cs
-    c = Cs.h2_nscbc(n=(32, 32, 32), length=(1.25e-3, 1.25e-3, 1.25e-3), mixavg=True, nsteps=1)
+    c = _baseline_case()
     o = Oracle(c)
     o.set_timestep(c.dump.tstep)
     o.set_state(c.dump.state, 0.0)
@@ -102,6 +107,35 @@ def cpu_baseline(seconds_target=15.0):
     return {"value": npts * nsub / dt / 1e6, "unit": "Mpoint-RK-substages/s", "cores": 1, "kind": "port",
             "sample": f"C oracle, 32^3 H2-air NSCBC mixture-averaged FP64, {nsub} substages "
                       f"({nsub // 5} RK steps) in {dt:.1f} s on one host core"}
+
+
+def cpu_baseline(nsteps=10):
+    """The reference SENGA2 itself, compiled from its sources (oracle/_ref,
+    single rank, one core) on a 32^3 sample of the same H2 NSCBC mixture-
+    averaged physics: wall time of nsteps full RK steps minus that of a
+    0-step run (input, TEMPIN and output).  Falls back to the C oracle where
+    the reference binary was not built; the oracle's figure is reported too."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle", "ref"))
+    port = cpu_baseline_port()
+    try:
+        import run_ref
+        c = _baseline_case()
+        run_ref.build_for(c)
+        t = {}
+        for n in (0, nsteps):
+            t0 = time.perf_counter()
+            run_ref.run(c, 2, nsteps=n)
+            t[n] = time.perf_counter() - t0
+        dt = t[nsteps] - t[0]
+    except (FileNotFoundError, RuntimeError, ImportError) as e:
+        print(f"reference binary unavailable ({e}); oracle baseline only", file=sys.stderr, flush=True)
+        return port
+    npts = 32 ** 3
+    return {"value": npts * 5 * nsteps / dt / 1e6, "unit": "Mpoint-RK-substages/s", "cores": 1, "kind": "reference",
+            "sample": f"reference SENGA2 built from its own sources (flang -O2, one rank, its mpi_stub), "
+                      f"32^3 H2-air NSCBC mixture-averaged FP64, {nsteps} RK steps: {t[nsteps]:.1f} s minus "
+                      f"{t[0]:.1f} s for a 0-step run, one host core",
+            "port": port}
 
 
 def mixed_run(abi, c, a, h, s64):

@@ -85,8 +85,11 @@ def build(name, nx, ny, nz, nspec, nstep, jobs=8):
     objd = os.path.join(out, "obj")
     os.makedirs(srcd, exist_ok=True)
     os.makedirs(objd, exist_ok=True)
-    with open(os.path.join(srcd, "com_senga2.h"), "w") as f:
-        f.write(case_header(nx, ny, nz, nspec, nstep))
+    hdr = os.path.join(srcd, "com_senga2.h")
+    text = case_header(nx, ny, nz, nspec, nstep)
+    if not os.path.exists(hdr) or open(hdr).read() != text:   # unchanged: objects stay valid
+        with open(hdr, "w") as f:
+            f.write(text)
     link(os.path.join(REF, "com_espect.h"), os.path.join(srcd, "com_espect.h"))
     link(os.path.join(REF, "mpi_stub", "mpif.h"), os.path.join(srcd, "mpif.h"))
     units = []

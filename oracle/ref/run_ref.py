@@ -22,9 +22,22 @@ sys.path.insert(0, os.path.join(ROOT, "senga-versiona_amd"))
 import build_ref  # noqa: E402
 
 
-def build_for(case):
+def build_name(case):
     nx, ny, nz = case.shape
-    name = f"n{nx}x{ny}x{nz}_s{case.nspec}_r{case.tables.nstep}"
+    return f"n{nx}x{ny}x{nz}_s{case.nspec}_r{case.tables.nstep}"
+
+
+def build_for(case):
+    """The reference binary for this case's compile-time sizes: built from
+    the reference sources where they are present, else the one built earlier
+    (the build directory travels with the repository)."""
+    nx, ny, nz = case.shape
+    name = build_name(case)
+    exe = os.path.join(ROOT, "oracle", "_ref", name, "refrun")
+    if not os.path.isdir(build_ref.REF):
+        if os.path.exists(exe):
+            return exe
+        raise FileNotFoundError(f"{exe} is not built and the reference sources are absent")
     return build_ref.build(name, nx, ny, nz, case.nspec, case.tables.nstep)
 
 
