@@ -146,7 +146,7 @@ def mixed_run(abi, c, a, h, s64):
     s = abi.Solver(c, precision=1)
     s.set_timestep(c.dump.tstep)
     s.upload(c.dump.state, 0.0)
-    tuned = s.tiles() + (None,) if a.no_autotune else s.autotune(5)
+    tuned = (s.tiles(), None) if a.no_autotune else s.autotune(5)
     stream = C.c_void_p(s.stream())
     ev0, ev1 = C.c_void_p(), C.c_void_p()
     h.hipEventCreate(C.byref(ev0))
@@ -181,7 +181,7 @@ def mixed_run(abi, c, a, h, s64):
             "max_rel_err_T": rel(T32, T64), "max_rel_err_Y": max(rel(r32[5 + k], r64[5 + k]) for k in range(ns)),
             "max_rel_err_conserved": max(rel(r32[v], r64[v]) for v in range(5 + ns)),
             "steps_compared": a.warmup + a.steps + a.profile_steps,
-            "tiles": {"narrow_pass_tiles": list(tuned[0]), "final_ychunk": tuned[1], "autotuned": not a.no_autotune},
+            "tiles": {"narrow_pass_tiles": list(tuned[0]), "autotuned": not a.no_autotune},
             "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in prof.items() if v[1] > 0}}
 
 
@@ -385,7 +385,7 @@ def main():
     s = abi.Solver(c)
     s.set_timestep(c.dump.tstep)
     s.upload(c.dump.state, 0.0)
-    tuned = s.tiles() + (None,) if a.no_autotune else s.autotune(5)
+    tuned = (s.tiles(), None) if a.no_autotune else s.autotune(5)
     print(f"tiles {tuned}", file=sys.stderr, flush=True)
     h = hip()
     if a.copy_calibration:
@@ -466,7 +466,7 @@ def main():
         "valu_roofline": valu,
         "substage_roofline_frac": substage_bytes * value * 1e6 / HBM_PEAK,
         "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kern.items()},
-        "tiles": {"narrow_pass_tiles": list(tuned[0]), "final_ychunk": tuned[1], "autotuned": not a.no_autotune},
+        "tiles": {"narrow_pass_tiles": list(tuned[0]), "autotuned": not a.no_autotune},
     }
     if not a.no_mixed:
         out["mixed"] = mixed_run(abi, c, a, h, s)

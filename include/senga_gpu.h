@@ -226,14 +226,13 @@ int sg_memory(sg_handle *h, size_t *bytes);
  * narrow[d] selects the tile of the direction-d derivative pass: 0 = rows of
  * 32 points (default), 1 = rows of 16 points and twice the rows (no idle
  * lanes when n_x is an odd multiple of 16, a shorter halo along y and z).
- * ychunk (> 0, dividing ny; 0 keeps the current one) is the number of y rows
- * the final kernel sweeps together.  Neither changes any result.
- * sg_autotune times each choice on the handle's current state (reps launches
- * each, 5 if reps <= 0), keeps the fastest, and returns in *ms the tuned
- * passes' and final kernel's time per substage; the state it leaves is the
- * one it found, so tuned and untuned runs compute identical results. */
-int sg_set_tiles(sg_handle *h, const int narrow[3], int ychunk);
-int sg_get_tiles(sg_handle *h, int narrow[3], int *ychunk);
+ * No choice changes any result.  sg_autotune times each choice on the
+ * handle's current state (reps launches each, 5 if reps <= 0), keeps the
+ * fastest, and returns in *ms the tuned passes' time per substage; the state
+ * it leaves is the one it found, so tuned and untuned runs compute identical
+ * results. */
+int sg_set_tiles(sg_handle *h, const int narrow[3]);
+int sg_get_tiles(sg_handle *h, int narrow[3]);
 int sg_autotune(sg_handle *h, int reps, double *ms);
 
 /* Device bytes sg_create would allocate for cfg (without allocating), for

@@ -264,9 +264,6 @@ static int setup_params(sg_handle *h) {
   P.mixavg = c.mixavg;
   P.precision = c.precision;
   P.pofs = c.precision ? c.prin : 0.0;
-  P.ychunk = 1;
-  for (int d = 32; d >= 1; d--)
-    if (P.ny % d == 0) { P.ychunk = d; break; }
   P.alamda = c.alamda; P.rlamda = c.rlamda; P.prantl = c.prantl;
   P.tdifgb = c.tdifgb; P.pdifgb = c.pdifgb;
   return 0;
@@ -1120,20 +1117,16 @@ static double time_launches(hipStream_t st, int reps, Fn fn) {
 
 extern "C" {
 
-int sg_set_tiles(sg_handle *h, const int narrow[3], int ychunk) {
+int sg_set_tiles(sg_handle *h, const int narrow[3]) {
   if (!h || !narrow) return SG_EINVAL;
   if (h->batch) return set_err(h, SG_ESTATE, "handle advances in a batch");
-  if (ychunk < 0 || (ychunk > 0 && h->P.ny % ychunk != 0)) return set_err(h, SG_EINVAL, "ychunk must divide ny");
-  if (ychunk > 0) h->P.ychunk = ychunk;
   if (init_launch(h, narrow) != 0) return set_err(h, SG_ENOMEM, "launch table allocation failed");
   return refresh_member(h);
 }
 
-int sg_get_tiles(sg_handle *h, int narrow[3], int *ychunk) {
-  if (!h) return SG_EINVAL;
-  if (narrow)
-    for (int d = 0; d < 3; d++) narrow[d] = h->L.pass_nw[d];
-  if (ychunk) *ychunk = h->P.ychunk;
+int sg_get_tiles(sg_handle *h, int narrow[3]) {
+  if (!h || !narrow) return SG_EINVAL;
+  for (int d = 0; d < 3; d++) narrow[d] = h->L.pass_nw[d];
   return 0;
 }
 
@@ -1166,31 +1159,6 @@ int sg_autotune(sg_handle *h, int reps, double *ms) {
   }
   if (!rc && init_launch(h, nw) != 0) rc = set_err(h, SG_ENOMEM, "launch table allocation failed");
   if (!rc) rc = refresh_member(h);
-  // point order of the final kernel: rows per y-chunk
-  double *out = nullptr;
-  if (!rc && hipMalloc((void **)&out, sizeof(double) * (size_t)h->P.fstride * (5 + h->P.ns)) != hipSuccess)
-    rc = set_err(h, SG_ENOMEM, "scratch allocation failed");
-  if (!rc) {
-    for (int d = 0; d < 3; d++) launch_pass(d, h->L, h->stream);
-    if (h->L.bndx) launch_nscbc(h->L, h->stream);
-    double tb = 1e30;
-    int bc = h->P.ychunk;
-    const int cand[7] = {1, 2, 4, 8, 16, 32, 64};
-    for (int q = 0; q < 7 && !rc; q++) {
-      if (h->P.ny % cand[q] != 0) continue;
-      h->P.ychunk = cand[q];
-      rc = refresh_member(h);
-      if (rc) break;
-      const double t = time_launches(h->stream, reps, [&] { launch_final(h->L, 0, out, h->stream); });
-      if (t < tb) { tb = t; bc = cand[q]; }
-    }
-    h->P.ychunk = bc;
-    total += tb;
-  }
-  if (out) {
-    hipStreamSynchronize(h->stream);
-    hipFree(out);
-  }
   int rc2 = snap_restore(h, sn, h->stream);
   if (!rc) rc = rc2;
   if (!rc) rc = refresh_member(h);

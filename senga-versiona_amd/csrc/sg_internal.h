@@ -29,7 +29,6 @@ struct Params {
   int rklo[6];                 // first x index of the RK update for D,U,V,W,E,Y
   double drin, uin, cobcxr, pinfxr;
   double yrin[SG_MAX_SPEC];    // inflow mass fractions (NSBCI3)
-  int ychunk;                  // rows per y-chunk of the final kernel's point order (divides ny)
   int precision;               // 0: FP64; 1: mixed (FP32 primitives, rates, RHS; FP64 registers, T)
   double pofs;                 // stored pressure is p - pofs (mixed: the default pressure, else 0)
   int mixavg;
@@ -104,7 +103,7 @@ inline __host__ __device__ int face_corr(int ns) { return F_NSCAL + ns; }
 // kernel kind.  The grid of a kind is the concatenation of every member's own
 // grid of that kind: flattened block b belongs to member map[kind][b] and is
 // that member's block b - blk0[kind].  A single case is a launch of one member.
-enum LaunchKind { L_FACE = 0, L_FACE2, L_POINT, L_PASSX, L_PASSY, L_PASSZ, L_NKIND };   // L_FACE2: both x faces
+enum LaunchKind { L_FACE = 0, L_FACE2, L_POINT, L_PASSX, L_PASSY, L_PASSZ, L_FINAL, L_NKIND };   // L_FACE2: both x faces
 
 struct Member {
   Params P;

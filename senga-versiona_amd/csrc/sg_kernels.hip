@@ -1712,49 +1712,34 @@ __global__ void __launch_bounds__(NSCBC_BLK) k_nscbc(const Member *__restrict__ 
 
 // ---------------------------------------------------------------------------
 // Final kernel: cross-direction viscous terms (src/rhsvel.f:992-2185), NSCBC
-// face corrections (src/bounds.f:115-206, 610-682, 1196-1458) and the RK
-// register update (src/lincom.f:53-169, src/fincom.f:54-168).
+// face corrections (added; formed by k_nscbc) and the RK register update
+// (src/lincom.f:53-169, src/fincom.f:54-168).
+//
+// A block owns an FX x FY x FZ tile of points, NPT per thread along z.  The
+// six mixed derivatives (d2u/dxdy, d2u/dxdz, d2v/dxdy, d2v/dydz, d2w/dxdz,
+// d2w/dydz) read their diagonal neighbours from u, v and w staged one at a
+// time into an LDS tile with a 5-deep halo in every direction (wrapped where
+// periodic, clamped where bounded), instead of 120 neighbour loads per point.
+struct FinalTile {
+  static constexpr int FX = 16, FY = 8, FZ = 4;
+  static constexpr int NPT = FX * FY * FZ / BLK;   // points per thread
+  static constexpr int ZS = BLK / (FX * FY);        // z stride between a thread's points
+  static constexpr int GX = FX + 10, GY = FY + 10, GZ = FZ + 10, GN = GX * GY * GZ;
+};
+
+// The per-point part of the final kernel at (i, j, k), given its six mixed
+// derivatives.
 template <bool BND, int NS, class R>
-__global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, const uint32_t *__restrict__ map,
-                                               const sg_config *__restrict__ c, int irk, double *__restrict__ out) {
-  int lb;
-  const Member &M = member_at<L_POINT>(mem, map, xcd_block(blockIdx.x, gridDim.x), lb);
-  const Params P = M.P;
-  const Fields F = global_fields(M.F);
-  // substage irk: LINCOM (mode 0) or FINCOM at the last substage (mode 1);
-  // irk = 0 writes the right-hand side to out (mode 2)
-  const int mode = irk == 0 ? 2 : (irk < 5 ? 0 : 1);
-  const double lhs = irk ? M.rkl[irk - 1] : 0.0, rhsc = irk ? M.rkr[irk - 1] : 0.0,
-               erc = irk ? M.rke[irk - 1] : 0.0;
-  // bounded x: mixed-derivative row weights in LDS (uniform row form)
-  __shared__ R mtab[BND ? SG_NROW * 7 : 1];   // in the working precision
-  const int tl = threadIdx.x;
-  if (BND) {
-    if (tl < SG_NROW * 7) mtab[tl] = (R)(&F.tab->mixw[0][0])[tl];
-    __syncthreads();
-  }
-  // Point order: x fastest, then z, then y within chunks of P.ychunk rows,
-  // so that the +-5-plane z neighbours of the mixed derivatives stay in the
-  // XCD's L2 while its blocks sweep a chunk.
-  const long lp = (long)lb * BLK + tl;
-  if (lp >= P.npts) return;
-  int i, j, k;
-  {
-    const long row = lp / P.nx;
-    i = (int)(lp - row * P.nx);
-    const long per = (long)P.ychunk * P.nz;
-    const int jc = (int)(row / per);
-    const int rem = (int)(row - (long)jc * per);
-    k = rem / P.ychunk;
-    j = jc * P.ychunk + (rem - k * P.ychunk);
-  }
+__device__ __forceinline__ void final_point(const Params &P, const Fields &F, const sg_config *__restrict__ c,
+                                            int i, int j, int k, int mode, double lhs, double rhsc, double erc,
+                                            double *__restrict__ out, R uxy, R vxy, R uxz, R wxz, R vyz, R wyz) {
   const long idx = ((long)k * P.ny + j) * P.nx + i;
   const long fs = P.fstride;
   constexpr int ES = (int)sizeof(R);      // bytes per element of the FP32/FP64 fields
   const uint32_t bo = (uint32_t)idx * (uint32_t)ES, bo8 = (uint32_t)idx * 8u;
   const R *pr = (const R *)F.prim;
-  const R *uu = pr + P_U * fs, *vv = pr + P_V * fs, *ww = pr + P_W * fs;
-  const R u = fld(uu, bo), v = fld(vv, bo), w = fld(ww, bo), mu = fld(pr + P_MU * fs, bo);
+  const R u = fld(pr + P_U * fs, bo), v = fld(pr + P_V * fs, bo), w = fld(pr + P_W * fs, bo),
+          mu = fld(pr + P_MU * fs, bo);
   const R *G = (const R *)F.grad;
   auto gr = [&](int q) { return fld(G + q * fs, bo); };
   const R *A = (const R *)F.acc;
@@ -1786,53 +1771,9 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
     rW = rW + t;
     rE = rE + w * t;
   }
-  // Cross derivatives from neighbours at byte offsets: x bounded (closure
-  // rows stay inside) or periodic, y and z periodic.
-  const int nx = P.nx, ny = P.ny, nz = P.nz;
-  auto ox = [&](int a) -> int {
-    if (BND) return a * ES;
-    int ii = i + a;
-    ii += ii < 0 ? nx : (ii >= nx ? -nx : 0);
-    return (ii - i) * ES;
-  };
-  auto oy = [&](int b) -> int {
-    int jj = j + b;
-    jj += jj < 0 ? ny : (jj >= ny ? -ny : 0);
-    return (jj - j) * nx * ES;
-  };
-  auto oz = [&](int b) -> int {
-    int kk = k + b;
-    kk += kk < 0 ? nz : (kk >= nz ? -nz : 0);
-    return (kk - k) * nx * ny * ES;
-  };
-  // x closure row of this point; waves holding any closure row use the
-  // uniform row form, others the interior diagonal stencil
-  int row = 0;
-  if (BND) {
-    if (i < 5) row = 1 + i;
-    else if (i > nx - 6) row = 6 + (nx - 1 - i);
-  }
-  const bool xedge = BND && __ballot(row != 0) != 0;
-  int pk[5] = {0, 0, 0, 0, 0};
-  if (xedge) {
-#pragma unroll
-    for (int q = 0; q < 5; q++) pk[q] = c_mix_row_offsets.w[row][q];
-  }
-  Sten S0;
-  S0.mode = 0;
-  auto mixx = [&](const R *f, auto od2, int d2) {
-    auto g = [&](int a, int b) { return fld(f, bo + (uint32_t)(ox(a) + od2(b))); };
-    if (xedge)
-      return dmix_rows<R>(P, pk, 0, d2, [&](int q) { return mtab[BND ? row * 7 + q : 0]; }, g);
-    return dmix<false, R>(P, S0, 0, d2, [&](int k1, int k2) { return g(k1 - 8, k2 - 8); });
-  };
-  auto myz = [&](const R *f) {
-    return dmix<false, R>(P, S0, 1, 2, [&](int k1, int k2) { return fld(f, bo + (uint32_t)(oy(k1 - 8) + oz(k2 - 8))); });
-  };
   {  // xy pair
     const R S7 = gr(4) + gr(1);   // du/dy + dv/dx
-    R ma = mixx(uu, oy, 1), mb = mixx(vv, oy, 1);
-    pin(ma); pin(mb);
+    const R ma = uxy, mb = vxy;
     R t = fz(P.fvsn, 0) ? R(0) : tthd * mu * mb;
     rU = rU - t; rE = rE - u * t;
     t = fz(P.fvsn, 1) ? R(0) : tthd * mu * ma;
@@ -1845,8 +1786,7 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
   }
   {  // xz pair
     const R S7 = gr(8) + gr(2);   // du/dz + dw/dx
-    R ma = mixx(uu, oz, 2), mb = mixx(ww, oz, 2);
-    pin(ma); pin(mb);
+    const R ma = uxz, mb = wxz;
     R t = fz(P.fvsn, 0) ? R(0) : tthd * mu * mb;
     rU = rU - t; rE = rE - u * t;
     t = fz(P.fvsn, 2) ? R(0) : tthd * mu * ma;
@@ -1859,8 +1799,7 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
   }
   {  // yz pair
     const R S7 = gr(9) + gr(6);   // dv/dz + dw/dy
-    R ma = myz(vv), mb = myz(ww);
-    pin(ma); pin(mb);
+    const R ma = vyz, mb = wyz;
     R t = fz(P.fvsn, 1) ? R(0) : tthd * mu * mb;
     rV = rV - t; rE = rE - v * t;
     t = fz(P.fvsn, 2) ? R(0) : tthd * mu * ma;
@@ -1894,7 +1833,7 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
   if (mode == 2) {
 #pragma unroll
     for (int vv2 = 0; vv2 < 5 + NS; vv2++) out[vv2 * fs + idx] = Rv[vv2];
-    return;
+    return;   // no register update
   }
 #pragma unroll
   for (int vv2 = 0; vv2 < 5 + NS; vv2++) {
@@ -1914,6 +1853,108 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
     } else {
       fld(qv, bo8) = (double)Rv[vv2];
     }
+  }
+}
+
+template <bool BND, int NS, class R>
+__global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, const uint32_t *__restrict__ map,
+                                               const sg_config *__restrict__ c, int irk, double *__restrict__ out) {
+  using FT = FinalTile;
+  constexpr int FX = FT::FX, FY = FT::FY, FZ = FT::FZ, NPT = FT::NPT, GX = FT::GX, GY = FT::GY;
+  int lb;
+  const Member &M = member_at<L_FINAL>(mem, map, xcd_block(blockIdx.x, gridDim.x), lb);
+  const Params P = M.P;
+  const Fields F = global_fields(M.F);
+  // substage irk: LINCOM (mode 0) or FINCOM at the last substage (mode 1);
+  // irk = 0 writes the right-hand side to out (mode 2)
+  const int mode = irk == 0 ? 2 : (irk < 5 ? 0 : 1);
+  const double lhs = irk ? M.rkl[irk - 1] : 0.0, rhsc = irk ? M.rkr[irk - 1] : 0.0,
+               erc = irk ? M.rke[irk - 1] : 0.0;
+  // bounded x: mixed-derivative row weights (uniform row form); the field tile
+  __shared__ R mtab[BND ? SG_NROW * 7 : 1];   // in the working precision
+  __shared__ R tile[FT::GN];
+  const int t = threadIdx.x;
+  if (BND && t < SG_NROW * 7) mtab[t] = (R)(&F.tab->mixw[0][0])[t];   // ordered by the first staging barrier
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  const int nbx = (nx + FX - 1) / FX, nby = (ny + FY - 1) / FY;
+  const int x0 = (lb % nbx) * FX, y0 = ((lb / nbx) % nby) * FY, z0 = (lb / (nbx * nby)) * FZ;
+  const int lx = t % FX, ly = (t / FX) % FY, lz0 = t / (FX * FY);
+  const int i = x0 + lx, j = y0 + ly;
+  // x closure row of this thread's points; waves holding any closure row use
+  // the uniform row form, others the interior diagonal stencil
+  int row = 0;
+  if (BND && i < nx) {
+    if (i < 5) row = 1 + i;
+    else if (i > nx - 6) row = 6 + (nx - 1 - i);
+  }
+  const bool xedge = BND && __ballot(row != 0) != 0;
+  int pk[5] = {0, 0, 0, 0, 0};
+  if (xedge) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) pk[q] = c_mix_row_offsets.w[row][q];
+  }
+  Sten S0;
+  S0.mode = 0;
+  const long fs = P.fstride;
+  const R *pr = (const R *)F.prim;
+  // halo index along one direction: wrapped when periodic, clamped when
+  // bounded; a second clamp keeps the halo of points past the end of a
+  // partial tile (whose results are not used) inside the field
+  auto hidx = [&](int g, int n, bool bounded) {
+    if (bounded) return g < 0 ? 0 : (g >= n ? n - 1 : g);
+    g = g < 0 ? g + n : (g >= n ? g - n : g);
+    return g < n ? g : n - 1;
+  };
+  auto stage = [&](const R *f) {
+    for (int l = t; l < FT::GN; l += BLK) {
+      const int tx = l % GX, ty = (l / GX) % GY, tz = l / (GX * GY);
+      const int gx = hidx(x0 - 5 + tx, nx, BND), gy = hidx(y0 - 5 + ty, ny, false),
+                gz = hidx(z0 - 5 + tz, nz, false);
+      tile[l] = f[((long)gz * ny + gy) * nx + gx];
+    }
+    __syncthreads();
+  };
+  auto base = [&](int p) { return (lz0 + p * FT::ZS + 5) * GX * GY + (ly + 5) * GX + (lx + 5); };
+  auto mix_x = [&](int B, int d2) {   // d2 = 1: (x, y); d2 = 2: (x, z)
+    const int s2 = d2 == 1 ? GX : GX * GY;
+    auto g = [&](int a, int b) { return tile[B + a + b * s2]; };
+    if (xedge)
+      return dmix_rows<R>(P, pk, 0, d2, [&](int q) { return mtab[BND ? row * 7 + q : 0]; }, g);
+    return dmix<false, R>(P, S0, 0, d2, [&](int k1, int k2) { return g(k1 - 8, k2 - 8); });
+  };
+  auto mix_yz = [&](int B) {
+    return dmix<false, R>(P, S0, 1, 2, [&](int k1, int k2) { return tile[B + (k1 - 8) * GX + (k2 - 8) * GX * GY]; });
+  };
+  R uxy[NPT], uxz[NPT], vxy[NPT], vyz[NPT], wxz[NPT], wyz[NPT];
+  stage(pr + P_U * fs);
+#pragma unroll
+  for (int p = 0; p < NPT; p++) {
+    uxy[p] = mix_x(base(p), 1);
+    uxz[p] = mix_x(base(p), 2);
+    pin(uxy[p]); pin(uxz[p]);
+  }
+  __syncthreads();
+  stage(pr + P_V * fs);
+#pragma unroll
+  for (int p = 0; p < NPT; p++) {
+    vxy[p] = mix_x(base(p), 1);
+    vyz[p] = mix_yz(base(p));
+    pin(vxy[p]); pin(vyz[p]);
+  }
+  __syncthreads();
+  stage(pr + P_W * fs);
+#pragma unroll
+  for (int p = 0; p < NPT; p++) {
+    wxz[p] = mix_x(base(p), 2);
+    wyz[p] = mix_yz(base(p));
+    pin(wxz[p]); pin(wyz[p]);
+  }
+#pragma unroll
+  for (int p = 0; p < NPT; p++) {
+    const int k = z0 + lz0 + p * FT::ZS;
+    if (i < nx && j < ny && k < nz)
+      final_point<BND, NS, R>(P, F, c, i, j, k, mode, lhs, rhsc, erc, out, uxy[p], vxy[p], uxz[p], wxz[p],
+                              vyz[p], wyz[p]);
   }
 }
 
@@ -1998,8 +2039,8 @@ void launch_nscbc_t(const Launch &L, hipStream_t st) {
 }
 template <int NS>
 void launch_final_t(const Launch &L, int irk, double *out, hipStream_t st) {
-  const dim3 g(L.nblk[L_POINT]), b(BLK);
-  const uint32_t *map = L.map[L_POINT];
+  const dim3 g(L.nblk[L_FINAL]), b(BLK);
+  const uint32_t *map = L.map[L_FINAL];
   if (L.precision) {
     if (L.bndx) hipLaunchKernelGGL((k_final<true, NS, float>), g, b, 0, st, L.mem, map, L.c, irk, out);
     else hipLaunchKernelGGL((k_final<false, NS, float>), g, b, 0, st, L.mem, map, L.c, irk, out);
@@ -2061,6 +2102,7 @@ void member_blocks(const Params &P, const int nw[3], int nblk[L_NKIND]) {
   auto tiles = [&](int tx, int ty, int tz) {
     return ((P.nx + tx - 1) / tx) * ((P.ny + ty - 1) / ty) * ((P.nz + tz - 1) / tz);
   };
+  nblk[L_FINAL] = tiles(FinalTile::FX, FinalTile::FY, FinalTile::FZ);
   nblk[L_PASSX] = nw[0] ? tiles(PassTile<0, 1>::TX, PassTile<0, 1>::TY, PassTile<0, 1>::TZ)
                         : tiles(PassTile<0, 0>::TX, PassTile<0, 0>::TY, PassTile<0, 0>::TZ);
   nblk[L_PASSY] = nw[1] ? tiles(PassTile<1, 1>::TX, PassTile<1, 1>::TY, PassTile<1, 1>::TZ)

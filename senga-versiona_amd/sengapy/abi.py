@@ -204,8 +204,8 @@ def load_library(path=None):
     lib.sg_get_timestep.argtypes = [vp, dp, dp, dp]
     lib.sg_footprint.argtypes = [P(Config), P(C.c_size_t)]
     i3 = C.c_int * 3
-    lib.sg_set_tiles.argtypes = [vp, i3, C.c_int]
-    lib.sg_get_tiles.argtypes = [vp, i3, P(C.c_int)]
+    lib.sg_set_tiles.argtypes = [vp, i3]
+    lib.sg_get_tiles.argtypes = [vp, i3]
     lib.sg_autotune.argtypes = [vp, C.c_int, dp]
     lib.sg_batch_set_tiles.argtypes = [vp, i3]
     lib.sg_batch_get_tiles.argtypes = [vp, i3]
@@ -346,24 +346,22 @@ class Solver:
         self._chk(self.lib.sg_memory(self.h, C.byref(b)))
         return b.value
 
-    def set_tiles(self, narrow=(0, 0, 0), ychunk=0):
-        """Pass tiles per direction (0: rows of 32, 1: rows of 16) and the
-        final kernel's y-chunk (0 keeps it); results do not change."""
-        self._chk(self.lib.sg_set_tiles(self.h, (C.c_int * 3)(*narrow), int(ychunk)))
+    def set_tiles(self, narrow=(0, 0, 0)):
+        """Pass tiles per direction (0: rows of 32, 1: rows of 16); results do
+        not change."""
+        self._chk(self.lib.sg_set_tiles(self.h, (C.c_int * 3)(*narrow)))
 
     def tiles(self):
         nw = (C.c_int * 3)()
-        yc = C.c_int(0)
-        self._chk(self.lib.sg_get_tiles(self.h, nw, C.byref(yc)))
-        return tuple(nw), yc.value
+        self._chk(self.lib.sg_get_tiles(self.h, nw))
+        return tuple(nw)
 
     def autotune(self, reps=5):
         """Time the tile choices on the current state and keep the fastest;
-        returns (narrow, ychunk, ms per substage of the tuned kernels)."""
+        returns (narrow, ms per substage of the tuned passes)."""
         ms = C.c_double(0)
         self._chk(self.lib.sg_autotune(self.h, int(reps), C.byref(ms)))
-        nw, yc = self.tiles()
-        return nw, yc, ms.value
+        return self.tiles(), ms.value
 
 
 class BatchError(SgError):
