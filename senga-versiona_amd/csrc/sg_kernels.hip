@@ -1905,14 +1905,34 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
     g = g < 0 ? g + n : (g >= n ? g - n : g);
     return g < n ? g : n - 1;
   };
-  auto stage = [&](const R *f) {
-    for (int l = t; l < FT::GN; l += BLK) {
-      const int tx = l % GX, ty = (l / GX) % GY, tz = l / (GX * GY);
-      const int gx = hidx(x0 - 5 + tx, nx, BND), gy = hidx(y0 - 5 + ty, ny, false),
-                gz = hidx(z0 - 5 + tz, nz, false);
-      tile[l] = f[((long)gz * ny + gy) * nx + gx];
+  // staging: the tile is GX*GY columns of GZ values; thread t fetches
+  // columns t and t + BLK, all loads in flight at once, into registers and
+  // then stores them to LDS.  The next field's fetch is issued before the
+  // current field's derivatives are evaluated.  Column offsets are per
+  // thread, plane offsets are uniform.
+  constexpr int NCOL = GX * GY, GZ = FT::GZ;
+  static_assert(NCOL > BLK && NCOL <= 2 * BLK, "two columns per thread");
+  const bool col2 = t + BLK < NCOL;
+  auto coloff = [&](int cc) {
+    const int tx = cc % GX, ty = cc / GX;
+    return (uint32_t)(hidx(y0 - 5 + ty, ny, false) * nx + hidx(x0 - 5 + tx, nx, BND));
+  };
+  const uint32_t co0 = coloff(t), co1 = coloff(col2 ? t + BLK : t);
+  R stg[2][GZ];
+  auto fetch = [&](const R *f) {
+#pragma unroll
+    for (int z = 0; z < GZ; z++) {
+      const uint32_t zo = (uint32_t)hidx(z0 - 5 + z, nz, false) * (uint32_t)(nx * ny);
+      stg[0][z] = fld(f, (zo + co0) * (uint32_t)sizeof(R));
+      if (col2) stg[1][z] = fld(f, (zo + co1) * (uint32_t)sizeof(R));
     }
-    __syncthreads();
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int z = 0; z < GZ; z++) {
+      tile[z * NCOL + t] = stg[0][z];
+      if (col2) tile[z * NCOL + t + BLK] = stg[1][z];
+    }
   };
   auto base = [&](int p) { return (lz0 + p * FT::ZS + 5) * GX * GY + (ly + 5) * GX + (lx + 5); };
   auto mix_x = [&](int B, int d2) {   // d2 = 1: (x, y); d2 = 2: (x, z)
@@ -1926,7 +1946,10 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
     return dmix<false, R>(P, S0, 1, 2, [&](int k1, int k2) { return tile[B + (k1 - 8) * GX + (k2 - 8) * GX * GY]; });
   };
   R uxy[NPT], uxz[NPT], vxy[NPT], vyz[NPT], wxz[NPT], wyz[NPT];
-  stage(pr + P_U * fs);
+  fetch(pr + P_U * fs);
+  store();
+  __syncthreads();
+  fetch(pr + P_V * fs);
 #pragma unroll
   for (int p = 0; p < NPT; p++) {
     uxy[p] = mix_x(base(p), 1);
@@ -1934,7 +1957,9 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
     pin(uxy[p]); pin(uxz[p]);
   }
   __syncthreads();
-  stage(pr + P_V * fs);
+  store();
+  __syncthreads();
+  fetch(pr + P_W * fs);
 #pragma unroll
   for (int p = 0; p < NPT; p++) {
     vxy[p] = mix_x(base(p), 1);
@@ -1942,7 +1967,8 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
     pin(vxy[p]); pin(vyz[p]);
   }
   __syncthreads();
-  stage(pr + P_W * fs);
+  store();
+  __syncthreads();
 #pragma unroll
   for (int p = 0; p < NPT; p++) {
     wxz[p] = mix_x(base(p), 2);
