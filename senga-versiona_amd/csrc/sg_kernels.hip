@@ -1938,8 +1938,18 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
   auto mix_x = [&](int B, int d2) {   // d2 = 1: (x, y); d2 = 2: (x, z)
     const int s2 = d2 == 1 ? GX : GX * GY;
     auto g = [&](int a, int b) { return tile[B + a + b * s2]; };
-    if (xedge)
-      return dmix_rows<R>(P, pk, 0, d2, [&](int q) { return mtab[BND ? row * 7 + q : 0]; }, g);
+    if (xedge) {
+      // fresh copies of the row offsets for every call, so that the
+      // neighbour addresses are formed per call instead of being kept live
+      // for the whole kernel (they would set its occupancy)
+      int pq[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        pq[q] = pk[q];
+        asm volatile("" : "+v"(pq[q]));
+      }
+      return dmix_rows<R>(P, pq, 0, d2, [&](int q) { return mtab[BND ? row * 7 + q : 0]; }, g);
+    }
     return dmix<false, R>(P, S0, 0, d2, [&](int k1, int k2) { return g(k1 - 8, k2 - 8); });
   };
   auto mix_yz = [&](int B) {
