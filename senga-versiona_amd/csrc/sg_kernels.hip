@@ -713,6 +713,13 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   }
 }
 
+// FP32 quotient and reciprocal from the hardware reciprocal (v_rcp_f32, about
+// 1 ulp; the quotient within 2 ulp) instead of the IEEE-exact FP32 division
+// sequence (~10 instructions each), for the FP32 transport and chemistry of
+// the mixed-precision point kernel.  Operands there are normal numbers.
+__device__ __forceinline__ float rf(float y) { return __builtin_amdgcn_rcpf(y); }
+__device__ __forceinline__ float qf(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+
 // Mixed-precision point physics: the temperature Newton iteration, the
 // primitive variables and the enthalpies are formed in FP64 as in k_prim and
 // stored in FP32; transport and chemistry run in FP32 with the hardware
@@ -830,7 +837,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   for (int s = 0; s < NS; s++) ryf[s] = (float)ry[s];
   const float df = (float)d;
   if (P.mixavg) {
-    transp = __logf(tf / (float)P.tdifgb);
+    transp = __logf(tf * (float)(1.0 / P.tdifgb));
     float c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -842,13 +849,13 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       float ctr = __expf(f);
       float fornow = ryf[s] * tb->ovwf[s];
       c1 = c1 + fornow * ctr;
-      c2 = c2 + __fdividef(fornow, ctr);
+      c2 = c2 + qf(fornow, ctr);
       c3 = c3 + fornow;
     }
-    c3 = __frcp_rn(c3);
+    c3 = rf(c3);
     c1 = c1 * c3;
     c2 = c2 * c3;
-    lam = 0.5f * (c1 + __frcp_rn(c2));
+    lam = 0.5f * (c1 + rf(c2));
     wmomix = df * c3;
   }
   fld(pr + P_LAM * fs, b4) = lam;
@@ -890,8 +897,8 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       if (ml != 0) {
         const double *rl = c->rclind[ml - 1];
         float s4 = __expf((float)(rl[0] + rl[1] * lt - rl[2] * ovt));
-        float preduc = s3 * s4 / s1;
-        float fornow = (float)rl[3] * preduc / (1.0f + preduc);
+        float preduc = qf(s3 * s4, s1);
+        float fornow = qf((float)rl[3] * preduc, 1.0f + preduc);
         s1 = s1 * fornow;
         s2d = (double)__logf(s1);
         flthrd = false;
@@ -901,18 +908,18 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
         const double *rt = c->rctroe[mt - 1];
         float omalph = 1.0f - (float)rt[3];
         float s4 = __expf((float)(rt[0] + rt[1] * lt - rt[2] * ovt));
-        float preduc = s3 * s4 / s1;
+        float preduc = qf(s3 * s4, s1);
         float trats1 = tf * (float)rt[4], trats2 = (float)(-rt[5] * ovt), trats3 = tf * (float)rt[6];
         float ftcent = omalph * __expf(trats3) + (float)rt[3] * __expf(trats1) + __expf(trats2);
         ftcent = __log10f(ftcent);
         float cfactr = (float)rt[7] + (float)rt[8] * ftcent;
         float enfact = (float)rt[9] + (float)rt[10] * ftcent;
         float fornow = __log10f(preduc) + cfactr;
-        fornow = fornow / (enfact - (float)rt[11] * fornow);
+        fornow = qf(fornow, enfact - (float)rt[11] * fornow);
         fornow = 1.0f + fornow * fornow;
-        float fbroad = ftcent / fornow;
+        float fbroad = qf(ftcent, fornow);
         fbroad = __expf(fbroad * (float)c->clnten);
-        fornow = fbroad * preduc / (1.0f + preduc);
+        fornow = qf(fbroad * preduc, 1.0f + preduc);
         s1 = s1 * fornow;
         s2d = (double)__logf(s1);
         flthrd = false;
@@ -921,15 +928,15 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       if (ms != 0) {
         const double *rs = c->rcsrif[ms - 1];
         float s4 = __expf((float)(rs[0] + rs[1] * lt - rs[2] * ovt));
-        float preduc = s3 * s4 / s1;
+        float preduc = qf(s3 * s4, s1);
         float fornow = (float)(rs[4] * ovt);
         float fbroad = (float)rs[3] * __expf(fornow);
         fornow = tf * (float)rs[5];
         fbroad = fbroad + __expf(fornow);
         fornow = __log10f(preduc);
-        fornow = 1.0f / (1.0f + __log10f(fornow));
+        fornow = rf(1.0f + __log10f(fornow));
         fbroad = (float)rs[6] * __expf(fornow * __logf(fbroad));
-        fornow = fbroad * preduc / (1.0f + preduc);
+        fornow = qf(fbroad * preduc, 1.0f + preduc);
         s1 = s1 * fornow;
         s2d = (double)__logf(s1);
         flthrd = false;
@@ -943,7 +950,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
         float fornow = cyv(c->nrcpec[st][m]);
         fornow = fornow > ysmall ? fornow : ysmall;
         fornow = __expf(tb->crspecf[st][m] * __logf(fornow));
-        fornow = fornow / (1.0f + ydenom * fornow);
+        fornow = qf(fornow, 1.0f + ydenom * fornow);
         s1 = s1 * fornow;
       }
       if (c->mglist[st] != 0) {
@@ -958,7 +965,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
           float fornow = cyv(c->npcpec[st][m]);
           fornow = fornow > ysmall ? fornow : ysmall;
           fornow = __expf(tb->cpspecf[st][m] * __logf(fornow));
-          fornow = fornow / (1.0f + ydenom * fornow);
+          fornow = qf(fornow, 1.0f + ydenom * fornow);
           s2 = s2 * fornow;
         }
         s1 = s1 - s2;
@@ -1039,7 +1046,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       for (int s = 0; s < NS; s++) {
         float c1 = c1t - yt[s];
         float c2 = (cs[s] - self[s]) * wmomix;
-        fld(pr + prim_rd(P, s), b4) = __fdividef(df * c1, c2);
+        fld(pr + prim_rd(P, s), b4) = qf(df * c1, c2);
       }
     } else {
 #pragma unroll
@@ -1053,7 +1060,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
         }
         float c1 = c1t - yt[s];
         c2 = (c2 - self) * wmomix;
-        fld(pr + prim_rd(P, s), b4) = __fdividef(df * c1, c2);
+        fld(pr + prim_rd(P, s), b4) = qf(df * c1, c2);
       }
     }
     SG_PHASE();
@@ -1062,8 +1069,8 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       mus[s] = __expf(fit(tb->viscf[s]));
-      sq[s] = sqrtf(mus[s]);
-      rsq[s] = __frcp_rn(sq[s]);
+      sq[s] = __builtin_amdgcn_sqrtf(mus[s]);
+      rsq[s] = rf(sq[s]);
       yw[s] = y[s] * tb->ovwf[s];
     }
     float c1 = 0.0f;
@@ -1077,7 +1084,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
         fornow = tb->wilk1f[s][js] * fornow * fornow;
         c2 = c2 + yw[js] * fornow;
       }
-      c1 = c1 + yw[s] * __fdividef(mus[s], c2);
+      c1 = c1 + yw[s] * qf(mus[s], c2);
     }
     fld(pr + P_MU * fs, b4) = c1;
   }
