@@ -222,6 +222,15 @@ int sg_profile_read(sg_handle *h, int idx, char *name, size_t n, double *total_m
 /* Device bytes held by the handle. */
 int sg_memory(sg_handle *h, size_t *bytes);
 
+/* *compiled = 1 when the configuration's chemistry is, bit for bit, the
+ * mechanism compiled into the library (the 9-species, 21-step H2-air
+ * mechanism of the reference's input/chem.dat): the point kernels then run
+ * the CHRATE step loop (src/chrate.f:146-760) unrolled over its constants
+ * instead of reading the step tables at run time.  Both evaluate the same
+ * arithmetic; any other mechanism takes the table-driven loop.  No
+ * counterpart in the reference, which reads chem.dat at run time. */
+int sg_mechanism_compiled(const sg_handle *h, int *compiled);
+
 /* ---- Tiles and their autotuner (no counterpart in the reference) ---------
  * narrow[d] selects the tile of the direction-d derivative pass: 0 = rows of
  * 32 points (default), 1 = rows of 16 points and twice the rows (no idle

@@ -39,6 +39,11 @@ def _stale(obj, deps):
 
 
 def build_library(jobs=8, force=False, species=None):
+    # the compiled mechanism header follows data/h2_9sp/chem.dat
+    sys.path.insert(0, PKG)
+    import mechgen
+    if mechgen.main(["--check"]) != 0:
+        mechgen.main([])
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]

@@ -14,6 +14,7 @@
 
 #include "sg_internal.h"
 #include "sg_stencil.h"
+#include "sg_mech_h2.h"
 
 using namespace sg;
 
@@ -44,6 +45,7 @@ struct sg_handle {
   void *ebase;           // error register allocation (adaptive only)
   size_t ebytes;
   int have_state;
+  int mechfix;           // chemistry is the compiled mechanism (mech_compiled)
   int err_code;
   int err_ijk[3];
   std::string err_msg;
@@ -271,6 +273,26 @@ static int setup_params(sg_handle *h) {
 
 // One-member launch set of a handle: its member record and all-zero block
 // maps (every block belongs to member 0, blk0 = 0).
+// Whether a configuration's chemistry is, bit for bit, the mechanism compiled
+// into the library (MechH2, the 9-species H2-air mechanism): every chemistry
+// member of sg_config as sg_create holds it, and the dense rate table derived
+// from it.  Then the point kernels run the step loop unrolled over its
+// constants, which evaluates the same expressions on the same values.
+static int mech_compiled(const sg_config &c, const Tables &t) {
+  using T = MechH2;
+  if (c.nspec != T::nspec || c.nstep != T::nstep || t.ratedense != T::ratedense) return 0;
+#define SAME(f)                                                                      \
+  static_assert(sizeof(((const sg_config *)nullptr)->f) == sizeof(T::f), #f " size"); \
+  if (memcmp(&c.f, &T::f, sizeof T::f) != 0) return 0;
+  SAME(ovwmol) SAME(rparam) SAME(nsslen) SAME(nrslen) SAME(npslen) SAME(nrclen) SAME(npclen) SAME(mblist)
+  SAME(mglist) SAME(mllist) SAME(mtlist) SAME(mslist) SAME(nsspec) SAME(nrspec) SAME(npspec) SAME(nrcpec)
+  SAME(npcpec) SAME(diffmu) SAME(diffmw) SAME(crspec) SAME(cpspec) SAME(effy3b) SAME(rclind) SAME(rctroe)
+  SAME(rcsrif) SAME(clnten)
+#undef SAME
+  static_assert(sizeof t.nuw == sizeof T::nuw, "nuw size");
+  return memcmp(&t.nuw, &T::nuw, sizeof T::nuw) == 0;
+}
+
 static int init_launch(sg_handle *h, const int nw[3]) {
   if (h->lbase) {
     hipStreamSynchronize(h->stream);
@@ -307,6 +329,7 @@ static int init_launch(sg_handle *h, const int nw[3]) {
   L.precision = h->P.precision;
   L.bndx = h->P.bnd[0];
   L.inflow = h->P.inflow_xl;
+  L.mechfix = h->mechfix;
   return 0;
 }
 
@@ -423,6 +446,7 @@ static int batch_prepare(sg_batch *b) {
     b->L.ns = h0->P.ns;
     b->L.precision = h0->P.precision;
     b->L.bndx = h0->P.bnd[0];
+    b->L.mechfix = h0->mechfix;   // members share one mechanism
     b->L.inflow = 0;
     for (int m = 0; m < nm; m++) b->L.inflow |= b->members[m]->P.inflow_xl;
     if (b->exec) hipGraphExecDestroy(b->exec);
@@ -537,6 +561,7 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
     const int tr[6] = {3, 7, 8, 9, 10, 11};
     for (int q = 0; q < 6; q++) h->cfg.rctroe[j][tr[q]] = (double)(float)h->cfg.rctroe[j][tr[q]];
   }
+  h->mechfix = mech_compiled(h->cfg, h->tab);
   int rc = setup_params(h);
   if (!rc && h->P.fstride * (long)sizeof(double) >= (1L << 32)) rc = SG_EINVAL;  // 32-bit field offsets
   if (rc) { delete h; return rc; }
@@ -1122,6 +1147,12 @@ int sg_set_tiles(sg_handle *h, const int narrow[3]) {
   if (h->batch) return set_err(h, SG_ESTATE, "handle advances in a batch");
   if (init_launch(h, narrow) != 0) return set_err(h, SG_ENOMEM, "launch table allocation failed");
   return refresh_member(h);
+}
+
+int sg_mechanism_compiled(const sg_handle *h, int *compiled) {
+  if (!h || !compiled) return SG_EINVAL;
+  *compiled = h->mechfix;
+  return 0;
 }
 
 int sg_get_tiles(sg_handle *h, int narrow[3]) {

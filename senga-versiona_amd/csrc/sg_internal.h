@@ -124,6 +124,7 @@ struct Launch {
   const Tables *tab;
   int ns, precision, bndx, inflow;
   int pass_nw[3];   // pass tile per direction: 0 wide (rows of 32), 1 narrow (rows of 16)
+  int mechfix;      // the chemistry is the compiled mechanism (MechH2): unrolled step loop
 };
 
 // member block counts of each kind (sg_kernels.hip)

@@ -14,8 +14,16 @@
 
 #include "sg_internal.h"
 #include "sg_stencil.h"
+#include "sg_mech_h2.h"
 
 namespace sg {
+
+// Point kernels and their helpers: a multiply and an add are fused only
+// within one expression, so the fusing does not depend on how the compiler
+// lays out the surrounding code (the compiled-mechanism and table-driven
+// step loops then give identical results).  The stencil kernels below
+// switch back to fusing across statements.
+#pragma clang fp contract(on)
 
 #define BLK 256
 
@@ -335,7 +343,221 @@ __device__ __forceinline__ double trfit(const double *a, double x) {
   return f;
 }
 
-template <int NS>
+
+// ---------------------------------------------------------------------------
+// CHRATE step loop (src/chrate.f:146-760), shared by the two sources of
+// mechanism constants: RtMech reads the runtime configuration and derived
+// tables; CtMech<T> reads compile-time tables (a mechanism compiled into the
+// library, e.g. MechH2), for which the loop is unrolled over the steps, every
+// list length, species index and rate constant becomes an immediate, and the
+// per-point concentrations and Gibbs functions stay in registers.  Both run
+// the same expressions in the same order on the same values, so they give
+// the same results.
+struct RtMech {
+  const sg_config *c;
+  const Tables *tb;
+  static constexpr bool fixed = false;
+  __device__ int nst() const { return c->nstep; }
+  __device__ int mb(int st) const { return c->mblist[st]; }
+  __device__ int mg(int st) const { return c->mglist[st]; }
+  __device__ int ml(int st) const { return c->mllist[st]; }
+  __device__ int mt(int st) const { return c->mtlist[st]; }
+  __device__ int ms(int st) const { return c->mslist[st]; }
+  __device__ double rp(int st, int q) const { return c->rparam[st][q]; }
+  __device__ double rl(int j, int q) const { return c->rclind[j][q]; }
+  __device__ double rt(int j, int q) const { return c->rctroe[j][q]; }
+  __device__ double rs(int j, int q) const { return c->rcsrif[j][q]; }
+  __device__ int nrs(int st) const { return c->nrslen[st]; }
+  __device__ int nrsp(int st, int m) const { return c->nrspec[st][m]; }
+  __device__ int nrc(int st) const { return c->nrclen[st]; }
+  __device__ int nrcp(int st, int m) const { return c->nrcpec[st][m]; }
+  __device__ double crs(int st, int m) const { return c->crspec[st][m]; }
+  __device__ float crsf(int st, int m) const { return tb->crspecf[st][m]; }
+  __device__ int nss(int st) const { return c->nsslen[st]; }
+  __device__ int nssp(int st, int m) const { return c->nsspec[st][m]; }
+  __device__ double dmu(int st, int m) const { return c->diffmu[st][m]; }
+  __device__ double dmw(int st, int m) const { return c->diffmw[st][m]; }
+  __device__ float dmwf(int st, int m) const { return tb->diffmwf[st][m]; }
+  __device__ int nps(int st) const { return c->npslen[st]; }
+  __device__ int npsp(int st, int m) const { return c->npspec[st][m]; }
+  __device__ int npc(int st) const { return c->npclen[st]; }
+  __device__ int npcp(int st, int m) const { return c->npcpec[st][m]; }
+  __device__ double cps(int st, int m) const { return c->cpspec[st][m]; }
+  __device__ float cpsf(int st, int m) const { return tb->cpspecf[st][m]; }
+  __device__ double ovw(int s) const { return c->ovwmol[s]; }
+  __device__ float ovwf(int s) const { return tb->ovwf[s]; }
+  __device__ double eff(int b, int s) const { return c->effy3b[b][s]; }
+  __device__ float efff(int b, int s) const { return tb->effyf[b][s]; }
+  __device__ double clnten() const { return c->clnten; }
+  __device__ bool dense() const { return tb->ratedense != 0; }
+  __device__ double nuw(int st, int u) const { return tb->nuw[st][u]; }
+  __device__ float nuwf(int st, int u) const { return tb->nuwf[st][u]; }
+};
+
+template <class T>
+struct CtMech {
+  static constexpr bool fixed = true;
+  __device__ static constexpr int nst() { return T::nstep; }
+  __device__ static constexpr int mb(int st) { return T::mblist[st]; }
+  __device__ static constexpr int mg(int st) { return T::mglist[st]; }
+  __device__ static constexpr int ml(int st) { return T::mllist[st]; }
+  __device__ static constexpr int mt(int st) { return T::mtlist[st]; }
+  __device__ static constexpr int ms(int st) { return T::mslist[st]; }
+  __device__ static constexpr double rp(int st, int q) { return T::rparam[st][q]; }
+  __device__ static constexpr double rl(int j, int q) { return T::rclind[j][q]; }
+  __device__ static constexpr double rt(int j, int q) { return T::rctroe[j][q]; }
+  __device__ static constexpr double rs(int j, int q) { return T::rcsrif[j][q]; }
+  __device__ static constexpr int nrs(int st) { return T::nrslen[st]; }
+  __device__ static constexpr int nrsp(int st, int m) { return T::nrspec[st][m]; }
+  __device__ static constexpr int nrc(int st) { return T::nrclen[st]; }
+  __device__ static constexpr int nrcp(int st, int m) { return T::nrcpec[st][m]; }
+  __device__ static constexpr double crs(int st, int m) { return T::crspec[st][m]; }
+  __device__ static constexpr float crsf(int st, int m) { return (float)T::crspec[st][m]; }
+  __device__ static constexpr int nss(int st) { return T::nsslen[st]; }
+  __device__ static constexpr int nssp(int st, int m) { return T::nsspec[st][m]; }
+  __device__ static constexpr double dmu(int st, int m) { return T::diffmu[st][m]; }
+  __device__ static constexpr double dmw(int st, int m) { return T::diffmw[st][m]; }
+  __device__ static constexpr float dmwf(int st, int m) { return (float)T::diffmw[st][m]; }
+  __device__ static constexpr int nps(int st) { return T::npslen[st]; }
+  __device__ static constexpr int npsp(int st, int m) { return T::npspec[st][m]; }
+  __device__ static constexpr int npc(int st) { return T::npclen[st]; }
+  __device__ static constexpr int npcp(int st, int m) { return T::npcpec[st][m]; }
+  __device__ static constexpr double cps(int st, int m) { return T::cpspec[st][m]; }
+  __device__ static constexpr float cpsf(int st, int m) { return (float)T::cpspec[st][m]; }
+  __device__ static constexpr double ovw(int s) { return T::ovwmol[s]; }
+  __device__ static constexpr float ovwf(int s) { return (float)T::ovwmol[s]; }
+  __device__ static constexpr double eff(int b, int s) { return T::effy3b[b][s]; }
+  __device__ static constexpr float efff(int b, int s) { return (float)T::effy3b[b][s]; }
+  __device__ static constexpr double clnten() { return T::clnten; }
+  __device__ static constexpr bool dense() { return T::ratedense != 0; }
+  __device__ static constexpr double nuw(int st, int u) { return T::nuw[st][u]; }
+  __device__ static constexpr float nuwf(int st, int u) { return (float)T::nuw[st][u]; }
+};
+
+// f(st) for every step: unrolled for compile-time tables
+template <class M, class Fn>
+__device__ __forceinline__ void for_steps(const M &m, Fn f) {
+  if constexpr (M::fixed) {
+#pragma unroll
+    for (int st = 0; st < M::nst(); st++) f(st);
+  } else {
+    for (int st = 0; st < m.nst(); st++) f(st);
+  }
+}
+
+// One step in FP64: rate coefficients with falloff, forward and reverse
+// products, added to rate[] (src/chrate.f:177-747).  cyv(s) and gbv(s) give
+// the point's molar concentration and Gibbs function of species s.
+template <int NS, class M, class CY, class GB>
+__device__ __forceinline__ void chem_step64(const M &m, int st, double t, double lt, double ovt,
+                                            const double *ry, CY cyv, GB gbv, double *rate) {
+  const double ysmall = 1.0e-30, ydenom = 1.0e-15;
+  bool flthrd = true;
+  double s3 = 0.0;
+  const int mb = m.mb(st);
+  if (mb != 0) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) s3 = s3 + ry[s] * (m.ovw(s) * m.eff(mb - 1, s));
+  }
+  double s2 = m.rp(st, 0) + m.rp(st, 1) * lt - m.rp(st, 2) * ovt;
+  double s1 = exp(s2);
+  const int ml = m.ml(st);
+  if (ml != 0) {
+    const int j = ml - 1;
+    double s4 = m.rl(j, 0) + m.rl(j, 1) * lt - m.rl(j, 2) * ovt;
+    s4 = exp(s4);
+    double preduc = s3 * s4 / s1;
+    double fornow = m.rl(j, 3) * preduc / (1.0 + preduc);  // rl[3] pre-rounded to REAL
+    s1 = s1 * fornow;
+    s2 = log(s1);
+    flthrd = false;
+  }
+  const int mt = m.mt(st);
+  if (mt != 0) {
+    const int j = mt - 1;
+    double omalph = 1.0 - m.rt(j, 3);
+    double s4 = m.rt(j, 0) + m.rt(j, 1) * lt - m.rt(j, 2) * ovt;
+    s4 = exp(s4);
+    double preduc = s3 * s4 / s1;
+    double trats1 = t * m.rt(j, 4), trats2 = -m.rt(j, 5) * ovt, trats3 = t * m.rt(j, 6);
+    double ftcent = omalph * exp(trats3) + m.rt(j, 3) * exp(trats1) + exp(trats2);
+    ftcent = log10(ftcent);
+    double cfactr = m.rt(j, 7) + m.rt(j, 8) * ftcent;
+    double enfact = m.rt(j, 9) + m.rt(j, 10) * ftcent;
+    double fornow = log10(preduc) + cfactr;
+    fornow = fornow / (enfact - m.rt(j, 11) * fornow);
+    fornow = 1.0 + fornow * fornow;
+    double fbroad = ftcent / fornow;
+    fbroad = exp(fbroad * m.clnten());
+    fornow = fbroad * preduc / (1.0 + preduc);
+    s1 = s1 * fornow;
+    s2 = log(s1);
+    flthrd = false;
+  }
+  const int ms = m.ms(st);
+  if (ms != 0) {
+    const int j = ms - 1;
+    double s4 = m.rs(j, 0) + m.rs(j, 1) * lt - m.rs(j, 2) * ovt;
+    s4 = exp(s4);
+    double preduc = s3 * s4 / s1;
+    double fornow = m.rs(j, 4) * ovt;
+    double fbroad = m.rs(j, 3) * exp(fornow);
+    fornow = t * m.rs(j, 5);
+    fbroad = fbroad + exp(fornow);
+    fornow = log10(preduc);
+    fornow = 1.0 / (1.0 + log10(fornow));
+    fbroad = m.rs(j, 6) * exp(fornow * log(fbroad));
+    fornow = fbroad * preduc / (1.0 + preduc);
+    s1 = s1 * fornow;
+    s2 = log(s1);
+    flthrd = false;
+  }
+  for (int q = 0; q < m.nrs(st); q++) {
+    double fornow = cyv(m.nrsp(st, q));
+    fornow = fornow > 0.0 ? fornow : 0.0;
+    s1 = s1 * fornow;
+  }
+  for (int q = 0; q < m.nrc(st); q++) {
+    double fornow = cyv(m.nrcp(st, q));
+    fornow = fornow > ysmall ? fornow : ysmall;
+    fornow = exp(m.crs(st, q) * log(fornow));
+    fornow = fornow / (1.0 + ydenom * fornow);
+    s1 = s1 * fornow;
+  }
+  if (m.mg(st) != 0) {
+    for (int q = 0; q < m.nss(st); q++) s2 = s2 + m.dmu(st, q) * gbv(m.nssp(st, q));
+    s2 = exp(s2);
+    for (int q = 0; q < m.nps(st); q++) {
+      double fornow = cyv(m.npsp(st, q));
+      fornow = fornow > 0.0 ? fornow : 0.0;
+      s2 = s2 * fornow;
+    }
+    for (int q = 0; q < m.npc(st); q++) {
+      double fornow = cyv(m.npcp(st, q));
+      fornow = fornow > ysmall ? fornow : ysmall;
+      fornow = exp(m.cps(st, q) * log(fornow));
+      fornow = fornow / (1.0 + ydenom * fornow);
+      s2 = s2 * fornow;
+    }
+    s1 = s1 - s2;
+  }
+  if (mb != 0 && flthrd) s1 = s1 * s3;
+  if (m.dense()) {
+    // every species once per list: absent species add s1*0, an exact zero
+#pragma unroll
+    for (int u = 0; u < NS; u++) rate[u] = rate[u] + s1 * m.nuw(st, u);
+  } else {
+    for (int q = 0; q < m.nss(st); q++) {
+      int sp = m.nssp(st, q);
+      double add = s1 * m.dmw(st, q);
+#pragma unroll
+      for (int u = 0; u < NS; u++)
+        if (u == sp) rate[u] = rate[u] + add;
+    }
+  }
+}
+
+template <int NS, bool FIX>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_prim(const Member *__restrict__ mem,
     const uint32_t *__restrict__ map, const sg_config *__restrict__ c, const Tables *__restrict__ tb) {
   int lb;
@@ -460,7 +682,6 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   SG_PHASE();
   // --- chemistry (src/chrate.f:146-760) on rho Y ---
   {
-    const double ysmall = 1.0e-30, ydenom = 1.0e-15;
     double rate[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) rate[s] = 0.0;
@@ -473,122 +694,31 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       int it = itn[s];
       gibbs[s] = gibbs_of(tb, s, it, t, ovt, lt);
     }
-    // Molar concentrations and Gibbs functions of this point in LDS, so that
-    // a step's species lists (uniform, known only at run time) index them
-    // with one LDS read instead of a select over all species.
-    __shared__ double lcy[NS][BLK], lgb[NS][BLK];
-    const int tl = threadIdx.x;
+    if constexpr (FIX) {
+      using MT = CtMech<MechH2>;
+      double cy[NS];
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-      lcy[s][tl] = ry[s] * c->ovwmol[s];
-      lgb[s][tl] = gibbs[s];
-    }
-    auto cyv = [&](int s) { return lcy[s][tl]; };
-    auto gbv = [&](int s) { return lgb[s][tl]; };
-    for (int st = 0; st < c->nstep; st++) {
-      bool flthrd = true;
-      double s3 = 0.0;
-      int mb = c->mblist[st];
-      if (mb != 0) {
+      for (int s = 0; s < NS; s++) cy[s] = ry[s] * MT::ovw(s);
+      for_steps(MT{}, [&](int st) {
+        chem_step64<NS>(MT{}, st, t, lt, ovt, ry, [&](int s) { return cy[s]; }, [&](int s) { return gibbs[s]; },
+                        rate);
+      });
+    } else {
+      // Molar concentrations and Gibbs functions of this point in LDS, so
+      // that a step's species lists (uniform, known only at run time) index
+      // them with one LDS read instead of a select over all species.
+      __shared__ double lcy[NS][BLK], lgb[NS][BLK];
+      const int tl = threadIdx.x;
 #pragma unroll
-        for (int s = 0; s < NS; s++) s3 = s3 + ry[s] * (c->ovwmol[s] * c->effy3b[mb - 1][s]);
+      for (int s = 0; s < NS; s++) {
+        lcy[s][tl] = ry[s] * c->ovwmol[s];
+        lgb[s][tl] = gibbs[s];
       }
-      double s2 = c->rparam[st][0] + c->rparam[st][1] * lt - c->rparam[st][2] * ovt;
-      double s1 = exp(s2);
-      int ml = c->mllist[st];
-      if (ml != 0) {
-        const double *rl = c->rclind[ml - 1];
-        double s4 = rl[0] + rl[1] * lt - rl[2] * ovt;
-        s4 = exp(s4);
-        double preduc = s3 * s4 / s1;
-        double fornow = rl[3] * preduc / (1.0 + preduc);  // rl[3] pre-rounded to REAL
-        s1 = s1 * fornow;
-        s2 = log(s1);
-        flthrd = false;
-      }
-      int mt = c->mtlist[st];
-      if (mt != 0) {
-        const double *rt = c->rctroe[mt - 1];
-        double omalph = 1.0 - rt[3];
-        double s4 = rt[0] + rt[1] * lt - rt[2] * ovt;
-        s4 = exp(s4);
-        double preduc = s3 * s4 / s1;
-        double trats1 = t * rt[4], trats2 = -rt[5] * ovt, trats3 = t * rt[6];
-        double ftcent = omalph * exp(trats3) + rt[3] * exp(trats1) + exp(trats2);
-        ftcent = log10(ftcent);
-        double cfactr = rt[7] + rt[8] * ftcent;
-        double enfact = rt[9] + rt[10] * ftcent;
-        double fornow = log10(preduc) + cfactr;
-        fornow = fornow / (enfact - rt[11] * fornow);
-        fornow = 1.0 + fornow * fornow;
-        double fbroad = ftcent / fornow;
-        fbroad = exp(fbroad * c->clnten);
-        fornow = fbroad * preduc / (1.0 + preduc);
-        s1 = s1 * fornow;
-        s2 = log(s1);
-        flthrd = false;
-      }
-      int ms = c->mslist[st];
-      if (ms != 0) {
-        const double *rs = c->rcsrif[ms - 1];
-        double s4 = rs[0] + rs[1] * lt - rs[2] * ovt;
-        s4 = exp(s4);
-        double preduc = s3 * s4 / s1;
-        double fornow = rs[4] * ovt;
-        double fbroad = rs[3] * exp(fornow);
-        fornow = t * rs[5];
-        fbroad = fbroad + exp(fornow);
-        fornow = log10(preduc);
-        fornow = 1.0 / (1.0 + log10(fornow));
-        fbroad = rs[6] * exp(fornow * log(fbroad));
-        fornow = fbroad * preduc / (1.0 + preduc);
-        s1 = s1 * fornow;
-        s2 = log(s1);
-        flthrd = false;
-      }
-      for (int m = 0; m < c->nrslen[st]; m++) {
-        double fornow = cyv(c->nrspec[st][m]);
-        fornow = fornow > 0.0 ? fornow : 0.0;
-        s1 = s1 * fornow;
-      }
-      for (int m = 0; m < c->nrclen[st]; m++) {
-        double fornow = cyv(c->nrcpec[st][m]);
-        fornow = fornow > ysmall ? fornow : ysmall;
-        fornow = exp(c->crspec[st][m] * log(fornow));
-        fornow = fornow / (1.0 + ydenom * fornow);
-        s1 = s1 * fornow;
-      }
-      if (c->mglist[st] != 0) {
-        for (int m = 0; m < c->nsslen[st]; m++) s2 = s2 + c->diffmu[st][m] * gbv(c->nsspec[st][m]);
-        s2 = exp(s2);
-        for (int m = 0; m < c->npslen[st]; m++) {
-          double fornow = cyv(c->npspec[st][m]);
-          fornow = fornow > 0.0 ? fornow : 0.0;
-          s2 = s2 * fornow;
-        }
-        for (int m = 0; m < c->npclen[st]; m++) {
-          double fornow = cyv(c->npcpec[st][m]);
-          fornow = fornow > ysmall ? fornow : ysmall;
-          fornow = exp(c->cpspec[st][m] * log(fornow));
-          fornow = fornow / (1.0 + ydenom * fornow);
-          s2 = s2 * fornow;
-        }
-        s1 = s1 - s2;
-      }
-      if (mb != 0 && flthrd) s1 = s1 * s3;
-      if (tb->ratedense) {
-        // every species once per list: absent species add s1*0, an exact zero
-#pragma unroll
-        for (int u = 0; u < NS; u++) rate[u] = rate[u] + s1 * tb->nuw[st][u];
-      } else {
-        for (int m = 0; m < c->nsslen[st]; m++) {
-          int s = c->nsspec[st][m];
-          double add = s1 * c->diffmw[st][m];
-#pragma unroll
-          for (int u = 0; u < NS; u++)
-            if (u == s) rate[u] = rate[u] + add;
-        }
-      }
+      const RtMech mr{c, tb};
+      for_steps(mr, [&](int st) {
+        chem_step64<NS>(mr, st, t, lt, ovt, ry, [&](int s) { return lcy[s][tl]; }, [&](int s) { return lgb[s][tl]; },
+                        rate);
+      });
     }
 #pragma unroll
     for (int s = 0; s < NS; s++) fld(F.rate + s * fs, bo) = rate[s];
@@ -720,13 +850,120 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
 __device__ __forceinline__ float rf(float y) { return __builtin_amdgcn_rcpf(y); }
 __device__ __forceinline__ float qf(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
 
+// One step of the mixed-precision form: the rate exponents and Gibbs sums
+// assembled in FP64, everything else in FP32 (as chem_step64 otherwise).
+template <int NS, class M, class CY, class GB>
+__device__ __forceinline__ void chem_stepmx(const M &m, int st, float tf, double lt, double ovt, const float *ryf,
+                                            CY cyv, GB gbv, float *rate) {
+  const float ysmall = 1.0e-30f, ydenom = 1.0e-15f;
+  bool flthrd = true;
+  float s3 = 0.0f;
+  const int mb = m.mb(st);
+  if (mb != 0) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) s3 = s3 + ryf[s] * (m.ovwf(s) * m.efff(mb - 1, s));
+  }
+  double s2d = m.rp(st, 0) + m.rp(st, 1) * lt - m.rp(st, 2) * ovt;
+  float s1 = __expf((float)s2d);
+  const int ml = m.ml(st);
+  if (ml != 0) {
+    const int j = ml - 1;
+    float s4 = __expf((float)(m.rl(j, 0) + m.rl(j, 1) * lt - m.rl(j, 2) * ovt));
+    float preduc = qf(s3 * s4, s1);
+    float fornow = qf((float)m.rl(j, 3) * preduc, 1.0f + preduc);
+    s1 = s1 * fornow;
+    s2d = (double)__logf(s1);
+    flthrd = false;
+  }
+  const int mt = m.mt(st);
+  if (mt != 0) {
+    const int j = mt - 1;
+    float omalph = 1.0f - (float)m.rt(j, 3);
+    float s4 = __expf((float)(m.rt(j, 0) + m.rt(j, 1) * lt - m.rt(j, 2) * ovt));
+    float preduc = qf(s3 * s4, s1);
+    float trats1 = tf * (float)m.rt(j, 4), trats2 = (float)(-m.rt(j, 5) * ovt), trats3 = tf * (float)m.rt(j, 6);
+    float ftcent = omalph * __expf(trats3) + (float)m.rt(j, 3) * __expf(trats1) + __expf(trats2);
+    ftcent = __log10f(ftcent);
+    float cfactr = (float)m.rt(j, 7) + (float)m.rt(j, 8) * ftcent;
+    float enfact = (float)m.rt(j, 9) + (float)m.rt(j, 10) * ftcent;
+    float fornow = __log10f(preduc) + cfactr;
+    fornow = qf(fornow, enfact - (float)m.rt(j, 11) * fornow);
+    fornow = 1.0f + fornow * fornow;
+    float fbroad = qf(ftcent, fornow);
+    fbroad = __expf(fbroad * (float)m.clnten());
+    fornow = qf(fbroad * preduc, 1.0f + preduc);
+    s1 = s1 * fornow;
+    s2d = (double)__logf(s1);
+    flthrd = false;
+  }
+  const int ms = m.ms(st);
+  if (ms != 0) {
+    const int j = ms - 1;
+    float s4 = __expf((float)(m.rs(j, 0) + m.rs(j, 1) * lt - m.rs(j, 2) * ovt));
+    float preduc = qf(s3 * s4, s1);
+    float fornow = (float)(m.rs(j, 4) * ovt);
+    float fbroad = (float)m.rs(j, 3) * __expf(fornow);
+    fornow = tf * (float)m.rs(j, 5);
+    fbroad = fbroad + __expf(fornow);
+    fornow = __log10f(preduc);
+    fornow = rf(1.0f + __log10f(fornow));
+    fbroad = (float)m.rs(j, 6) * __expf(fornow * __logf(fbroad));
+    fornow = qf(fbroad * preduc, 1.0f + preduc);
+    s1 = s1 * fornow;
+    s2d = (double)__logf(s1);
+    flthrd = false;
+  }
+  for (int q = 0; q < m.nrs(st); q++) {
+    float fornow = cyv(m.nrsp(st, q));
+    fornow = fornow > 0.0f ? fornow : 0.0f;
+    s1 = s1 * fornow;
+  }
+  for (int q = 0; q < m.nrc(st); q++) {
+    float fornow = cyv(m.nrcp(st, q));
+    fornow = fornow > ysmall ? fornow : ysmall;
+    fornow = __expf(m.crsf(st, q) * __logf(fornow));
+    fornow = qf(fornow, 1.0f + ydenom * fornow);
+    s1 = s1 * fornow;
+  }
+  if (m.mg(st) != 0) {
+    for (int q = 0; q < m.nss(st); q++) s2d = s2d + m.dmu(st, q) * gbv(m.nssp(st, q));
+    float s2 = __expf((float)s2d);
+    for (int q = 0; q < m.nps(st); q++) {
+      float fornow = cyv(m.npsp(st, q));
+      fornow = fornow > 0.0f ? fornow : 0.0f;
+      s2 = s2 * fornow;
+    }
+    for (int q = 0; q < m.npc(st); q++) {
+      float fornow = cyv(m.npcp(st, q));
+      fornow = fornow > ysmall ? fornow : ysmall;
+      fornow = __expf(m.cpsf(st, q) * __logf(fornow));
+      fornow = qf(fornow, 1.0f + ydenom * fornow);
+      s2 = s2 * fornow;
+    }
+    s1 = s1 - s2;
+  }
+  if (mb != 0 && flthrd) s1 = s1 * s3;
+  if (m.dense()) {
+#pragma unroll
+    for (int u = 0; u < NS; u++) rate[u] = rate[u] + s1 * m.nuwf(st, u);
+  } else {
+    for (int q = 0; q < m.nss(st); q++) {
+      int sp = m.nssp(st, q);
+      float add = s1 * m.dmwf(st, q);
+#pragma unroll
+      for (int u = 0; u < NS; u++)
+        if (u == sp) rate[u] = rate[u] + add;
+    }
+  }
+}
+
 // Mixed-precision point physics: the temperature Newton iteration, the
 // primitive variables and the enthalpies are formed in FP64 as in k_prim and
 // stored in FP32; transport and chemistry run in FP32 with the hardware
 // exp2/log2 based intrinsics, except that the rate exponents (ln k = ln A +
 // n ln T - E/RT, Gibbs sums) are assembled in FP64 before the FP32
 // exponential, since they are differences of O(10..100) terms.
-template <int NS>
+template <int NS, bool FIX>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_prim_mx(const Member *__restrict__ mem,
     const uint32_t *__restrict__ map, const sg_config *__restrict__ c, const Tables *__restrict__ tb) {
   int lb;
@@ -862,7 +1099,6 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   SG_PHASE();
   // --- chemistry (src/chrate.f:146-760) on rho Y ---
   {
-    const float ysmall = 1.0e-30f, ydenom = 1.0e-15f;
     float rate[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) rate[s] = 0.0f;
@@ -873,116 +1109,29 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       int it = itn[s];
       gibbs[s] = gibbs_of(tb, s, it, t, ovt, lt);
     }
-    __shared__ float lcy[NS][BLK];
-    __shared__ double lgb[NS][BLK];
-    const int tl = threadIdx.x;
+    if constexpr (FIX) {
+      using MT = CtMech<MechH2>;
+      float cy[NS];
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-      lcy[s][tl] = ryf[s] * tb->ovwf[s];
-      lgb[s][tl] = gibbs[s];
-    }
-    auto cyv = [&](int s) { return lcy[s][tl]; };
-    auto gbv = [&](int s) { return lgb[s][tl]; };
-    for (int st = 0; st < c->nstep; st++) {
-      bool flthrd = true;
-      float s3 = 0.0f;
-      int mb = c->mblist[st];
-      if (mb != 0) {
+      for (int s = 0; s < NS; s++) cy[s] = ryf[s] * MT::ovwf(s);
+      for_steps(MT{}, [&](int st) {
+        chem_stepmx<NS>(MT{}, st, tf, lt, ovt, ryf, [&](int s) { return cy[s]; }, [&](int s) { return gibbs[s]; },
+                        rate);
+      });
+    } else {
+      __shared__ float lcy[NS][BLK];
+      __shared__ double lgb[NS][BLK];
+      const int tl = threadIdx.x;
 #pragma unroll
-        for (int s = 0; s < NS; s++) s3 = s3 + ryf[s] * (tb->ovwf[s] * tb->effyf[mb - 1][s]);
+      for (int s = 0; s < NS; s++) {
+        lcy[s][tl] = ryf[s] * tb->ovwf[s];
+        lgb[s][tl] = gibbs[s];
       }
-      double s2d = c->rparam[st][0] + c->rparam[st][1] * lt - c->rparam[st][2] * ovt;
-      float s1 = __expf((float)s2d);
-      int ml = c->mllist[st];
-      if (ml != 0) {
-        const double *rl = c->rclind[ml - 1];
-        float s4 = __expf((float)(rl[0] + rl[1] * lt - rl[2] * ovt));
-        float preduc = qf(s3 * s4, s1);
-        float fornow = qf((float)rl[3] * preduc, 1.0f + preduc);
-        s1 = s1 * fornow;
-        s2d = (double)__logf(s1);
-        flthrd = false;
-      }
-      int mt = c->mtlist[st];
-      if (mt != 0) {
-        const double *rt = c->rctroe[mt - 1];
-        float omalph = 1.0f - (float)rt[3];
-        float s4 = __expf((float)(rt[0] + rt[1] * lt - rt[2] * ovt));
-        float preduc = qf(s3 * s4, s1);
-        float trats1 = tf * (float)rt[4], trats2 = (float)(-rt[5] * ovt), trats3 = tf * (float)rt[6];
-        float ftcent = omalph * __expf(trats3) + (float)rt[3] * __expf(trats1) + __expf(trats2);
-        ftcent = __log10f(ftcent);
-        float cfactr = (float)rt[7] + (float)rt[8] * ftcent;
-        float enfact = (float)rt[9] + (float)rt[10] * ftcent;
-        float fornow = __log10f(preduc) + cfactr;
-        fornow = qf(fornow, enfact - (float)rt[11] * fornow);
-        fornow = 1.0f + fornow * fornow;
-        float fbroad = qf(ftcent, fornow);
-        fbroad = __expf(fbroad * (float)c->clnten);
-        fornow = qf(fbroad * preduc, 1.0f + preduc);
-        s1 = s1 * fornow;
-        s2d = (double)__logf(s1);
-        flthrd = false;
-      }
-      int ms = c->mslist[st];
-      if (ms != 0) {
-        const double *rs = c->rcsrif[ms - 1];
-        float s4 = __expf((float)(rs[0] + rs[1] * lt - rs[2] * ovt));
-        float preduc = qf(s3 * s4, s1);
-        float fornow = (float)(rs[4] * ovt);
-        float fbroad = (float)rs[3] * __expf(fornow);
-        fornow = tf * (float)rs[5];
-        fbroad = fbroad + __expf(fornow);
-        fornow = __log10f(preduc);
-        fornow = rf(1.0f + __log10f(fornow));
-        fbroad = (float)rs[6] * __expf(fornow * __logf(fbroad));
-        fornow = qf(fbroad * preduc, 1.0f + preduc);
-        s1 = s1 * fornow;
-        s2d = (double)__logf(s1);
-        flthrd = false;
-      }
-      for (int m = 0; m < c->nrslen[st]; m++) {
-        float fornow = cyv(c->nrspec[st][m]);
-        fornow = fornow > 0.0f ? fornow : 0.0f;
-        s1 = s1 * fornow;
-      }
-      for (int m = 0; m < c->nrclen[st]; m++) {
-        float fornow = cyv(c->nrcpec[st][m]);
-        fornow = fornow > ysmall ? fornow : ysmall;
-        fornow = __expf(tb->crspecf[st][m] * __logf(fornow));
-        fornow = qf(fornow, 1.0f + ydenom * fornow);
-        s1 = s1 * fornow;
-      }
-      if (c->mglist[st] != 0) {
-        for (int m = 0; m < c->nsslen[st]; m++) s2d = s2d + c->diffmu[st][m] * gbv(c->nsspec[st][m]);
-        float s2 = __expf((float)s2d);
-        for (int m = 0; m < c->npslen[st]; m++) {
-          float fornow = cyv(c->npspec[st][m]);
-          fornow = fornow > 0.0f ? fornow : 0.0f;
-          s2 = s2 * fornow;
-        }
-        for (int m = 0; m < c->npclen[st]; m++) {
-          float fornow = cyv(c->npcpec[st][m]);
-          fornow = fornow > ysmall ? fornow : ysmall;
-          fornow = __expf(tb->cpspecf[st][m] * __logf(fornow));
-          fornow = qf(fornow, 1.0f + ydenom * fornow);
-          s2 = s2 * fornow;
-        }
-        s1 = s1 - s2;
-      }
-      if (mb != 0 && flthrd) s1 = s1 * s3;
-      if (tb->ratedense) {
-#pragma unroll
-        for (int u = 0; u < NS; u++) rate[u] = rate[u] + s1 * tb->nuwf[st][u];
-      } else {
-        for (int m = 0; m < c->nsslen[st]; m++) {
-          int s = c->nsspec[st][m];
-          float add = s1 * tb->diffmwf[st][m];
-#pragma unroll
-          for (int u = 0; u < NS; u++)
-            if (u == s) rate[u] = rate[u] + add;
-        }
-      }
+      const RtMech mr{c, tb};
+      for_steps(mr, [&](int st) {
+        chem_stepmx<NS>(mr, st, tf, lt, ovt, ryf, [&](int s) { return lcy[s][tl]; }, [&](int s) { return lgb[s][tl]; },
+                        rate);
+      });
     }
 #pragma unroll
     for (int s = 0; s < NS; s++) fld((float *)F.rate + s * fs, b4) = rate[s];
@@ -1101,6 +1250,8 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
     }
   }
 }
+
+#pragma clang fp contract(fast)
 
 // ---------------------------------------------------------------------------
 // Direction pass: all derivatives along DIR and the RHS terms separable in
@@ -2046,8 +2197,15 @@ void launch_bountt_t(const Launch &L, hipStream_t st) {
 template <int NS>
 void launch_prim_t(const Launch &L, hipStream_t st) {
   const dim3 g(L.nblk[L_POINT]), b(BLK);
-  if (L.precision) hipLaunchKernelGGL(k_prim_mx<NS>, g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
-  else hipLaunchKernelGGL(k_prim<NS>, g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
+  if constexpr (NS == MechH2::nspec) {
+    if (L.mechfix) {
+      if (L.precision) hipLaunchKernelGGL((k_prim_mx<NS, true>), g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
+      else hipLaunchKernelGGL((k_prim<NS, true>), g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
+      return;
+    }
+  }
+  if (L.precision) hipLaunchKernelGGL((k_prim_mx<NS, false>), g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
+  else hipLaunchKernelGGL((k_prim<NS, false>), g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
 }
 template <int NS, class R, int NW>
 static void launch_pass_w(int dir, const Launch &L, hipStream_t st) {

@@ -67,43 +67,11 @@ class Config(C.Structure):
     ]
 
 
-def config_from_case(case, precision=0) -> Config:
-    """Fill sg_config from a Case (control + derived tables); precision 0 is
-    FP64, 1 the mixed FP32/FP64 mode."""
-    c = Config()
-    ctl, t, m = case.control, case.tables, case.tables.mech
+def fill_chemistry(c: Config, t) -> None:
+    """The chemistry members of sg_config (step lists, rate parameters, third
+    bodies, falloff constants) from derived tables ``t`` (sengapy.chem.derive)."""
+    m = t.mech
     ns, nst = t.nspec, t.nstep
-    if ns > MAX_SPEC or nst > MAX_STEP:
-        raise ValueError("mechanism too large for sg_config")
-    c.nx, c.ny, c.nz = ctl.nx, ctl.ny, ctl.nz
-    c.nspec, c.nstep = ns, nst
-    c.mixavg = 1 if ctl.nfmavt == 1 else 0
-    c.precision = precision
-    c.xgdlen, c.ygdlen, c.zgdlen = ctl.xgdlen, ctl.ygdlen, ctl.zgdlen
-    for f, (typ, ip, rp) in enumerate(ctl.bc):
-        c.bc_type[f] = typ
-        for q in range(4):
-            c.bc_ipar[f][q] = ip[q]
-            c.bc_rpar[f][q] = rp[q]
-    c.prin, c.trin, c.urin, c.vrin, c.wrin = ctl.prin, ctl.trin, ctl.urin, ctl.vrin, ctl.wrin
-    for s in range(ns):
-        c.yrin[s] = ctl.yrin[s] if s < len(ctl.yrin) else 0.0
-    from .chem import RLAMDA, PRANTL
-    c.alamda, c.rlamda, c.prantl = t.alamda, RLAMDA, PRANTL
-    for s in range(ns):
-        c.olewis[s] = t.olewis[s]
-        c.wmolar[s], c.ovwmol[s], c.rgspec[s] = t.wmolar[s], t.ovwmol[s], t.rgspec[s]
-        c.ntint[s] = int(t.ntint[s])
-        for it in range(int(t.ntint[s])):
-            c.ncpoly[s][it] = int(t.ncpoly[it, s])
-            c.ncenth[s][it] = int(t.ncenth[it, s])
-            c.ncenpy[s][it] = int(t.ncenpy[it, s])
-            c.tinthi[s][it] = t.tinthi[it, s]
-            for q in range(NCOF):
-                c.amascp[s][it][q] = t.amascp[q, it, s]
-                c.amasct[s][it][q] = t.amasct[q, it, s]
-                c.amasch[s][it][q] = t.amasch[q, it, s]
-                c.amolgb[s][it][q] = t.amolgb[q, it, s]
     for st in range(nst):
         c.rparam[st][0], c.rparam[st][1], c.rparam[st][2] = t.rparam[:, st]
         c.nsslen[st] = len(m.nsspec[st])
@@ -143,6 +111,46 @@ def config_from_case(case, precision=0) -> Config:
         for q in range(8):
             c.rcsrif[j][q] = t.rcsrif[q, j]
     c.clnten = math.log(1.0e1)          # src/indata.f:1071
+
+
+def config_from_case(case, precision=0) -> Config:
+    """Fill sg_config from a Case (control + derived tables); precision 0 is
+    FP64, 1 the mixed FP32/FP64 mode."""
+    c = Config()
+    ctl, t, m = case.control, case.tables, case.tables.mech
+    ns, nst = t.nspec, t.nstep
+    if ns > MAX_SPEC or nst > MAX_STEP:
+        raise ValueError("mechanism too large for sg_config")
+    c.nx, c.ny, c.nz = ctl.nx, ctl.ny, ctl.nz
+    c.nspec, c.nstep = ns, nst
+    c.mixavg = 1 if ctl.nfmavt == 1 else 0
+    c.precision = precision
+    c.xgdlen, c.ygdlen, c.zgdlen = ctl.xgdlen, ctl.ygdlen, ctl.zgdlen
+    for f, (typ, ip, rp) in enumerate(ctl.bc):
+        c.bc_type[f] = typ
+        for q in range(4):
+            c.bc_ipar[f][q] = ip[q]
+            c.bc_rpar[f][q] = rp[q]
+    c.prin, c.trin, c.urin, c.vrin, c.wrin = ctl.prin, ctl.trin, ctl.urin, ctl.vrin, ctl.wrin
+    for s in range(ns):
+        c.yrin[s] = ctl.yrin[s] if s < len(ctl.yrin) else 0.0
+    from .chem import RLAMDA, PRANTL
+    c.alamda, c.rlamda, c.prantl = t.alamda, RLAMDA, PRANTL
+    for s in range(ns):
+        c.olewis[s] = t.olewis[s]
+        c.wmolar[s], c.ovwmol[s], c.rgspec[s] = t.wmolar[s], t.ovwmol[s], t.rgspec[s]
+        c.ntint[s] = int(t.ntint[s])
+        for it in range(int(t.ntint[s])):
+            c.ncpoly[s][it] = int(t.ncpoly[it, s])
+            c.ncenth[s][it] = int(t.ncenth[it, s])
+            c.ncenpy[s][it] = int(t.ncenpy[it, s])
+            c.tinthi[s][it] = t.tinthi[it, s]
+            for q in range(NCOF):
+                c.amascp[s][it][q] = t.amascp[q, it, s]
+                c.amasct[s][it][q] = t.amasct[q, it, s]
+                c.amasch[s][it][q] = t.amasch[q, it, s]
+                c.amolgb[s][it][q] = t.amolgb[q, it, s]
+    fill_chemistry(c, t)
     dt = case.diff
     if dt is not None:
         c.pdifgb, c.tdifgb = dt.pdifgb, dt.tdifgb
@@ -170,7 +178,8 @@ ABI_FUNCS = ["sg_abi_version", "sg_config_size", "sg_create", "sg_destroy", "sg_
              "sg_profile", "sg_profile_read", "sg_memory", "sg_set_adaptive", "sg_get_timestep",
              "sg_batch_create", "sg_batch_destroy", "sg_batch_add", "sg_batch_remove", "sg_batch_size",
              "sg_batch_member", "sg_batch_step", "sg_batch_sync", "sg_footprint", "sg_set_tiles",
-             "sg_get_tiles", "sg_autotune", "sg_batch_set_tiles", "sg_batch_get_tiles", "sg_batch_autotune"]
+             "sg_get_tiles", "sg_autotune", "sg_batch_set_tiles", "sg_batch_get_tiles", "sg_batch_autotune",
+             "sg_mechanism_compiled"]
 
 
 def load_library(path=None):
@@ -207,6 +216,7 @@ def load_library(path=None):
     lib.sg_set_tiles.argtypes = [vp, i3]
     lib.sg_get_tiles.argtypes = [vp, i3]
     lib.sg_autotune.argtypes = [vp, C.c_int, dp]
+    lib.sg_mechanism_compiled.argtypes = [vp, P(C.c_int)]
     lib.sg_batch_set_tiles.argtypes = [vp, i3]
     lib.sg_batch_get_tiles.argtypes = [vp, i3]
     lib.sg_batch_autotune.argtypes = [vp, C.c_int, dp]
@@ -245,12 +255,13 @@ class SgError(RuntimeError):
 class Solver:
     """One case on the current GPU through the C ABI."""
 
-    def __init__(self, case, lib=None, precision=0):
+    def __init__(self, case, lib=None, precision=0, cfg=None):
+        """cfg: a Config to create from instead of config_from_case(case)."""
         self.lib = lib or load_library()
         if self.lib.sg_config_size() != C.sizeof(Config):
             raise RuntimeError("sg_config layout mismatch between library and binding")
         self.case = case
-        self.cfg = config_from_case(case, precision)
+        self.cfg = cfg if cfg is not None else config_from_case(case, precision)
         h = C.c_void_p()
         rc = self.lib.sg_create(C.byref(self.cfg), C.byref(h))
         if rc != 0:
@@ -345,6 +356,13 @@ class Solver:
         b = C.c_size_t(0)
         self._chk(self.lib.sg_memory(self.h, C.byref(b)))
         return b.value
+
+    def mechanism_compiled(self) -> bool:
+        """True when the chemistry is the mechanism compiled into the library
+        (the point kernels then run its unrolled step loop)."""
+        v = C.c_int(0)
+        self._chk(self.lib.sg_mechanism_compiled(self.h, C.byref(v)))
+        return bool(v.value)
 
     def set_tiles(self, narrow=(0, 0, 0)):
         """Pass tiles per direction (0: rows of 32, 1: rows of 16); results do
