@@ -437,7 +437,8 @@ def main():
     vfile = os.path.join(ROOT, "profiles", "valu.json")
     if os.path.exists(vfile):
         vj = json.load(open(vfile))
-        k = vj.get("kernels", {}).get(f"k_prim<{ns}>")
+        which = "true" if s.mechanism_compiled() else "false"   # compiled or table-driven step loop
+        k = vj.get("kernels", {}).get(f"k_prim<{ns},{which}>")
         if vj.get("workload") == f"h2_nscbc_mix_{n}^3" and k:
             valu = {"kernel": "prim", "bound": "fp64 valu", "achieved": k["fp64_tflops"], "peak": 78.6,
                     "unit": "TFLOP/s", "frac": k["fp64_tflops"] / 78.6, "valu_issue_use": k["valu_issue_util"],
