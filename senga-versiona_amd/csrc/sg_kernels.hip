@@ -1289,10 +1289,8 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
   // every thread loads and stores unconditionally
   constexpr int NIT = (LN + NTH - 1) / NTH;
   // Two buffers of up to STW fields each: a stage stages one field, or the
-  // three fields of a species (Y_k, rho D_k, h_k) under one barrier.  The
-  // FP64 x pass stages one field at a time: the extra prefetch registers
-  // would spill there.
-  constexpr bool SPW = !(DIR == 0 && sizeof(R) == 8);
+  // three fields of a species (Y_k, rho D_k, h_k) under one barrier.
+  constexpr bool SPW = true;
   // The velocity, temperature and flux stages are grouped the same way where
   // that does not spill (measured: the y pass, and the FP32 z pass).
   constexpr bool EMW = DIR == 1 || (DIR == 2 && sizeof(R) == 4);
@@ -1332,6 +1330,17 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
     for (int q = 0; q < 5; q++) pk[q] = c_row_offsets.w[row][q];
     edge = __ballot(S.mode != 0) != 0;
   }
+  // Row offsets for one stencil.  In FP64 each stencil takes fresh copies, so
+  // that its neighbour addresses are formed per stencil rather than held live
+  // for the whole pass (they made the FP64 x pass spill); the FP32 pass has
+  // the registers and is faster keeping them.
+  auto pkc = [&](int (&pq)[5]) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      pq[q] = pk[q];
+      if constexpr (sizeof(R) == 8) asm volatile("" : "+v"(pq[q]));
+    }
+  };
   // LDS position of this point
   const int lbase = (lz + (DIR == 2 ? 5 : 0)) * LX * LY + (ly + (DIR == 1 ? 5 : 0)) * LX + lx + (DIR == 0 ? 5 : 0);
   // staging: global source index of LDS slot l (wrap when periodic, clamp when bounded)
@@ -1383,13 +1392,13 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
   auto D1f = [&](int f) {
     auto at = [&](int kk) { return tile[buf][f][lbase + (kk - 8) * LSTR]; };
     auto fq = [&](int q) { return tile[buf][f][lbase + q * LSTR]; };
-    if (BND && edge) return d1_rows<R>(P, pk, DIR, wq, fq);
+    if (BND && edge) { int pq[5]; pkc(pq); return d1_rows<R>(P, pq, DIR, wq, fq); }
     return d1<false, R>(P, S, DIR, at);
   };
   auto D2f = [&](int f) {
     auto at = [&](int kk) { return tile[buf][f][lbase + (kk - 8) * LSTR]; };
     auto fq = [&](int q) { return tile[buf][f][lbase + q * LSTR]; };
-    if (BND && edge) return d2_rows<R>(P, pk, DIR, wq, fq);
+    if (BND && edge) { int pq[5]; pkc(pq); return d2_rows<R>(P, pq, DIR, wq, fq); }
     return d2<false, R>(P, S, DIR, at);
   };
   auto ctrf = [&](int f) { return tile[buf][f][lbase]; };
@@ -1430,7 +1439,7 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
   auto atm = [&](int kk) { return tmd[lbase + (kk - 8) * LSTR]; };
   auto fqm = [&](int q) { return tmd[lbase + q * LSTR]; };
   auto D1M = [&]() {
-    if (BND && edge) return d1_rows<R>(P, pk, DIR, wq, fqm);
+    if (BND && edge) { int pq[5]; pkc(pq); return d1_rows<R>(P, pq, DIR, wq, fqm); }
     return d1<false, R>(P, S, DIR, atm);
   };
   // The product stencil reads twice the operands of a plain one; its loads
@@ -1447,7 +1456,7 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
     };
     SG_PHASE();
     R r;
-    if (BND && edge) r = d1_rows<R>(P, pk, DIR, wq, fqp);
+    if (BND && edge) { int pq[5]; pkc(pq); r = d1_rows<R>(P, pq, DIR, wq, fqp); }
     else r = d1<false, R>(P, S, DIR, atp);
     pin(r);
     SG_PHASE();
