@@ -1291,9 +1291,9 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
   // Two buffers of up to STW fields each: a stage stages one field, or the
   // three fields of a species (Y_k, rho D_k, h_k) under one barrier.
   constexpr bool SPW = true;
-  // The velocity, temperature and flux stages are grouped the same way where
-  // that does not spill (measured: the y pass, and the FP32 z pass).
-  constexpr bool EMW = DIR == 1 || (DIR == 2 && sizeof(R) == 4);
+  // The velocity, temperature and flux stages are grouped the same way in the
+  // y and z passes (measured faster there, a few spills in FP64 included).
+  constexpr bool EMW = DIR != 0;
   constexpr int STW = SPW ? 3 : 1;
   __shared__ R tile[2][STW][NIT * NTH];
   const int t = threadIdx.x;
