@@ -1288,13 +1288,13 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
   // LN staged values per field, padded to whole block-sized rows so that
   // every thread loads and stores unconditionally
   constexpr int NIT = (LN + NTH - 1) / NTH;
-  // Two buffers of up to STW fields each: a stage stages one field, or the
-  // three fields of a species (Y_k, rho D_k, h_k) under one barrier.
-  constexpr bool SPW = true;
+  // Two buffers of up to STW fields each: a stage stages one field, or three
+  // fields under one barrier (a species' Y_k, rho D_k and h_k; grouped
+  // primitive and flux stages below).
   // The velocity, temperature and flux stages are grouped the same way in the
   // y and z passes (measured faster there, a few spills in FP64 included).
   constexpr bool EMW = DIR != 0;
-  constexpr int STW = SPW ? 3 : 1;
+  constexpr int STW = 3;
   __shared__ R tile[2][STW][NIT * NTH];
   const int t = threadIdx.x;
   const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
@@ -1473,10 +1473,8 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
   auto prefetch_species = [&](int sp) {
     const R *Ys = pr + prim_y(P, sp), *Rs = pr + prim_rd(P, sp), *Hs = pr + prim_h(P, sp);
     prefetch_to(0, [Ys](long g) { return Ys[g]; });
-    if constexpr (SPW) {
-      prefetch_to(1, [Rs](long g) { return Rs[g]; });
-      prefetch_to(2, [Hs](long g) { return Hs[g]; });
-    }
+    prefetch_to(1, [Rs](long g) { return Rs[g]; });
+    prefetch_to(2, [Hs](long g) { return Hs[g]; });
   };
   auto f_pp = [&](long g) { return pp[g] - pref; };
   auto f_mu = [&](long g) { return mu[g]; };
@@ -1685,41 +1683,20 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
 #pragma unroll
   for (int sp = 0; sp < NS; sp++) {
     R dY, d2Y, yc, dYm, drD, rdc, dh, hc;
-    if constexpr (SPW) {
-      stage_n(3);
-      if (sp + 1 < NS) prefetch_species(sp + 1);
-      else prefetch([&](long g) { return hm[g]; });
-      dY = D1f(0); d2Y = D2f(0); yc = ctrf(0);
-      pin(dY); pin(d2Y); pin(yc);
-      dYm = D1P();
-      pin(dYm);
-      SG_PHASE();
-      drD = D1f(1); rdc = ctrf(1);
-      pin(drD); pin(rdc);
-      SG_PHASE();
-      dh = D1f(2);
-      hc = ctrf(2);
-      next();
-    } else {
-      const R *Rs = pr + prim_rd(P, sp), *Hs = pr + prim_h(P, sp);
-      stage();
-      prefetch([&](long g) { return Rs[g]; });
-      dY = D1(); d2Y = D2(); yc = ctr();
-      dYm = D1P();
-      pin(dY); pin(d2Y); pin(yc); pin(dYm);
-      next();
-      stage();
-      prefetch([&](long g) { return Hs[g]; });
-      drD = D1(); rdc = ctr();
-      pin(drD); pin(rdc);
-      next();
-      stage();
-      if (sp + 1 < NS) prefetch_species(sp + 1);
-      else prefetch([&](long g) { return hm[g]; });
-      dh = D1();
-      hc = ctr();
-      next();
-    }
+    stage_n(3);
+    if (sp + 1 < NS) prefetch_species(sp + 1);
+    else prefetch([&](long g) { return hm[g]; });
+    dY = D1f(0); d2Y = D2f(0); yc = ctrf(0);
+    pin(dY); pin(d2Y); pin(yc);
+    dYm = D1P();
+    pin(dYm);
+    SG_PHASE();
+    drD = D1f(1); rdc = ctrf(1);
+    pin(drD); pin(rdc);
+    SG_PHASE();
+    dh = D1f(2);
+    hc = ctrf(2);
+    next();
     R ys = -R(0.5) * yc * dm - R(0.5) * dYm - R(0.5) * md0 * dY;
     const R g1 = (zdif ? R(0) : drD) * dY;
     const R g2 = rdc * (zdif ? R(0) : d2Y);
