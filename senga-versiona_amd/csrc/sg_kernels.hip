@@ -1293,6 +1293,8 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
   // primitive and flux stages below).
   // The velocity, temperature and flux stages are grouped the same way in the
   // y and z passes (measured faster there, a few spills in FP64 included).
+  // Grouping changes the order of some sums, so both tiles of a direction
+  // take the same path: the tile choice never changes a result.
   constexpr bool EMW = DIR != 0;
   constexpr int STW = 3;
   __shared__ R tile[2][STW][NIT * NTH];
