@@ -666,10 +666,7 @@ int sg_upload_state(sg_handle *h, const double *run, double etime) {
   size_t fb = sizeof(double) * P.npts;
   for (int v = 0; v < 5 + P.ns; v++) {
     HIPCHK(h, hipMemcpyAsync(h->F.run + v * P.fstride, run + v * P.npts, fb, hipMemcpyHostToDevice, h->stream));
-    // the FP32 stage register of the mixed form is filled from the solution
-    // register by TEMPIN below
-    if (!P.precision)
-      HIPCHK(h, hipMemcpyAsync(h->F.q + v * P.fstride, run + v * P.npts, fb, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->F.q + v * P.fstride, run + v * P.npts, fb, hipMemcpyHostToDevice, h->stream));
   }
   HIPCHK(h, hipMemsetAsync(h->F.rate, 0, (P.precision ? sizeof(float) : sizeof(double)) * P.fstride * P.ns, h->stream));
   if (h->F.erk) launch_zero(h->F.erk, (long)(5 + P.ns) * P.fstride, h->stream);

@@ -129,14 +129,7 @@ __global__ void __launch_bounds__(BLK) k_tempin(Params P, const sg_config *__res
   long idx = (long)blockIdx.x * BLK + threadIdx.x;
   if (idx >= P.npts) return;
   const long fs = P.fstride;
-  // the solution register (the stage register holds the same state after an
-  // upload); in the mixed form the stage register is FP32 and filled here
-  const double *q = F.run;
-  if constexpr (sizeof(R) == 4) {
-    float *qf = (float *)F.q;
-#pragma unroll
-    for (int v = 0; v < 5 + NS; v++) qf[v * fs + idx] = (float)q[v * fs + idx];
-  }
+  const double *q = F.q;
   double d = q[idx], ru = q[fs + idx], rv = q[2 * fs + idx], rw = q[3 * fs + idx];
   double ry[NS];
 #pragma unroll
@@ -222,12 +215,6 @@ __global__ void __launch_bounds__(BLK) k_tempin(Params P, const sg_config *__res
   pr[P_P * fs + idx] = (R)(t * s7 - P.pofs);
 }
 
-// Element o of the stage register, which is FP32 in the mixed form.
-__device__ __forceinline__ void set_q(const Params &P, const Fields &F, long o, double v) {
-  if (P.precision) ((float *)F.q)[o] = (float)v;
-  else F.q[o] = v;
-}
-
 // ---------------------------------------------------------------------------
 // BOUNDT for NSBCI3 at x-left (src/boundt.f:177-218; BCDTXL, BCUTXL type 1, BCYTXL)
 template <int NS>
@@ -242,12 +229,12 @@ __global__ void __launch_bounds__(BLK) k_boundt(const Member *__restrict__ mem, 
   long p = t * P.nx;  // i = 0
   const long fs = P.fstride;
   double strd = P.drin, stru = P.uin;
-  set_q(P, F, p, strd);
-  set_q(P, F, fs + p, strd * stru);
-  set_q(P, F, 2 * fs + p, strd * 0.0);
-  set_q(P, F, 3 * fs + p, strd * 0.0);
+  F.q[p] = strd;
+  F.q[fs + p] = strd * stru;
+  F.q[2 * fs + p] = strd * 0.0;
+  F.q[3 * fs + p] = strd * 0.0;
 #pragma unroll
-  for (int s = 0; s < NS; s++) set_q(P, F, (5 + s) * fs + p, strd * P.yrin[s]);
+  for (int s = 0; s < NS; s++) F.q[(5 + s) * fs + p] = strd * P.yrin[s];
 }
 
 // BOUNTT for NSBCI3 at x-left (src/bountt.f:195-262)
@@ -266,13 +253,13 @@ __global__ void __launch_bounds__(BLK) k_bountt(const Member *__restrict__ mem, 
   double v[4] = {strd, strd * stru, strd * 0.0, strd * 0.0};
 #pragma unroll
   for (int m = 0; m < 4; m++) {
-    set_q(P, F, m * fs + p, v[m]);
+    F.q[m * fs + p] = v[m];
     F.run[m * fs + p] = v[m];
   }
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     double y = strd * P.yrin[s];
-    set_q(P, F, (5 + s) * fs + p, y);
+    F.q[(5 + s) * fs + p] = y;
     F.run[(5 + s) * fs + p] = y;
   }
   // the imposed values carry no RK error (src/bountt.f)
@@ -987,13 +974,13 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   if (idx >= P.npts) return;
   const uint32_t bo = (uint32_t)idx * 8u, b4 = (uint32_t)idx * 4u;
   const long fs = P.fstride;
-  const float *q = (const float *)F.q;   // the stage register is FP32 in this form
+  const double *q = F.q;
   float *pr = (float *)F.prim;
-  double d = fld(q, b4), ru = fld(q + fs, b4), rv = fld(q + 2 * fs, b4), rw = fld(q + 3 * fs, b4);
-  double re = fld(q + 4 * fs, b4);
+  double d = fld(q, bo), ru = fld(q + fs, bo), rv = fld(q + 2 * fs, bo), rw = fld(q + 3 * fs, bo);
+  double re = fld(q + 4 * fs, bo);
   double ry[NS];
 #pragma unroll
-  for (int s = 0; s < NS; s++) ry[s] = fld(q + (5 + s) * fs, b4);
+  for (int s = 0; s < NS; s++) ry[s] = fld(q + (5 + s) * fs, bo);
   // --- TEMPER (src/temper.f:58-160) ---
   const double ovd = 1.0 / d;   // one reciprocal for every division by rho
   double ukuk = (ru * ru + rv * rv + rw * rw) * ovd;
@@ -1422,10 +1409,10 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
   auto ctr = [&]() { return ctrf(0); };
   auto next = [&]() { buf ^= 1; };
   const long fs = P.fstride;
-  const R *Q = (const R *)F.q;   // the stage register, in the working precision
+  const double *Q = F.q;
   const R *pr = (const R *)F.prim;
-  const R *mx = Q + fs, *my = Q + 2 * fs, *mz = Q + 3 * fs;
-  const R *md = DIR == 0 ? mx : (DIR == 1 ? my : mz);
+  const double *mx = Q + fs, *my = Q + 2 * fs, *mz = Q + 3 * fs;
+  const double *md = DIR == 0 ? mx : (DIR == 1 ? my : mz);
   const R *uu = pr + P_U * fs, *vv = pr + P_V * fs, *ww = pr + P_W * fs;
   const R *ee = pr + P_E * fs, *TT = pr + P_T * fs, *pp = pr + P_P * fs;
   const R *la = pr + P_LAM * fs, *mu = pr + P_MU * fs, *hm = pr + P_HMIX * fs;
@@ -1802,8 +1789,7 @@ __global__ void __launch_bounds__(NSCBC_BLK) k_nscbc(const Member *__restrict__ 
     const double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
     const double strg = fc[F_CP * nf + fq], strr = fc[F_RR * nf + fq];
     const double strt = pr[P_T * fs + idx];
-    const R *Qr = (const R *)F.q;   // the stage register, in the working precision
-    const double strd = (double)Qr[idx];
+    const double strd = F.q[idx];
     double gam1 = strg - strr;
     const double stre = pr[P_E * fs + idx] - gam1 * strt;
     gam1 = strr / gam1;
@@ -1817,7 +1803,7 @@ __global__ void __launch_bounds__(NSCBC_BLK) k_nscbc(const Member *__restrict__ 
       // NSBCI3, src/bounds.f:610-682 (imposed values have zero time derivative)
       // drho/dx at the face (src/rhscal.f:3966-3990), one-sided closure row
       const Sten Sx = make_sten<BND>(i, P.nx, 1);
-      const double drdx = d1<BND>(P, Sx, 0, [&](int kk) { return (double)Qr[idx + Sx.o[kk]]; });
+      const double drdx = d1<BND>(P, Sx, 0, [&](int kk) { return F.q[idx + Sx.o[kk]]; });
       double bcl1 = dux, bcl2 = drdx, bcl3 = gr(1), bcl4 = gr(2), bcl5 = fc[F_DPDX * nf + fq];
       fornow = strd * acou * bcl1;
       bcl1 = 0.5 * (u - acou) * (bcl5 - fornow);
@@ -1998,21 +1984,20 @@ __device__ __forceinline__ void final_point(const Params &P, const Fields &F, co
 #pragma unroll
   for (int vv2 = 0; vv2 < 5 + NS; vv2++) {
     const int slot = vv2 < 5 ? vv2 : 5;
-    double *runv = F.run + vv2 * fs;
-    R *qv = (R *)F.q + vv2 * fs;   // the stage register, in the working precision
+    double *runv = F.run + vv2 * fs, *qv = F.q + vv2 * fs;
     if (i >= P.rklo[slot]) {
       if (F.erk) fld(F.erk + vv2 * fs, bo8) += erc * (double)Rv[vv2];   // *ERR += RKERR * R
       double old = fld(runv, bo8);
       if (mode == 0) {
         fld(runv, bo8) = old + lhs * (double)Rv[vv2];
-        fld(qv, bo) = (R)(old + rhsc * (double)Rv[vv2]);
+        fld(qv, bo8) = old + rhsc * (double)Rv[vv2];
       } else {
         double nw = old + lhs * (double)Rv[vv2];
         fld(runv, bo8) = nw;
-        fld(qv, bo) = (R)nw;
+        fld(qv, bo8) = nw;
       }
     } else {
-      fld(qv, bo) = Rv[vv2];
+      fld(qv, bo8) = (double)Rv[vv2];
     }
   }
 }
