@@ -444,6 +444,17 @@ def main():
                     "unit": "TFLOP/s", "frac": k["fp64_tflops"] / 78.6, "valu_issue_use": k["valu_issue_util"],
                     "source": "profiles/valu_h2_nscbc_mix_128.md (rocprofv3 --pmc)"}
     substage_bytes = 32 * (5 + ns) + 16      # compulsory state traffic of one fused substage
+    # HBM bytes one substage actually moves (the per-kernel rocprofv3 FETCH_SIZE
+    # + WRITE_SIZE of the same workload) over the measured substage time
+    hbm_sub = None
+    if os.path.exists(tfile):
+        t = json.load(open(tfile))
+        if t.get("workload") == f"h2_nscbc_mix_{n}^3":
+            per = sum(t["kernels"].get(k, 0.0) for k in ("prim", "pass_x", "pass_y", "pass_z", "nscbc", "final"))
+            sub_s = ms.value / a.steps / 5 / 1e3
+            hbm_sub = {"traffic_bytes": per, "bytes_per_point": per / npts, "achieved": per / sub_s / 1e9,
+                       "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": per / sub_s / HBM_PEAK,
+                       "source": "profiles/traffic_h2_nscbc_mix_128.md (rocprofv3 --pmc, default tiles)"}
     out = {
         "metric": "Mpoint-RK-substages/s",
         "value": round(value, 3),
@@ -466,6 +477,7 @@ def main():
                      "mean_launch_ms": dom_ms},
         "valu_roofline": valu,
         "substage_roofline_frac": substage_bytes * value * 1e6 / HBM_PEAK,
+        "substage_hbm": hbm_sub,
         "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kern.items()},
         "tiles": {"narrow_pass_tiles": list(tuned[0]), "autotuned": not a.no_autotune},
     }
