@@ -23,7 +23,7 @@ program senga_gpu_driver
   real(c_double), allocatable, target :: run(:, :)
   real(c_double) :: etime, tstep, errold, errldr, t0, t1
   integer :: ierr, nargs, itime, irk, ns, npts, idflag, u
-  integer(c_int) :: rc
+  integer(c_int) :: rc, compiled
   integer(c_int8_t), pointer :: bytes(:)
   logical :: cfgonly
   integer(8) :: c0, c1, crate
@@ -89,6 +89,12 @@ contains
   if (rc /= SG_OK) call sgfail(h, rc)
 
   write(*, '(a, 3i6, a, i3, a, es12.5)') 'senga_gpu: grid', ctl%nx, ctl%ny, ctl%nz, ' species', ns, ' tstep', tstep
+  rc = sg_mechanism_compiled(h, compiled)
+  if (compiled /= 0) then
+    write(*, '(a)') 'senga_gpu: chemistry runs the compiled mechanism (unrolled step loop)'
+  else
+    write(*, '(a)') 'senga_gpu: chemistry runs the table-driven step loop'
+  end if
   idflag = 0
   call system_clock(c0, crate)
   do itime = ctl%ntime1, ctl%ntime1 + ctl%ntime - 1

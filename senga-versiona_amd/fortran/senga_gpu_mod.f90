@@ -81,6 +81,7 @@ module senga_gpu
   public :: sg_upload_state, sg_download_state, sg_set_timestep
   public :: sg_substage, sg_end_step, sg_step, sg_rhs, sg_derivatives
   public :: sg_sync, sg_last_error, sg_memory, sg_set_adaptive, sg_get_timestep
+  public :: sg_mechanism_compiled
 
   interface
     integer(c_int) function sg_abi_version() bind(C, name="sg_abi_version")
@@ -170,6 +171,11 @@ module senga_gpu
       import :: c_int, c_ptr, c_size_t
       type(c_ptr), value :: h
       integer(c_size_t), intent(out) :: bytes
+    end function
+    integer(c_int) function sg_mechanism_compiled(h, compiled) bind(C, name="sg_mechanism_compiled")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: compiled
     end function
   end interface
 end module senga_gpu

@@ -68,8 +68,12 @@ def test_fortran_driver_run_matches_library_and_oracle(tmp_path):
     r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
+    # the driver's configuration is the binding's byte for byte, so the H2
+    # mechanism it reads from chem.dat is the one compiled into the library
+    assert "compiled mechanism" in r.stdout
     d = read_dump(str(tmp_path / "output" / "dmpi0000000.dat"))
     s = Solver(c)
+    assert s.mechanism_compiled()
     s.set_timestep(c.dump.tstep)
     s.upload(c.dump.state, 0.0)
     s.step(steps)
