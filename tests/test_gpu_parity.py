@@ -69,7 +69,8 @@ def test_rhs_and_steps_match_reference(name):
 
 @pytest.mark.parametrize("kind,n,steps", [("periodic_1step", (32, 32, 32), 10),
                                           ("h2_nscbc_mix", (40, 24, 24), 4),
-                                          ("h2_nscbc_cle", (24, 24, 24), 4)])
+                                          ("h2_nscbc_cle", (24, 24, 24), 4),
+                                          ("h2_kernel", (24, 20, 16), 4)])
 def test_multistep_against_oracle(kind, n, steps):
     """Larger grids than the fixtures: GPU vs the oracle after several steps."""
     from pyoracle import Oracle
@@ -79,6 +80,12 @@ def test_multistep_against_oracle(kind, n, steps):
         c = C.periodic_1step(n=n, nsteps=steps)
     elif kind == "h2_nscbc_mix":
         c = C.h2_nscbc(n=n, length=(4e-3, 2.4e-3, 2.4e-3), mixavg=True, nsteps=steps)
+    elif kind == "h2_kernel":
+        # triply periodic H2-air flame kernel (hot spot in turbulence), mixture-averaged:
+        # the periodic-x pass and final kernel with the compiled mechanism
+        spec = C.kernel_specs(1, seed=13)[0]
+        spec["n"] = n
+        c = C.make_case(spec)
     else:
         c = C.h2_nscbc(n=n, length=(2.4e-3, 2.4e-3, 2.4e-3), mixavg=False, nsteps=steps)
     o = Oracle(c)
