@@ -1908,14 +1908,14 @@ struct FinalTile {
 template <bool BND, int NS, class R>
 __device__ __forceinline__ void final_point(const Params &P, const Fields &F, const sg_config *__restrict__ c,
                                             int i, int j, int k, int mode, double lhs, double rhsc, double erc,
-                                            double *__restrict__ out, R uxy, R vxy, R uxz, R wxz, R vyz, R wyz) {
+                                            double *__restrict__ out, R uxy, R vxy, R uxz, R wxz, R vyz, R wyz,
+                                            R u, R v, R w) {
   const long idx = ((long)k * P.ny + j) * P.nx + i;
   const long fs = P.fstride;
   constexpr int ES = (int)sizeof(R);      // bytes per element of the FP32/FP64 fields
   const uint32_t bo = (uint32_t)idx * (uint32_t)ES, bo8 = (uint32_t)idx * 8u;
   const R *pr = (const R *)F.prim;
-  const R u = fld(pr + P_U * fs, bo), v = fld(pr + P_V * fs, bo), w = fld(pr + P_W * fs, bo),
-          mu = fld(pr + P_MU * fs, bo);
+  const R mu = fld(pr + P_MU * fs, bo);   // u, v, w come from the staged tile
   const R *G = (const R *)F.grad;
   auto gr = [&](int q) { return fld(G + q * fs, bo); };
   const R *A = (const R *)F.acc;
@@ -2131,13 +2131,16 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
   auto mix_yz = [&](int B) {
     return dmix<false, R>(P, S0, 1, 2, [&](int k1, int k2) { return tile[B + (k1 - 8) * GX + (k2 - 8) * GX * GY]; });
   };
-  R uxy[NPT], uxz[NPT], vxy[NPT], vyz[NPT], wxz[NPT], wyz[NPT];
+  // the derivatives, and each point's own u, v, w read back from the tile
+  // (the values a global load would give) for the per-point part
+  R uxy[NPT], uxz[NPT], vxy[NPT], vyz[NPT], wxz[NPT], wyz[NPT], uc[NPT], vc[NPT], wc[NPT];
   fetch(pr + P_U * fs);
   store();
   __syncthreads();
   fetch(pr + P_V * fs);
 #pragma unroll
   for (int p = 0; p < NPT; p++) {
+    uc[p] = tile[base(p)];
     uxy[p] = mix_x(base(p), 1);
     uxz[p] = mix_x(base(p), 2);
     pin(uxy[p]); pin(uxz[p]);
@@ -2148,6 +2151,7 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
   fetch(pr + P_W * fs);
 #pragma unroll
   for (int p = 0; p < NPT; p++) {
+    vc[p] = tile[base(p)];
     vxy[p] = mix_x(base(p), 1);
     vyz[p] = mix_yz(base(p));
     pin(vxy[p]); pin(vyz[p]);
@@ -2157,6 +2161,7 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < NPT; p++) {
+    wc[p] = tile[base(p)];
     wxz[p] = mix_x(base(p), 2);
     wyz[p] = mix_yz(base(p));
     pin(wxz[p]); pin(wyz[p]);
@@ -2166,7 +2171,7 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
     const int k = z0 + lz0 + p * FT::ZS;
     if (i < nx && j < ny && k < nz)
       final_point<BND, NS, R>(P, F, c, i, j, k, mode, lhs, rhsc, erc, out, uxy[p], vxy[p], uxz[p], wxz[p],
-                              vyz[p], wyz[p]);
+                              vyz[p], wyz[p], uc[p], vc[p], wc[p]);
   }
 }
 
