@@ -36,9 +36,9 @@
 extern "C" {
 #endif
 
-#define SG_ABI_VERSION 1
+#define SG_ABI_VERSION 2
 
-#define SG_MAX_SPEC 15   /* interval indices pack 15 species per int (NSPIMX) */
+#define SG_MAX_SPEC 18   /* the reference's largest shipped mechanism (casefiles/68step: NSPCMX=18) */
 #define SG_MAX_STEP 128
 #define SG_MAX_SSL 10    /* NSSMAX: species per step list */
 #define SG_MAX_BODY 16
@@ -61,6 +61,18 @@ extern "C" {
 #define SG_BC_INFLOW3 13  /* NSBCI3: subsonic reflecting inflow, specified density */
 #define SG_BC_OUTFLOW1 21 /* NSBCO1: subsonic non-reflecting outflow */
 
+/* sg_config.physics bits: the reference's optional physics switches from
+ * cont.dat (src/contin.f:194-209), set when the switch is on -- the four
+ * molecular-transport options only together with mixture-averaged transport,
+ * as src/diffin.f:66,384-407 applies them.  This library computes none of
+ * them: sg_create and sg_footprint return SG_EINVAL when any bit is set, so
+ * that such a case is refused instead of run without that physics. */
+#define SG_PHYS_MIXW 1    /* NFMIXW: molar-mass diffusion term (src/rhscal.f:951,1970) */
+#define SG_PHYS_MIXP 2    /* NFMIXP: pressure diffusion term (src/rhscal.f:973,2365) */
+#define SG_PHYS_SORET 4   /* NFMSOR: thermal diffusion of light species (src/rhscal.f:2760) */
+#define SG_PHYS_DUFOUR 8  /* NFMDUF: Dufour heat flux (src/rhscal.f:1552,2834,3108) */
+#define SG_PHYS_RADN 16   /* NFRADN: optically thin radiation (src/rhscal.f:810, src/radcal.f) */
+
 typedef struct sg_config {
   /* grid: global = local on one device (src/com_senga2.h:12-34) */
   int32_t nx, ny, nz;
@@ -71,7 +83,7 @@ typedef struct sg_config {
                                  assembly; FP64 RK registers, temperature
                                  Newton iteration, enthalpies and rate
                                  exponents */
-  int32_t reserved0;
+  int32_t physics;            /* SG_PHYS_* bits of switches that are on (all refused) */
   double xgdlen, ygdlen, zgdlen;  /* domain lengths; spacing L/(N-1) (src/dfinit.f:42-50) */
 
   /* boundary conditions, faces xl, xr, yl, yr, zl, zr (cont.dat lines 51-56) */
@@ -148,9 +160,10 @@ size_t sg_config_size(void);
  * stencil coefficients (src/dfinit.f), the BC flags and loop ranges
  * (src/bcinit.f), allocates the device registers.  Replaces the table set-up
  * part of INDATA/DFINIT/BCINIT.  Returns SG_EINVAL for a species count that is
- * not compiled in, nx/ny/nz < 11, more than 2^29 points per field, cp fits of
- * more than 5 coefficients (src/temper.f NCTMAX) or transport fits of more
- * than SG_NTRC coefficients. */
+ * not compiled in (2..SG_MAX_SPEC are), nx/ny/nz < 11, more than 2^29 points
+ * per field, cp fits of more than 5 coefficients (src/temper.f NCTMAX),
+ * transport fits of more than SG_NTRC coefficients, any SG_PHYS_* bit, or a
+ * step list, species index, third-body or falloff index out of range. */
 int sg_create(const sg_config *cfg, sg_handle **out);
 int sg_destroy(sg_handle *h);
 

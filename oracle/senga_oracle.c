@@ -19,12 +19,25 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define HALO 5
 #define NBOUND 1
 #define NPERI 2
 
 /* numbers as src/com_senga2.h:268-274 */
+/* Every loop nest over points below computes each point from inputs that
+   the nest does not write (no sums across points), so its outer loop runs
+   in parallel when built with OpenMP and the results do not depend on the
+   thread count. */
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define OMP_FOR
+#endif
+
 static const double ZERO = 0.0, ONE = 1.0, TWO = 2.0, THREE = 3.0, HALF = 0.5;
 
 struct orc {
@@ -59,7 +72,7 @@ struct orc {
   double *rhs;   /* big, 5+ns fields */
   /* workspace */
   double *trun, *prun, *transp, *store7, *utmp, *vtmp, *wtmp, *wmomix, *difmix;
-  uint32_t *itndex;
+  uint32_t *itndex;   /* ITNDEX: NINTMX = 2 words per point (see itint_get) */
   double *st1, *st2, *st3, *st4, *st5, *st6, *divm, *ucor, *vcor, *wcor;
   double *rate, *rrte;
   /* face arrays (ny*nz, and *ns for species) */
@@ -206,6 +219,7 @@ static void dfbyd(orc_t *o, int dir, const double *f, double *out) {
   int n = N[dir], L = n - 1;
   int bl = o->nend[2 * dir] == NBOUND, br = o->nend[2 * dir + 1] == NBOUND;
   const double *a = o->ac;
+  OMP_FOR
   for (int k = 0; k < o->nz; k++)
     for (int j = 0; j < o->ny; j++)
       for (int i = 0; i < o->nx; i++) {
@@ -266,6 +280,7 @@ static void d2fd2(orc_t *o, int dir, const double *f, double *out) {
   int n = N[dir], L = n - 1;
   int bl = o->nend[2 * dir] == NBOUND, br = o->nend[2 * dir + 1] == NBOUND;
   const double *a = o->sc;
+  OMP_FOR
   for (int k = 0; k < o->nz; k++)
     for (int j = 0; j < o->ny; j++)
       for (int i = 0; i < o->nx; i++) {
@@ -339,6 +354,7 @@ static void d2fdmix(orc_t *o, int d1, int d2, const double *f, double *out) {
   int L = N[d1] - 1;
   int bl = o->nend[2 * d1] == NBOUND, br = o->nend[2 * d1 + 1] == NBOUND;
   const double *a = o->mc;
+  OMP_FOR
   for (int k = 0; k < o->nz; k++)
     for (int j = 0; j < o->ny; j++)
       for (int i = 0; i < o->nx; i++) {
@@ -428,7 +444,16 @@ static void zero_flags(orc_t *o, const int *flag, double *ax, double *ay, double
 /* ------------------------------------------------------------------------ */
 /* point physics                                                            */
 
-static inline int itint_of(uint32_t idx, int isp) { return (int)((idx >> (2 * isp)) & 3u); }
+/* ITNDEX packing, src/com_senga2.h:88-98: base NTBASE = 4 digits, NSPIMX = 15
+   species per 32-bit integer, NINTMX = 2 integers per point; species isp is
+   digit isp % 15 of integer isp / 15. */
+#define NSPIMX 15
+static inline int itint_get(const uint32_t *w, int isp) {
+  return (int)((w[isp / NSPIMX] >> (2 * (isp % NSPIMX))) & 3u);
+}
+static inline void itint_put(uint32_t *w, int isp, int it) {
+  w[isp / NSPIMX] += (uint32_t)it << (2 * (isp % NSPIMX));
+}
 
 /* interval search: first interval whose upper limit is >= T */
 static inline int locate(const sg_config *c, int isp, double t) {
@@ -441,6 +466,8 @@ static inline int locate(const sg_config *c, int isp, double t) {
 static int temper(orc_t *o) {
   const sg_config *c = &o->c;
   const double toltmp = (double)1.0e-10f;   /* REAL literal, src/temper.f:33-34 */
+  int fail = 0;
+  OMP_FOR
   for (int k = o->tlo[2]; k <= o->thi[2]; k++)
     for (int j = o->tlo[1]; j <= o->thi[1]; j++)
       for (int i = o->tlo[0]; i <= o->thi[0]; i++) {
@@ -451,9 +478,9 @@ static int temper(orc_t *o) {
         tc[0] = HALF * ukuk - R(o, 4)[b];
         for (int q = 1; q <= 5; q++) { tc[q] = ZERO; td[q] = ZERO; }
         double s7 = ZERO;
-        uint32_t idx = o->itndex[b];
+        const uint32_t *idx = o->itndex + 2 * b;
         for (int isp = 0; isp < o->ns; isp++) {
-          int it = itint_of(idx, isp);
+          int it = itint_get(idx, isp);
           double y = R(o, 5 + isp)[b];
           tc[0] = tc[0] + y * c->amascp[isp][it][c->ncenth[isp][it] - 1];
           tc[1] = tc[1] + y * c->amasct[isp][it][0];
@@ -465,7 +492,7 @@ static int temper(orc_t *o) {
           s7 = s7 + y * c->rgspec[isp];
         }
         double t = o->trun[b];
-        int iter = 1;
+        int iter = 1, bad = 0;
         for (;;) {
           double fp = tc[5], dp = td[5];
           for (int q = 4; q >= 1; q--) {
@@ -480,32 +507,42 @@ static int temper(orc_t *o) {
               iter++;
               continue;
             }
-            return SG_ETEMPER;
+            bad = 1;   /* the reference stops here (src/temper.f:156-167) */
           }
           break;
         }
+        if (bad) {
+#ifdef _OPENMP
+#pragma omp atomic write
+#endif
+          fail = 1;
+          continue;
+        }
         o->trun[b] = t;
-        uint32_t nidx = 0;
+        uint32_t nidx[2] = {0u, 0u};
         double cp = ZERO;
         for (int isp = 0; isp < o->ns; isp++) {
           int it = locate(c, isp, t);
-          nidx += (uint32_t)it << (2 * isp);
+          itint_put(nidx, isp, it);
           double f = c->amascp[isp][it][c->ncpoly[isp][it] - 1];
           for (int q = c->ncpoly[isp][it] - 2; q >= 0; q--) f = f * t + c->amascp[isp][it][q];
           cp = cp + R(o, 5 + isp)[b] * f;
         }
-        o->itndex[b] = nidx;
+        o->itndex[2 * b] = nidx[0];
+        o->itndex[2 * b + 1] = nidx[1];
         o->transp[b] = cp / d;
         o->store7[b] = s7;
         o->prun[b] = t * s7;
       }
-  return 0;
+  return fail ? SG_ETEMPER : 0;
 }
 
 /* TEMPIN, src/tempin.f:55-512: bisection from TRIN in 50 K steps */
 static int tempin(orc_t *o) {
   const sg_config *c = &o->c;
   const double toltmp = 1.0e-4, tininc = 5.0e1, tlimlo = 2.0e2, tlimhi = 3.0e3;
+  int fail = 0;
+  OMP_FOR
   for (int k = o->tlo[2]; k <= o->thi[2]; k++)
     for (int j = o->tlo[1]; j <= o->thi[1]; j++)
       for (int i = o->tlo[0]; i <= o->thi[0]; i++) {
@@ -514,7 +551,7 @@ static int tempin(orc_t *o) {
         double ukuk = (ru * ru + rv * rv + rw * rw) / d;
         double tuk2me = HALF * ukuk - R(o, 4)[b];
         double tlower = tlimlo, tupper = tlimhi, t = c->trin;
-        int fnconv = 1;
+        int fnconv = 1, bad = 0;
 #define RESID(tt, res)                                                          \
   do {                                                                          \
     double tc_[6];                                                              \
@@ -544,7 +581,8 @@ static int tempin(orc_t *o) {
             if (fabs(res) < toltmp) { fnconv = 0; break; }
             if (res < ZERO) {
               if (t < tlimhi) continue;
-              return SG_ETEMPIN;
+              bad = 1;   /* the reference stops (src/tempin.f:238,348) */
+              break;
             }
             tupper = t;
             break;
@@ -557,11 +595,19 @@ static int tempin(orc_t *o) {
             if (fabs(res) < toltmp) { fnconv = 0; break; }
             if (res > ZERO) {
               if (t > tlimlo) continue;
-              return SG_ETEMPIN;
+              bad = 1;   /* the reference stops (src/tempin.f:238,348) */
+              break;
             }
             tlower = t;
             break;
           }
+        }
+        if (bad) {
+#ifdef _OPENMP
+#pragma omp atomic write
+#endif
+          fail = 1;
+          continue;
         }
         if (fnconv) {
           for (;;) {
@@ -574,22 +620,23 @@ static int tempin(orc_t *o) {
         }
 #undef RESID
         o->trun[b] = t;
-        uint32_t nidx = 0;
+        uint32_t nidx[2] = {0u, 0u};
         double s7 = ZERO, cp = ZERO;
         for (int isp = 0; isp < o->ns; isp++) {
           int it = locate(c, isp, t);
-          nidx += (uint32_t)it << (2 * isp);
+          itint_put(nidx, isp, it);
           double f = c->amascp[isp][it][c->ncpoly[isp][it] - 1];
           for (int q = c->ncpoly[isp][it] - 2; q >= 0; q--) f = f * t + c->amascp[isp][it][q];
           cp = cp + R(o, 5 + isp)[b] * f;
           s7 = s7 + R(o, 5 + isp)[b] * c->rgspec[isp];
         }
-        o->itndex[b] = nidx;
+        o->itndex[2 * b] = nidx[0];
+        o->itndex[2 * b + 1] = nidx[1];
         o->transp[b] = cp / d;
         o->store7[b] = s7;
         o->prun[b] = t * s7;
       }
-  return 0;
+  return fail ? SG_ETEMPIN : 0;
 }
 
 /* CHRATE, src/chrate.f:146-760, interior; yrhs holds rho Y */
@@ -597,12 +644,13 @@ static void chrate(orc_t *o) {
   const sg_config *c = &o->c;
   const double ysmall = 1.0e-30, ydenom = 1.0e-15;   /* src/chrate.f:39-40 */
   int ns = o->ns;
+  OMP_FOR
   for (int k = 0; k < o->nz; k++)
     for (int j = 0; j < o->ny; j++)
       for (int i = 0; i < o->nx; i++) {
         size_t b = BI(o, i, j, k), p = II(o, i, j, k);
         double t = o->trun[b];
-        uint32_t idx = o->itndex[b];
+        const uint32_t *idx = o->itndex + 2 * b;
         double rate[SG_MAX_SPEC];
         for (int s = 0; s < ns; s++) rate[s] = ZERO;
         for (int st = 0; st < c->nstep; st++) {
@@ -693,7 +741,7 @@ static void chrate(orc_t *o) {
           if (c->mglist[st] != 0) {
             for (int q = 0; q < c->nsslen[st]; q++) {
               int s = c->nsspec[st][q];
-              int it = itint_of(idx, s);
+              int it = itint_get(idx, s);
               double g = c->amolgb[s][it][c->ncpoly[s][it] - 1];
               for (int cc = c->ncpoly[s][it] - 2; cc >= 0; cc--) g = c->amolgb[s][it][cc] + g * t;
               g = c->amolgb[s][it][c->ncenth[s][it] - 1] / t -
@@ -730,15 +778,15 @@ static void chrate(orc_t *o) {
 /* ------------------------------------------------------------------------ */
 /* RHSCAL, src/rhscal.f:61-4047                                             */
 
-#define FOR_I(o) \
+#define FOR_I(o) OMP_FOR \
   for (int k = 0; k < (o)->nz; k++) for (int j = 0; j < (o)->ny; j++) for (int i = 0; i < (o)->nx; i++)
-#define FOR_B(o)                                   \
+#define FOR_B(o) OMP_FOR                           \
   for (int k = (o)->blo[2]; k <= (o)->bhi[2]; k++) \
     for (int j = (o)->blo[1]; j <= (o)->bhi[1]; j++) for (int i = (o)->blo[0]; i <= (o)->bhi[0]; i++)
-#define FOR_T(o)                                   \
+#define FOR_T(o) OMP_FOR                           \
   for (int k = (o)->tlo[2]; k <= (o)->thi[2]; k++) \
     for (int j = (o)->tlo[1]; j <= (o)->thi[1]; j++) for (int i = (o)->tlo[0]; i <= (o)->thi[0]; i++)
-#define FOR_FACE(o) for (int k = 0; k < (o)->nz; k++) for (int j = 0; j < (o)->ny; j++)
+#define FOR_FACE(o) OMP_FOR for (int k = 0; k < (o)->nz; k++) for (int j = 0; j < (o)->ny; j++)
 
 static int rhscal(orc_t *o) {
   const sg_config *c = &o->c;
@@ -943,7 +991,7 @@ static int rhscal(orc_t *o) {
     /* species enthalpy, rhscal.f:1444-1470 */
     FOR_B(o) {
       size_t b = BI(o, i, j, k);
-      int it = itint_of(o->itndex[b], s);
+      int it = itint_get(o->itndex + 2 * b, s);
       double t = o->trun[b];
       double fornow = c->amasch[s][it][c->ncpoly[s][it] - 1];
       for (int q = c->ncpoly[s][it] - 2; q >= 0; q--) fornow = fornow * t + c->amasch[s][it][q];
@@ -1471,6 +1519,7 @@ static void lincom(orc_t *o, int irk) {
   for (int v = 0; v < nv; v++) {
     int slot = v < 5 ? v : 5;
     double *r = R(o, v), *u = RUN(o, v), *e = ERR(o, v);
+    OMP_FOR
     for (int k = 0; k < o->nz; k++)
       for (int j = 0; j < o->ny; j++)
         for (int i = o->rklo[slot]; i <= o->rkhi[slot]; i++) {
@@ -1489,6 +1538,7 @@ static void fincom(orc_t *o) {
   for (int v = 0; v < nv; v++) {
     int slot = v < 5 ? v : 5;
     double *r = R(o, v), *u = RUN(o, v), *e = ERR(o, v);
+    OMP_FOR
     for (int k = 0; k < o->nz; k++)
       for (int j = 0; j < o->ny; j++)
         for (int i = o->rklo[slot]; i <= o->rkhi[slot]; i++) {
@@ -1531,7 +1581,7 @@ orc_t *orc_create(const sg_config *cfg, int *err) {
   double **big[] = {&o->trun, &o->prun, &o->transp, &o->store7, &o->utmp, &o->vtmp, &o->wtmp,
                     &o->wmomix, &o->difmix};
   for (size_t q = 0; q < sizeof(big) / sizeof(big[0]); q++) *big[q] = (double *)calloc(o->nb, sizeof(double));
-  o->itndex = (uint32_t *)calloc(o->nb, sizeof(uint32_t));
+  o->itndex = (uint32_t *)calloc(2 * o->nb, sizeof(uint32_t));
   double **in[] = {&o->st1, &o->st2, &o->st3, &o->st4, &o->st5, &o->st6, &o->divm, &o->ucor,
                    &o->vcor, &o->wcor};
   for (size_t q = 0; q < sizeof(in) / sizeof(in[0]); q++) *in[q] = (double *)calloc(o->ni, sizeof(double));
@@ -1720,7 +1770,7 @@ int orc_get_stage(orc_t *o, double *stage) {
 }
 
 int orc_get_itndex(orc_t *o, int *itndex) {
-  FOR_I(o) itndex[II(o, i, j, k)] = (int)o->itndex[BI(o, i, j, k)];
+  FOR_I(o) itndex[II(o, i, j, k)] = (int)o->itndex[2 * BI(o, i, j, k)];
   return 0;
 }
 
@@ -1741,4 +1791,14 @@ int orc_derivatives(orc_t *o, const double *f, double *out9) {
   return 0;
 }
 
-int orc_threads(int n) { return n > 0 ? 1 : 1; }
+/* Threads of the parallel loops (OpenMP builds): n > 0 sets the count;
+   returns the count in use (1 without OpenMP). */
+int orc_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}

@@ -43,7 +43,8 @@ int orc_get(orc_t *o, double *run, double *temp, double *pres, double *rate);
 /* stage register (DRHS..YRHS interior) */
 int orc_get_stage(orc_t *o, double *stage);
 int orc_derivatives(orc_t *o, const double *f, double *out9);
-/* initial interval index from the last TEMPIN/TEMPER (interior, packed) */
+/* interval indices of species 1-15 from the last TEMPIN/TEMPER: the first
+ * ITNDEX word (interior, packed as src/com_senga2.h:88-98) */
 int orc_get_itndex(orc_t *o, int *itndex);
 int orc_threads(int n);
 

@@ -16,7 +16,7 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "lib", "libsenga_gpu.so")
 HIPCC = "/opt/rocm/bin/hipcc"
-SPECIES = list(range(2, 13))
+SPECIES = list(range(2, 19))
 # Packed FP32 instructions are disabled for all device code: some of their forms have
 # given wrong results on this hardware, and unpacked FP32 keeps the arithmetic exact.
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -71,7 +71,7 @@ def build_oracle():
     src = os.path.join(ROOT, "oracle", "senga_oracle.c")
     out = os.path.join(ROOT, "oracle", "lib", "liboracle.so")
     os.makedirs(os.path.dirname(out), exist_ok=True)
-    _run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
+    _run(["gcc", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
           src, "-o", out, "-lm"])
     return out
 

@@ -493,9 +493,36 @@ static int batch_fence(sg_batch *b, sg_handle *h) {
   return 0;
 }
 
+// Every list length and index the kernels dereference: a list of step st
+// holds at most SG_MAX_SSL species, each a 0-based species index; third-body
+// indices are 1-based below SG_MAX_BODY, falloff indices 1-based below
+// SG_MAX_FALL (0 = none).
+static bool mechanism_in_range(const sg_config *c) {
+  const int ns = c->nspec;
+  auto list_ok = [&](int len, const int32_t *sp) {
+    if (len < 0 || len > SG_MAX_SSL) return false;
+    for (int q = 0; q < len; q++)
+      if (sp[q] < 0 || sp[q] >= ns) return false;
+    return true;
+  };
+  for (int st = 0; st < c->nstep; st++) {
+    if (!list_ok(c->nsslen[st], c->nsspec[st]) || !list_ok(c->nrslen[st], c->nrspec[st]) ||
+        !list_ok(c->npslen[st], c->npspec[st]) || !list_ok(c->nrclen[st], c->nrcpec[st]) ||
+        !list_ok(c->npclen[st], c->npcpec[st]))
+      return false;
+    if (c->mblist[st] < 0 || c->mblist[st] > SG_MAX_BODY) return false;
+    if (c->mllist[st] < 0 || c->mllist[st] > SG_MAX_FALL || c->mtlist[st] < 0 || c->mtlist[st] > SG_MAX_FALL ||
+        c->mslist[st] < 0 || c->mslist[st] > SG_MAX_FALL)
+      return false;
+  }
+  return true;
+}
+
 static int validate_config(const sg_config *cfg) {
   if (!species_supported(cfg->nspec)) return SG_EINVAL;   // compiled species counts
+  if (cfg->physics != 0) return SG_EINVAL;                // optional physics not computed here
   if (cfg->nstep < 0 || cfg->nstep > SG_MAX_STEP) return SG_EINVAL;
+  if (!mechanism_in_range(cfg)) return SG_EINVAL;
   if (cfg->nx < 11 || cfg->ny < 11 || cfg->nz < 11) return SG_EINVAL;
   if ((long)cfg->nx * cfg->ny * 8 >= (1L << 31)) return SG_EINVAL;
   if (cfg->precision != 0 && cfg->precision != 1) return SG_EINVAL;

@@ -6,7 +6,8 @@
 
 namespace sg {
 
-#define SG_NS_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+#define SG_NS_LIST(X) \
+  X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
 
 template <int NS> void launch_tempin_t(const Params &, const sg_config *, const Fields &, hipStream_t);
 template <int NS> void launch_boundt_t(const Launch &, hipStream_t);

@@ -69,7 +69,11 @@ __device__ __forceinline__ void coords(const Params &P, long idx, int &i, int &j
   i = r - j * P.nx;
 }
 
-__device__ __forceinline__ int itint_of(uint32_t idx, int s) { return (int)((idx >> (2 * s)) & 3u); }
+// Packed temperature-interval indices: one bit per species (the reference's
+// ITNDEX packing, src/com_senga2.h:88-98, with number base NTBASE = 2, which
+// holds the two intervals every species has here), species s in bit s.
+static_assert(SG_NTINT == 2 && SG_MAX_SPEC <= 32, "one index bit per species in a 32-bit word");
+__device__ __forceinline__ int itint_of(uint32_t idx, int s) { return (int)((idx >> s) & 1u); }
 
 __device__ __forceinline__ int locate(const sg_config *c, int s, double t) {
   int it = 0;
@@ -205,7 +209,7 @@ __global__ void __launch_bounds__(BLK) k_tempin(Params P, const sg_config *__res
   }
   uint32_t nidx = 0;
 #pragma unroll
-  for (int s = 0; s < NS; s++) nidx += (uint32_t)locate(c, s, t) << (2 * s);
+  for (int s = 0; s < NS; s++) nidx += (uint32_t)locate(c, s, t) << s;
   F.itndex[idx] = nidx;
   R *pr = (R *)F.prim;
   pr[P_T * fs + idx] = (R)t;
@@ -349,10 +353,15 @@ __device__ __forceinline__ double trfit(const double *a, double x) {
 // arithmetic", ACM TOMS 15(2), 1989): x = (64 m + j) ln2/64 + r, |r| <= ln2/128,
 // exp(x) = 2^m 2^(j/64) exp(r), with exp(r) - 1 from its degree-6 Taylor
 // polynomial (truncation < 3e-20).  n ln2/64 is split so that n * L1 is exact.
-// xt is the 2^(j/64) table (kExp2j) staged in LDS.  Within 1 ulp of a correctly
-// rounded exp over the whole double range (measured against glibc on 1.2e8
-// arguments); overflow gives inf, underflow 0, NaN propagates.  15 VALU
-// instructions and one LDS read, against ~22 for the library exp.
+// xt is the 2^(j/64) table (kExp2j) staged in LDS.  Domain: finite x with
+// |x| < 2^31 ln2/64 (about 2.3e7), where n fits the 32-bit exponent split.
+// There it is within 1 ulp of a correctly rounded exp (measured against glibc
+// on 1.2e8 arguments in [-745, 709.7]), gives inf above 709.78 and 0 below
+// -745.2; infinite arguments (which would give NaN) are outside the domain,
+// and the kernels never pass them: every call's argument is a rate exponent,
+// a transport fit or a falloff term of a temperature that TEMPER bounds.
+// NaN propagates.  15 VALU instructions and one LDS read, against ~22 for the
+// library exp.
 static constexpr double kExp2j[64] = {0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0, 0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0, 0x1.172b83c7d517bp+0, 0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0, 0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0, 0x1.2d285a6e4030bp+0, 0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0, 0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0, 0x1.3dea64c123422p+0, 0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0, 0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0, 0x1.56f4736b527dap+0, 0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0, 0x1.6247eb03a5585p+0, 0x1.6623882552225p+0, 0x1.6a09e667f3bcdp+0, 0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0, 0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0, 0x1.868d99b4492edp+0, 0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0, 0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0, 0x1.9c49182a3f090p+0, 0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0, 0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0, 0x1.bcc1e904bc1d2p+0, 0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0, 0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0, 0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0, 0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0};
 __device__ __forceinline__ double exp_t(double x, const double *xt) {
   const double INV = 0x1.71547652b82fep+6, L1 = 0x1.62e42ff000000p-7, L2 = -0x1.718432a1b0e26p-41;
@@ -671,7 +680,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   for (int s = 0; s < NS; s++) {
     int it = locate(c, s, t);
     itn[s] = it;
-    nidx += (uint32_t)it << (2 * s);
+    nidx += (uint32_t)it << s;
     cp = cp + ry[s] * horner(tb->cp[s], it, t);
   }
   cp = cp * ovd;
@@ -1077,7 +1086,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   for (int s = 0; s < NS; s++) {
     int it = locate(c, s, t);
     itn[s] = it;
-    nidx += (uint32_t)it << (2 * s);
+    nidx += (uint32_t)it << s;
     cp = cp + ry[s] * horner(tb->cp[s], it, t);
   }
   cp = cp * ovd;

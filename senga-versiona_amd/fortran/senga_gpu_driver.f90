@@ -66,7 +66,7 @@ contains
 
   subroutine run_case()
 
-  if (sg_abi_version() /= 1 .or. sg_config_size() /= c_sizeof(cfg)) call fail('library ABI mismatch')
+  if (sg_abi_version() /= 2 .or. sg_config_size() /= c_sizeof(cfg)) call fail('library ABI mismatch')
   ns = cfg%nspec
   npts = ctl%nx * ctl%ny * ctl%nz
   allocate(run(npts, 5 + ns))
@@ -74,9 +74,10 @@ contains
   call read_dump(trim(rundir) // '/output/dmpi0000000.dat', ctl%ndifmt == 1, ctl%nx, ctl%ny, ctl%nz, ns, &
                  run, etime, tstep, errold, errldr, ierr)
   if (ierr /= 0) call fail('cannot read output/dmpi0000000.dat')
-  ! On restart the reference keeps cont.dat's TSTEP through the REAL variable
-  ! TSOLD (src/indata.f:1420-1421,1476), i.e. rounded to single precision.
-  tstep = real(real(ctl%tstep, c_float), c_double)
+  ! On restart with a fixed step the reference keeps cont.dat's TSTEP through
+  ! the REAL variable TSOLD, i.e. rounded to single precision; with the
+  ! adaptive step it keeps the dump's TSTEP (src/indata.f:1423,1481).
+  if (ctl%nstpsw == 0) tstep = real(real(ctl%tstep, c_float), c_double)
 
   rc = sg_create(cfg, h)
   if (rc /= SG_OK) call fail('sg_create failed (unsupported configuration or no device)')

@@ -12,19 +12,22 @@ module senga_gpu
   implicit none
   private
 
-  integer, parameter, public :: SG_MAX_SPEC = 15, SG_MAX_STEP = 128, SG_MAX_SSL = 10
+  integer, parameter, public :: SG_MAX_SPEC = 18, SG_MAX_STEP = 128, SG_MAX_SSL = 10
   integer, parameter, public :: SG_MAX_BODY = 16, SG_MAX_FALL = 32
   integer, parameter, public :: SG_NCOF = 7, SG_NTINT = 2, SG_NTRC = 4
   integer, parameter, public :: SG_OK = 0, SG_EINVAL = 1, SG_ENOMEM = 2, SG_EDEVICE = 3
   integer, parameter, public :: SG_ETEMPER = 4, SG_ETEMPIN = 5, SG_ESTATE = 6
   integer, parameter, public :: SG_BC_PERIODIC = 1, SG_BC_INFLOW3 = 13, SG_BC_OUTFLOW1 = 21
+  ! sg_config%physics bits: optional physics switched on in cont.dat (all refused)
+  integer, parameter, public :: SG_PHYS_MIXW = 1, SG_PHYS_MIXP = 2, SG_PHYS_SORET = 4
+  integer, parameter, public :: SG_PHYS_DUFOUR = 8, SG_PHYS_RADN = 16
 
   type, bind(C), public :: sg_config_t
     integer(c_int32_t) :: nx, ny, nz
     integer(c_int32_t) :: nspec, nstep
     integer(c_int32_t) :: mixavg
     integer(c_int32_t) :: precision
-    integer(c_int32_t) :: reserved0
+    integer(c_int32_t) :: physics
     real(c_double) :: xgdlen, ygdlen, zgdlen
     integer(c_int32_t) :: bc_type(6)
     integer(c_int32_t) :: bc_ipar(4, 6)

@@ -32,7 +32,9 @@ module senga_io
     real(c_double) :: yrin(SG_MAX_SPEC)
     integer :: bctype(6), bcipar(4, 6)
     real(c_double) :: bcrpar(4, 6)
-    integer :: nfmavt
+    ! molecular transport and radiation switches (src/contin.f:194-209)
+    integer :: nfmavt, nfmixw, nfmixp, nfmsor, nfmduf, nfradn
+    real(c_double) :: wmltdr, trefrn
   end type control_t
 
   public :: read_cont, read_chem, read_diff, fill_config, zero_config
@@ -56,7 +58,15 @@ contains
     real(c_double) :: sparam(4)
     character(len=64) :: line
     ctl%yrin = 0.0d0
+    ! defaults of src/contin.f:194-201 when the block is absent
     ctl%nfmavt = 0
+    ctl%nfmixw = 0
+    ctl%nfmixp = 0
+    ctl%nfmsor = 0
+    ctl%nfmduf = 0
+    ctl%wmltdr = 0.0d0
+    ctl%nfradn = 0
+    ctl%trefrn = 3.0d0
     open(newunit=u, file=path, status='old', form='formatted', iostat=ierr)
     if (ierr /= 0) return
     do i = 1, 7
@@ -108,8 +118,13 @@ contains
     read(u, *)
     read(u, '(a)', iostat=ierr) line
     if (ierr == 0 .and. line(1:32) == 'Molecular transport control data') then
+      ! src/contin.f:203-210: switches, three label lines, radiation switch
       read(u, *)
-      read(u, *) ctl%nfmavt
+      read(u, *) ctl%nfmavt, ctl%nfmixw, ctl%nfmixp, ctl%nfmsor, ctl%nfmduf, ctl%wmltdr
+      read(u, *)
+      read(u, *)
+      read(u, *)
+      read(u, *) ctl%nfradn, ctl%trefrn
     end if
     ierr = 0
     close(u)
@@ -125,7 +140,7 @@ contains
     integer :: u, ns, nst, i, j, k, isp, ist, n, it, nc, icp, nbody, ngibb, nlind, ntroe, nsrif
     character(len=32) :: sym
     real(c_double) :: prefgb, giblet, clewis(SG_MAX_SPEC), tlo, thi
-    real(c_double) :: amolcp(SG_NCOF, SG_NTINT, SG_MAX_SPEC), diffmu_r
+    real(c_double) :: amolcp(SG_NCOF, SG_NTINT, SG_MAX_SPEC), diffmu_r, rdat(12)
     integer :: ncofcp(SG_NTINT, SG_MAX_SPEC)
     open(newunit=u, file=path, status='old', form='formatted', iostat=ierr)
     if (ierr /= 0) return
@@ -133,7 +148,7 @@ contains
       read(u, *)
     end do
     read(u, '(I5)') ns
-    if (ns > SG_MAX_SPEC) then
+    if (ns < 1 .or. ns > SG_MAX_SPEC) then
       ierr = 1
       close(u)
       return
@@ -148,11 +163,16 @@ contains
     ncofcp = 0
     do i = 1, ns
       read(u, '(I5,1PE12.4)') isp, tlo
+      if (isp < 1 .or. isp > ns) goto 900
       cfg%wmolar(isp) = tlo
       read(u, '(1PE12.4)') clewis(isp)
-      read(u, '(I5)') cfg%ntint(isp)
+      read(u, '(I5)') n
+      if (n < 1 .or. n > SG_NTINT) goto 900
+      cfg%ntint(isp) = n
       do it = 1, cfg%ntint(isp)
-        read(u, '(2(1PE12.4),I5)') tlo, thi, ncofcp(it, isp)
+        read(u, '(2(1PE12.4),I5)') tlo, thi, n
+        if (n < 1 .or. n > SG_NCOF) goto 900
+        ncofcp(it, isp) = n
         cfg%tinthi(it, isp) = thi
         do icp = 1, ncofcp(it, isp)
           read(u, '(1PE15.7)') amolcp(icp, it, isp)
@@ -161,7 +181,7 @@ contains
     end do
     read(u, *)
     read(u, '(I5)') nst
-    if (nst > SG_MAX_STEP) then
+    if (nst < 0 .or. nst > SG_MAX_STEP) then
       ierr = 1
       close(u)
       return
@@ -169,46 +189,64 @@ contains
     cfg%nstep = nst
     do j = 1, nst
       read(u, '(I5,3(1PE12.4))') ist, tlo, thi, diffmu_r
+      if (ist < 1 .or. ist > nst) goto 900
       cfg%rparam(:, ist) = [tlo, thi, diffmu_r]
     end do
     ! species, reactant and product lists
     read(u, *)
     do j = 1, nst
-      read(u, '(2I5)') ist, cfg%nsslen(ist)
+      read(u, '(2I5)') ist, n
+      if (ist < 1 .or. ist > nst .or. n < 0 .or. n > SG_MAX_SSL) goto 900
+      cfg%nsslen(ist) = n
       do k = 1, cfg%nsslen(ist)
         read(u, '(2I5,I8)') ist, i, n
+        if (ist < 1 .or. ist > nst .or. i < 1 .or. i > SG_MAX_SSL .or. n < 1 .or. n > ns) goto 900
         cfg%nsspec(i, ist) = n - 1
       end do
     end do
     read(u, *)
     do j = 1, nst
-      read(u, '(2I5)') ist, cfg%nrslen(ist)
+      read(u, '(2I5)') ist, n
+      if (ist < 1 .or. ist > nst .or. n < 0 .or. n > SG_MAX_SSL) goto 900
+      cfg%nrslen(ist) = n
       do k = 1, cfg%nrslen(ist)
         read(u, '(2I5,I8)') ist, i, n
+        if (ist < 1 .or. ist > nst .or. i < 1 .or. i > SG_MAX_SSL .or. n < 1 .or. n > ns) goto 900
         cfg%nrspec(i, ist) = n - 1
       end do
     end do
     read(u, *)
     do j = 1, nst
-      read(u, '(2I5)') ist, cfg%npslen(ist)
+      read(u, '(2I5)') ist, n
+      if (ist < 1 .or. ist > nst .or. n < 0 .or. n > SG_MAX_SSL) goto 900
+      cfg%npslen(ist) = n
       do k = 1, cfg%npslen(ist)
         read(u, '(2I5,I8)') ist, i, n
+        if (ist < 1 .or. ist > nst .or. i < 1 .or. i > SG_MAX_SSL .or. n < 1 .or. n > ns) goto 900
         cfg%npspec(i, ist) = n - 1
       end do
     end do
     read(u, *)
     do j = 1, nst
-      read(u, '(2I5)') ist, cfg%nrclen(ist)
+      read(u, '(2I5)') ist, n
+      if (ist < 1 .or. ist > nst .or. n < 0 .or. n > SG_MAX_SSL) goto 900
+      cfg%nrclen(ist) = n
       do k = 1, cfg%nrclen(ist)
-        read(u, '(2I5,I8,1PE12.4)') ist, i, n, cfg%crspec(i, ist)
+        read(u, '(2I5,I8,1PE12.4)') ist, i, n, diffmu_r
+        if (ist < 1 .or. ist > nst .or. i < 1 .or. i > SG_MAX_SSL .or. n < 1 .or. n > ns) goto 900
+        cfg%crspec(i, ist) = diffmu_r
         cfg%nrcpec(i, ist) = n - 1
       end do
     end do
     read(u, *)
     do j = 1, nst
-      read(u, '(2I5)') ist, cfg%npclen(ist)
+      read(u, '(2I5)') ist, n
+      if (ist < 1 .or. ist > nst .or. n < 0 .or. n > SG_MAX_SSL) goto 900
+      cfg%npclen(ist) = n
       do k = 1, cfg%npclen(ist)
-        read(u, '(2I5,I8,1PE12.4)') ist, i, n, cfg%cpspec(i, ist)
+        read(u, '(2I5,I8,1PE12.4)') ist, i, n, diffmu_r
+        if (ist < 1 .or. ist > nst .or. i < 1 .or. i > SG_MAX_SSL .or. n < 1 .or. n > ns) goto 900
+        cfg%cpspec(i, ist) = diffmu_r
         cfg%npcpec(i, ist) = n - 1
       end do
     end do
@@ -216,25 +254,31 @@ contains
     do j = 1, nst
       do k = 1, cfg%nsslen(j)
         read(u, '(2I5,F5.1)') ist, i, diffmu_r
+        if (ist < 1 .or. ist > nst .or. i < 1 .or. i > SG_MAX_SSL) goto 900
         cfg%diffmu(i, ist) = diffmu_r
       end do
     end do
     ! third bodies
     read(u, *)
     read(u, '(I5)') nbody
+    if (nbody < 0 .or. nbody > SG_MAX_BODY) goto 900
     do i = 1, nbody
       read(u, '(I5,3X,A)') n, sym
     end do
     read(u, *)
     if (nbody > 0) then
       do j = 1, nst
-        read(u, '(I5,I8)') ist, cfg%mblist(ist)
+        read(u, '(I5,I8)') ist, n
+        if (ist < 1 .or. ist > nst .or. n < 0 .or. n > nbody) goto 900
+        cfg%mblist(ist) = n
       end do
     end if
     read(u, *)
     do k = 1, nbody
       do i = 1, ns
-        read(u, '(2I5,1PE12.4)') n, isp, cfg%effy3b(isp, n)
+        read(u, '(2I5,1PE12.4)') n, isp, diffmu_r
+        if (n < 1 .or. n > nbody .or. isp < 1 .or. isp > ns) goto 900
+        cfg%effy3b(isp, n) = diffmu_r
       end do
     end do
     ! Gibbs, Lindemann, Troe, SRI step lists and data
@@ -242,43 +286,60 @@ contains
     read(u, '(I5)') ngibb
     if (ngibb > 0) then
       do j = 1, nst
-        read(u, '(I5,I8)') ist, cfg%mglist(ist)
+        read(u, '(I5,I8)') ist, n
+        if (ist < 1 .or. ist > nst) goto 900
+        cfg%mglist(ist) = n
       end do
     end if
     read(u, *)
     read(u, '(I5)') nlind
+    if (nlind < 0 .or. nlind > SG_MAX_FALL) goto 900
     if (nlind > 0) then
       do j = 1, nst
-        read(u, '(I5,I8)') ist, cfg%mllist(ist)
+        read(u, '(I5,I8)') ist, n
+        if (ist < 1 .or. ist > nst .or. n < 0 .or. n > nlind) goto 900
+        cfg%mllist(ist) = n
       end do
     end if
     read(u, *)
     do k = 1, nlind
-      read(u, '(I5,4(1PE12.4))') n, cfg%rclind(:, n)
+      read(u, '(I5,4(1PE12.4))') n, rdat(1:4)
+      if (n < 1 .or. n > nlind) goto 900
+      cfg%rclind(:, n) = rdat(1:4)
     end do
     read(u, *)
     read(u, '(I5)') ntroe
+    if (ntroe < 0 .or. ntroe > SG_MAX_FALL) goto 900
     if (ntroe > 0) then
       do j = 1, nst
-        read(u, '(I5,I8)') ist, cfg%mtlist(ist)
+        read(u, '(I5,I8)') ist, n
+        if (ist < 1 .or. ist > nst .or. n < 0 .or. n > ntroe) goto 900
+        cfg%mtlist(ist) = n
       end do
     end if
     read(u, *)
     do k = 1, ntroe
-      read(u, '(I5,6(1PE12.4))') n, cfg%rctroe(1:6, n)
-      read(u, '(I5,6(1PE12.4))') n, cfg%rctroe(7:12, n)
+      read(u, '(I5,6(1PE12.4))') n, rdat(1:6)
+      read(u, '(I5,6(1PE12.4))') n, rdat(7:12)
+      if (n < 1 .or. n > ntroe) goto 900
+      cfg%rctroe(:, n) = rdat(1:12)
     end do
     read(u, *)
     read(u, '(I5)') nsrif
+    if (nsrif < 0 .or. nsrif > SG_MAX_FALL) goto 900
     if (nsrif > 0) then
       do j = 1, nst
-        read(u, '(I5,I8)') ist, cfg%mslist(ist)
+        read(u, '(I5,I8)') ist, n
+        if (ist < 1 .or. ist > nst .or. n < 0 .or. n > nsrif) goto 900
+        cfg%mslist(ist) = n
       end do
     end if
     read(u, *)
     do k = 1, nsrif
-      read(u, '(I5,3(1PE12.4))') n, cfg%rcsrif(1:3, n)
-      read(u, '(I5,5(1PE12.4))') n, cfg%rcsrif(4:8, n)
+      read(u, '(I5,3(1PE12.4))') n, rdat(1:3)
+      read(u, '(I5,5(1PE12.4))') n, rdat(4:8)
+      if (n < 1 .or. n > nsrif) goto 900
+      cfg%rcsrif(:, n) = rdat(1:8)
     end do
     close(u)
 
@@ -341,6 +402,10 @@ contains
     cfg%prantl = PRANTL
     cfg%clnten = log(1.0d1)                 ! src/indata.f:1071
     ierr = 0
+    return
+    ! a step, list position, species, third-body or falloff index out of range
+900 ierr = 2
+    close(u)
   end subroutine read_chem
 
   ! ---------------------------------------------------------------- diff.dat
@@ -425,6 +490,17 @@ contains
     cfg%ny = ctl%ny
     cfg%nz = ctl%nz
     cfg%mixavg = merge(1, 0, ctl%nfmavt == 1)
+    ! optional physics that is switched on: the molecular-transport options
+    ! act only with mixture-averaged transport (src/diffin.f:66,384-407).
+    ! sg_create refuses any of them.
+    cfg%physics = 0
+    if (ctl%nfmavt == 1) then
+      if (ctl%nfmixw == 1) cfg%physics = ior(cfg%physics, SG_PHYS_MIXW)
+      if (ctl%nfmixp == 1) cfg%physics = ior(cfg%physics, SG_PHYS_MIXP)
+      if (ctl%nfmsor == 1) cfg%physics = ior(cfg%physics, SG_PHYS_SORET)
+      if (ctl%nfmduf == 1) cfg%physics = ior(cfg%physics, SG_PHYS_DUFOUR)
+    end if
+    if (ctl%nfradn == 1) cfg%physics = ior(cfg%physics, SG_PHYS_RADN)
     cfg%precision = 0
     cfg%xgdlen = ctl%xgdlen
     cfg%ygdlen = ctl%ygdlen

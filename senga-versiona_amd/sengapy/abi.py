@@ -13,7 +13,7 @@ import os
 
 import numpy as np
 
-MAX_SPEC = 15
+MAX_SPEC = 18
 MAX_STEP = 128
 MAX_SSL = 10
 MAX_BODY = 16
@@ -33,7 +33,7 @@ i32 = C.c_int32
 class Config(C.Structure):
     _fields_ = [
         ("nx", i32), ("ny", i32), ("nz", i32), ("nspec", i32), ("nstep", i32),
-        ("mixavg", i32), ("precision", i32), ("reserved0", i32),
+        ("mixavg", i32), ("precision", i32), ("physics", i32),
         ("xgdlen", d), ("ygdlen", d), ("zgdlen", d),
         ("bc_type", i32 * 6), ("bc_ipar", (i32 * 4) * 6), ("bc_rpar", (d * 4) * 6),
         ("prin", d), ("trin", d), ("urin", d), ("vrin", d), ("wrin", d),
@@ -113,6 +113,24 @@ def fill_chemistry(c: Config, t) -> None:
     c.clnten = math.log(1.0e1)          # src/indata.f:1071
 
 
+# sg_config.physics bits (include/senga_gpu.h SG_PHYS_*)
+PHYS_MIXW, PHYS_MIXP, PHYS_SORET, PHYS_DUFOUR, PHYS_RADN = 1, 2, 4, 8, 16
+
+
+def physics_bits(ctl) -> int:
+    """SG_PHYS_* bits of the optional physics switches of cont.dat that are on
+    (src/contin.f:194-209): the molecular-transport options count only with
+    mixture-averaged transport on, as src/diffin.f:66,384-407 applies them."""
+    bits = 0
+    if ctl.nfmavt == 1:
+        bits |= PHYS_MIXW if ctl.nfmixw == 1 else 0
+        bits |= PHYS_MIXP if ctl.nfmixp == 1 else 0
+        bits |= PHYS_SORET if ctl.nfmsor == 1 else 0
+        bits |= PHYS_DUFOUR if ctl.nfmduf == 1 else 0
+    bits |= PHYS_RADN if ctl.nfradn == 1 else 0
+    return bits
+
+
 def config_from_case(case, precision=0) -> Config:
     """Fill sg_config from a Case (control + derived tables); precision 0 is
     FP64, 1 the mixed FP32/FP64 mode."""
@@ -124,6 +142,7 @@ def config_from_case(case, precision=0) -> Config:
     c.nx, c.ny, c.nz = ctl.nx, ctl.ny, ctl.nz
     c.nspec, c.nstep = ns, nst
     c.mixavg = 1 if ctl.nfmavt == 1 else 0
+    c.physics = physics_bits(ctl)
     c.precision = precision
     c.xgdlen, c.ygdlen, c.zgdlen = ctl.xgdlen, ctl.ygdlen, ctl.zgdlen
     for f, (typ, ip, rp) in enumerate(ctl.bc):

@@ -101,58 +101,86 @@ class Server:
         res.finished = clock() - t0
         return res
 
+    def _admit(self, waiting, busy, mem, pts, t0, clock):
+        """Admit waiting requests first come first served while they fit the
+        budgets: each gets its handle, its state uploaded and its initial
+        temperature solved on its own stream.  busy: cases already in flight
+        (a request above the budgets still runs when nothing else does)."""
+        out = []
+        while waiting:
+            r = waiting[0]
+            need = abi.footprint(r.case, self.precision, self.lib)
+            busy_now = busy or bool(out)
+            if busy_now and mem + need > self.hbm_budget:
+                break
+            if busy_now and pts + _points(r.case) > self.point_budget:
+                break
+            waiting.popleft()
+            s = abi.Solver(r.case, self.lib, self.precision)
+            s.set_timestep(r.case.dump.tstep)
+            s.upload(r.case.dump.state, 0.0)
+            live = _Live(r, s, clock() - t0, need)
+            out.append(live)
+            mem += need
+            pts += live.points
+        return out, mem, pts
+
     def run(self, requests: List[Request], clock=time.perf_counter) -> List[Result]:
+        """Serve the requests.  Each chunk of k RK steps is queued for the
+        batch; while it computes, the cases that left at the previous
+        boundary download their results and newly arrived requests are
+        admitted and uploaded on their own streams.  They join the batch at
+        the next boundary."""
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         waiting: deque = deque()
         active: List[_Live] = []
+        staged: List[_Live] = []     # admitted and uploaded, joining at the next boundary
         retired: List[_Live] = []
         results: List[Result] = []
         batch = abi.Batch(self.lib)
         mem = pts = 0
         t0 = clock()
+
+        def arrivals():
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+
         try:
-            while pending or waiting or active or retired:
-                now = clock() - t0
-                while pending and pending[0].arrival <= now:
-                    waiting.append(pending.popleft())
-                # admission, first come first served
-                while waiting:
-                    r = waiting[0]
-                    need = abi.footprint(r.case, self.precision, self.lib)
-                    busy = bool(active)
-                    if mem + need > self.hbm_budget and busy:
-                        break
-                    if busy and pts + _points(r.case) > self.point_budget:
-                        break
-                    waiting.popleft()
-                    s = abi.Solver(r.case, self.lib, self.precision)
-                    s.set_timestep(r.case.dump.tstep)
-                    s.upload(r.case.dump.state, 0.0)
-                    live = _Live(r, s, clock() - t0, need)
-                    batch.add(s)
-                    active.append(live)
-                    mem += need
-                    pts += live.points
-                self.max_points_in_flight = max(self.max_points_in_flight, pts)
-                self.max_bytes_in_flight = max(self.max_bytes_in_flight, mem)
-                if not active:
+            while pending or waiting or active or staged or retired:
+                arrivals()
+                if not active and not staged:
+                    # idle GPU: results still held, then admit (blocking)
                     for live in retired:
                         results.append(self._finish(live, t0, clock))
                         live.solver.close()
                         mem -= live.bytes
                     retired = []
-                    if pending and not waiting:
-                        time.sleep(max(0.0, pending[0].arrival - (clock() - t0)))
-                    continue
+                    staged, mem, pts = self._admit(waiting, False, mem, pts, t0, clock)
+                    if not staged:
+                        if pending:
+                            time.sleep(max(0.0, pending[0].arrival - (clock() - t0)))
+                        continue
+                for live in staged:
+                    batch.add(live.solver)
+                    active.append(live)
+                staged = []
+                self.max_points_in_flight = max(self.max_points_in_flight, pts)
+                self.max_bytes_in_flight = max(self.max_bytes_in_flight, mem)
                 k = min(self.chunk, min(x.left for x in active))
                 batch.step(k)
-                # results of the cases that left at the last boundary come
-                # back while the batch computes
+                # while the batch computes: results of the cases that left at
+                # the last boundary come back, and new arrivals upload
                 for live in retired:
                     results.append(self._finish(live, t0, clock))
                 done_now = retired
                 retired = []
+                arrivals()
+                staged, mem, pts = self._admit(waiting, True, mem, pts, t0, clock)
+                self.max_points_in_flight = max(self.max_points_in_flight, pts)
+                self.max_bytes_in_flight = max(self.max_bytes_in_flight, mem)
                 batch.sync()
+                # freeing device memory synchronises the device: after the chunk
                 for live in done_now:
                     live.solver.close()
                     mem -= live.bytes
@@ -164,7 +192,7 @@ class Server:
                     pts -= live.points
                     retired.append(live)
         finally:
-            for live in active + retired:
+            for live in active + staged + retired:
                 live.solver.close()
             batch.close()
         results.sort(key=lambda r: r.rid)

@@ -36,7 +36,7 @@ def test_exports_every_declared_symbol(lib):
 
 
 def test_abi_version_and_layout(lib):
-    assert lib.sg_abi_version() == 1
+    assert lib.sg_abi_version() == 2
     assert lib.sg_config_size() == C.sizeof(abi.Config)
 
 
@@ -57,6 +57,65 @@ def test_rejects_unsupported_configs(lib):
     bad.precision = 3
     assert lib.sg_create(C.byref(bad), C.byref(h)) == 1
     assert lib.sg_create(None, C.byref(h)) == 1
+
+
+@pytest.mark.parametrize("switch,bit", [("nfmixw", abi.PHYS_MIXW), ("nfmixp", abi.PHYS_MIXP),
+                                        ("nfmsor", abi.PHYS_SORET), ("nfmduf", abi.PHYS_DUFOUR),
+                                        ("nfradn", abi.PHYS_RADN)])
+def test_rejects_each_optional_physics_switch(lib, switch, bit, tmp_path):
+    """Soret, Dufour, the molar-mass and pressure diffusion terms and radiation
+    (cont.dat switches, src/contin.f:194-209) are not computed here: a case
+    that switches one on is refused by sg_create and sg_footprint instead of
+    run without it.  The switch reaches sg_config through both readers."""
+    c = CS.h2_nscbc(n=(12, 11, 11), length=(1e-3, 1e-3, 1e-3), mixavg=True)
+    setattr(c.control, switch, 1)
+    p = tmp_path / "cont.dat"
+    c.control.write(str(p))
+    r = cont.read_cont(str(p))
+    assert getattr(r, switch) == 1
+    cfg = abi.config_from_case(c)
+    assert cfg.physics == bit
+    h = C.c_void_p()
+    out = C.c_size_t(0)
+    assert lib.sg_create(C.byref(cfg), C.byref(h)) == 1
+    assert lib.sg_footprint(C.byref(cfg), C.byref(out)) == 1
+    cfg.physics = 0
+    assert lib.sg_footprint(C.byref(cfg), C.byref(out)) == 0
+
+
+def test_transport_options_need_mixture_averaging():
+    """The four molecular-transport options act only with mixture-averaged
+    transport on (src/diffin.f:66,384-407); radiation acts either way."""
+    c = CS.h2_nscbc(n=(12, 11, 11), length=(1e-3, 1e-3, 1e-3), mixavg=False)
+    c.control.nfmsor = c.control.nfmduf = c.control.nfmixw = c.control.nfmixp = 1
+    assert abi.physics_bits(c.control) == 0
+    c.control.nfradn = 1
+    assert abi.physics_bits(c.control) == abi.PHYS_RADN
+
+
+@pytest.mark.parametrize("field,value", [
+    ("nsslen", 11), ("nrslen", -1), ("npslen", 11), ("nrclen", 11), ("npclen", 11),
+    ("nsspec", 9), ("nrspec", -1), ("npspec", 9), ("nrcpec", 9), ("npcpec", 9),
+    ("mblist", 17), ("mllist", 33), ("mtlist", -1), ("mslist", 33)])
+def test_rejects_out_of_range_mechanism_tables(lib, field, value):
+    """Every list length and index the kernels dereference is checked before
+    the tables go to the device: a malformed configuration gets SG_EINVAL."""
+    c = CS.h2_nscbc(n=(12, 11, 11), length=(1e-3, 1e-3, 1e-3), mixavg=True)
+    cfg = abi.config_from_case(c)
+    out = C.c_size_t(0)
+    assert lib.sg_footprint(C.byref(cfg), C.byref(out)) == 0
+    st = 4
+    arr = getattr(cfg, field)
+    if field.endswith("len") or field.endswith("list"):
+        arr[st] = value
+    else:
+        lenf = {"nsspec": "nsslen", "nrspec": "nrslen", "npspec": "npslen", "nrcpec": "nrclen",
+                "npcpec": "npclen"}[field]
+        getattr(cfg, lenf)[st] = max(getattr(cfg, lenf)[st], 1)
+        arr[st][0] = value
+    assert lib.sg_footprint(C.byref(cfg), C.byref(out)) == 1
+    h = C.c_void_p()
+    assert lib.sg_create(C.byref(cfg), C.byref(h)) == 1
 
 
 def test_chem_dat_round_trip(tmp_path):

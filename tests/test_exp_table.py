@@ -45,7 +45,8 @@ int main(void) {
       if (d < 0) d = -d;
       if (d > worst) worst = d;
     }
-  printf("%%lld %%.17g %%.17g %%.17g\n", (long long)worst, exp_t(0.0), exp_t(-800.0), exp_t(1.0));
+  printf("%%lld %%.17g %%.17g %%.17g %%.17g %%.17g %%.17g %%.17g\n", (long long)worst, exp_t(0.0), exp_t(-800.0),
+         exp_t(1.0), exp_t(709.8), exp_t(-1.0e7), exp_t(1.0e7), exp_t(-745.2));
   return 0;
 }
 """
@@ -71,3 +72,6 @@ def test_exp_t_within_one_ulp(tmp_path):
     worst, e0, eu, e1 = int(out[0]), float(out[1]), float(out[2]), float(out[3])
     assert worst <= 1, f"exp_t is {worst} ulp from exp"
     assert e0 == 1.0 and eu == 0.0 and abs(e1 - 2.718281828459045) < 1e-15
+    # the ends of the domain |x| < 2^31 ln2/64: overflow to inf, underflow to 0
+    eo, em, ep, el = (float(v) for v in out[4:8])
+    assert eo == float("inf") and ep == float("inf") and em == 0.0 and el == 0.0
