@@ -42,6 +42,8 @@ class Oracle:
         lib.orc_destroy.argtypes = [C.c_void_p]
         lib.orc_set_adaptive.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
         lib.orc_get_timestep.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3
+        lib.orc_threads.argtypes = [C.c_int]
+        lib.orc_threads.restype = C.c_int
         self.lib = lib
         self.case = case
         self.cfg = config_from_case(case)
@@ -60,6 +62,11 @@ class Oracle:
     def _chk(self, rc, what):
         if rc != 0:
             raise RuntimeError(f"{what} failed: {rc}")
+
+    def threads(self, n=0):
+        """OpenMP threads of the oracle's point loops: n > 0 sets the count
+        (process-wide); returns the count in use."""
+        return int(self.lib.orc_threads(int(n)))
 
     def set_state(self, run, etime=0.0):
         run = np.ascontiguousarray(run, dtype=np.float64)

@@ -13,6 +13,9 @@ C           RHSVEL,BOUNDS at RK substage 1), no register update
 C       2 = NTIME full RK steps, final solution register
 C       3 = the nine derivative operators applied to opin.bin
 C       4 = one substage (mode 1 plus LINCOM at substage 1)
+C       5 = mode 2 with OUTPUT called after every step as
+C           src/senga2.f:186 does (report, statistics, out*.res
+C           post-processing files and dumps in output/)
 C     refrun.ctl line 2: output file name
 C     -----------------------------------------------------------------
       INCLUDE 'com_senga2.h'
@@ -67,7 +70,7 @@ C     TEMPIN RESULTS (INITIAL TEMPERATURE AND INTERVAL INDEX)
      +             YRHS(1:NX,1:NY,1:NZ,1:NSPEC)
         ENDIF
 
-      ELSEIF(MODE.EQ.2)THEN
+      ELSEIF(MODE.EQ.2.OR.MODE.EQ.5)THEN
 
         DO JTIME = NTIME1,NTIME2
           ITIME = JTIME
@@ -90,6 +93,7 @@ C     TEMPIN RESULTS (INITIAL TEMPERATURE AND INTERVAL INDEX)
           ETIME = ETIME + TSTEP
           CALL BOUNTT
           CALL ADAPTT
+          IF(MODE.EQ.5)CALL OUTPUT
         ENDDO
         WRITE(92)DRUN(1:NX,1:NY,1:NZ),URUN(1:NX,1:NY,1:NZ),
      +           VRUN(1:NX,1:NY,1:NZ),WRUN(1:NX,1:NY,1:NZ),

@@ -42,6 +42,9 @@ def build_for(case):
 
 
 def run(case, mode, nsteps=None, opin=None, keep=None):
+    """Mode 5 (steps with OUTPUT) also returns the run's output files:
+    "rept" and "stat" (text of output/rept.res and output/stat.res) and
+    "outres" ({file name: raw bytes} of output/out*.res)."""
     exe = build_for(case)
     d = tempfile.mkdtemp(prefix="refrun_")
     try:
@@ -55,11 +58,20 @@ def run(case, mode, nsteps=None, opin=None, keep=None):
         if r.returncode != 0 or not os.path.exists(os.path.join(d, "out.bin")):
             raise RuntimeError(f"refrun failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
         raw = open(os.path.join(d, "out.bin"), "rb").read()
+        files = {}
+        if mode == 5:
+            od = os.path.join(d, "output")
+            files["rept"] = open(os.path.join(od, "rept.res")).read()
+            files["stat"] = open(os.path.join(od, "stat.res")).read()
+            files["outres"] = {f: open(os.path.join(od, f), "rb").read()
+                               for f in sorted(os.listdir(od)) if f.startswith("out")}
         if keep:
             shutil.copy(os.path.join(d, "out.bin"), keep)
     finally:
         shutil.rmtree(d, ignore_errors=True)
-    return parse(raw)
+    out = parse(raw)
+    out.update(files)
+    return out
 
 
 def parse(raw):
@@ -86,7 +98,7 @@ def parse(raw):
         if mode == 4:
             out["run"] = take(n * (5 + ns)).reshape(5 + ns, nz, ny, nx)
             out["stage"] = take(n * (5 + ns)).reshape(5 + ns, nz, ny, nx)
-    elif mode == 2:
+    elif mode in (2, 5):
         out["run"] = take(n * (5 + ns)).reshape(5 + ns, nz, ny, nx)
         out["T"] = take(n).reshape(nz, ny, nx)
         out["P"] = take(n).reshape(nz, ny, nx)

@@ -5,12 +5,17 @@
 !   src/chemin.f, src/diffin.f); restart from output/dmpi0000000.dat
 !   (src/indata.f:1419-1475); for each step five sg_substage calls in place of
 !   BOUNDT..LINCOM/FINCOM and sg_end_step in place of BOUNTT; dumps every NTDUMP
-!   steps, alternating output/dmpi0000000.dat and output/dmpi0000001.dat as
-!   src/output.f:273-335 does.
+!   steps, alternating output/dmpi0000001.dat and output/dmpi0000000.dat (the
+!   first dump does not overwrite the restart file) as src/output.f:273-335 does.
+!
+! The run writes output/rept.res, output/stat.res and, on every dump step,
+! output/outIIIIIPPPP.res as src/output.f:81-99,241-265,343-352,397-425 do.
 !
 ! Usage:  senga_gpu [rundir]          run the case in rundir (default '.')
 !         senga_gpu --config rundir out   write the sg_config built from rundir's
 !                                         inputs to file out and stop (no GPU use)
+!         senga_gpu --convert-dump in out  read a formatted dump and write it
+!                                         unformatted (no GPU use)
 program senga_gpu_driver
   use, intrinsic :: iso_c_binding
   use senga_gpu
@@ -20,13 +25,15 @@ program senga_gpu_driver
   type(sg_config_t), target :: cfg
   type(c_ptr) :: h
   character(len=512) :: rundir, arg, outcfg
-  real(c_double), allocatable, target :: run(:, :)
+  real(c_double), allocatable, target :: run(:, :), tfld(:), pfld(:), rate(:, :)
   real(c_double) :: etime, tstep, errold, errldr, t0, t1
   integer :: ierr, nargs, itime, irk, ns, npts, idflag, u
   integer(c_int) :: rc, compiled
   integer(c_int8_t), pointer :: bytes(:)
   logical :: cfgonly
   integer(8) :: c0, c1, crate
+  integer :: dnx, dny, dnz, dns
+  character(len=32) :: fname
 
   nargs = command_argument_count()
   cfgonly = .false.
@@ -37,6 +44,17 @@ program senga_gpu_driver
       cfgonly = .true.
       call get_command_argument(2, rundir)
       call get_command_argument(3, outcfg)
+    else if (trim(arg) == '--convert-dump') then
+      call get_command_argument(2, rundir)
+      call get_command_argument(3, outcfg)
+      call dump_header(trim(rundir), .true., dnx, dny, dnz, dns, ierr)
+      if (ierr /= 0) call fail('cannot read the formatted dump header')
+      allocate(run(dnx * dny * dnz, 5 + dns))
+      call read_dump(trim(rundir), .true., dnx, dny, dnz, dns, run, etime, tstep, errold, errldr, ierr)
+      if (ierr /= 0) call fail('cannot read the formatted dump')
+      call write_dump(trim(outcfg), .false., dnx, dny, dnz, dns, run, etime, tstep, errold, errldr, ierr)
+      if (ierr /= 0) call fail('cannot write the unformatted dump')
+      stop
     else
       rundir = arg
     end if
@@ -89,6 +107,9 @@ contains
   rc = sg_upload_state(h, run, etime)
   if (rc /= SG_OK) call sgfail(h, rc)
 
+  allocate(tfld(npts), pfld(npts), rate(npts, ns))
+  call report_open(trim(rundir) // '/output/rept.res', ctl, ns, tstep)
+  call stat_open(trim(rundir) // '/output/stat.res')
   write(*, '(a, 3i6, a, i3, a, es12.5)') 'senga_gpu: grid', ctl%nx, ctl%ny, ctl%nz, ' species', ns, ' tstep', tstep
   rc = sg_mechanism_compiled(h, compiled)
   if (compiled /= 0) then
@@ -96,7 +117,7 @@ contains
   else
     write(*, '(a)') 'senga_gpu: chemistry runs the table-driven step loop'
   end if
-  idflag = 0
+  idflag = 1   ! the first dump goes to dmpi0000001.dat (src/indata.f:127-134 leaves IDFLAG = 1)
   call system_clock(c0, crate)
   do itime = ctl%ntime1, ctl%ntime1 + ctl%ntime - 1
     do irk = 1, 5
@@ -107,14 +128,24 @@ contains
     if (rc /= SG_OK) call sgfail(h, rc)
     etime = etime + tstep
     rc = sg_get_timestep(h, tstep, errold, errldr)
+    ! OUTPUT (src/output.f): report, post-processing file and dump, statistics
+    if (mod(itime, ctl%ntrept) == 0) call report_step(trim(rundir) // '/output/rept.res', itime, etime, tstep)
     if (mod(itime, ctl%ntdump) == 0) then
-      rc = sg_download_state(h, c_loc(run), c_null_ptr, c_null_ptr, c_null_ptr)
+      rc = sg_download_state(h, c_loc(run), c_loc(tfld), c_loc(pfld), c_loc(rate))
       if (rc /= SG_OK) call sgfail(h, rc)
-      write(arg, '(a, i1, a)') '/output/dmpi000000', idflag, '.dat'
-      call write_dump(trim(rundir) // trim(arg), ctl%ndofmt == 1, ctl%nx, ctl%ny, ctl%nz, ns, run, &
+      write(fname, '(a, i5.5, i4.4, a)') 'output/out', itime / ctl%ntdump, 0, '.res'
+      call write_out_res(trim(rundir) // '/' // trim(fname), ctl%nx, ctl%ny, ctl%nz, ns, run, tfld, pfld, &
+                         rate, etime, ierr)
+      if (ierr /= 0) call fail('cannot write ' // trim(fname))
+      write(fname, '(a, i1, a)') 'output/dmpi000000', idflag, '.dat'
+      call write_dump(trim(rundir) // '/' // trim(fname), ctl%ndofmt == 1, ctl%nx, ctl%ny, ctl%nz, ns, run, &
                       etime, tstep, errold, errldr, ierr)
       if (ierr /= 0) call fail('cannot write dump')
+      call report_dump(trim(rundir) // '/output/rept.res', trim(fname))
       idflag = 1 - idflag
+    end if
+    if (ctl%ntstat > 0) then
+      if (mod(itime, ctl%ntstat) == 0) call stat_step(trim(rundir) // '/output/stat.res', itime)
     end if
   end do
   rc = sg_sync(h)

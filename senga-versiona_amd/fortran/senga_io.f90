@@ -7,7 +7,11 @@
 ! the restart dump dmpi in its unformatted and formatted forms
 ! (src/output.f:306-333, src/indata.f:1430-1475).  The edit descriptors are
 ! those of the file formats, so files written for the reference read here
-! unchanged.
+! unchanged.  The run's outputs follow src/output.f: the report file
+! output/rept.res (time step, elapsed time and next step every NTREPT steps,
+! one line per dump), the statistics file output/stat.res (the step number
+! every NTSTAT steps) and the post-processing file output/outIIIIIPPPP.res of
+! every dump step.
 module senga_io
   use, intrinsic :: iso_c_binding
   use senga_gpu
@@ -38,7 +42,8 @@ module senga_io
   end type control_t
 
   public :: read_cont, read_chem, read_diff, fill_config, zero_config
-  public :: read_dump, write_dump
+  public :: read_dump, write_dump, dump_header
+  public :: report_open, report_step, report_dump, stat_open, stat_step, write_out_res
 
 contains
 
@@ -572,5 +577,110 @@ contains
     close(u)
     ierr = 0
   end subroutine write_dump
+
+  ! Grid and species count of a dump, from its header (both forms).
+  subroutine dump_header(path, formatted, nx, ny, nz, ns, ierr)
+    character(len=*), intent(in) :: path
+    logical, intent(in) :: formatted
+    integer, intent(out) :: nx, ny, nz, ns, ierr
+    integer :: u
+    if (formatted) then
+      open(newunit=u, file=path, status='old', form='formatted', iostat=ierr)
+      if (ierr /= 0) return
+      read(u, *, iostat=ierr) nx, ny, nz, ns
+    else
+      open(newunit=u, file=path, status='old', form='unformatted', iostat=ierr)
+      if (ierr /= 0) return
+      read(u, iostat=ierr) nx, ny, nz, ns
+    end if
+    close(u)
+  end subroutine dump_header
+
+  ! ------------------------------------------------------------- run output
+  ! output/rept.res: created at start-up (src/indata.f:216-230 writes a
+  ! header); the loop appends to it (src/output.f:81-99, 343-352).
+  subroutine report_open(path, ctl, ns, tstep)
+    character(len=*), intent(in) :: path
+    type(control_t), intent(in) :: ctl
+    integer, intent(in) :: ns
+    real(c_double), intent(in) :: tstep
+    integer :: u
+    character(len=72) :: stars
+    stars = repeat('*', 72)
+    open(newunit=u, file=path, status='replace', form='formatted')
+    write(u, '(a)') stars
+    write(u, '(a)') '*  SENGA+ right-hand side on one MI355X (senga_gpu driver)'
+    write(u, '(a)') stars
+    write(u, *)
+    write(u, *) '==========='
+    write(u, *) 'REPORT FILE'
+    write(u, *) '==========='
+    write(u, *)
+    write(u, '(a, 3i6)') ' Global grid (nx,ny,nz): ', ctl%nx, ctl%ny, ctl%nz
+    write(u, '(a, 3(1pe12.4))') ' Domain size (m): ', ctl%xgdlen, ctl%ygdlen, ctl%zgdlen
+    write(u, '(a, i4)') ' Species: ', ns
+    write(u, '(a, 1pe12.4, a, i7, a, i7, a, i2)') ' Time step: ', tstep, '; first step', ctl%ntime1, &
+        '; steps', ctl%ntime, '; step switch', ctl%nstpsw
+    write(u, '(a, 6i4)') ' Boundary types: ', ctl%bctype
+    write(u, *)
+    close(u)
+  end subroutine report_open
+
+  ! every NTREPT steps: format 9100 and 9110 of src/output.f:440-441
+  subroutine report_step(path, itime, etime, tstep)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: itime
+    real(c_double), intent(in) :: etime, tstep
+    integer :: u
+    open(newunit=u, file=path, status='old', position='append', form='formatted')
+    write(u, '(''Time step number: '', i7)') itime
+    write(u, '(''Elapsed time: '', 1pe12.4, '';'', 2x, ''next time step:'', 1pe12.4)') etime, tstep
+    close(u)
+  end subroutine report_step
+
+  ! after each dump: format 9120 of src/output.f:442
+  subroutine report_dump(path, fname)
+    character(len=*), intent(in) :: path, fname
+    integer :: u
+    open(newunit=u, file=path, status='old', position='append', form='formatted')
+    write(u, '(''Dump completed: '', a)') fname
+    close(u)
+  end subroutine report_dump
+
+  ! output/stat.res: created empty at start-up (src/indata.f:1024-1026); the
+  ! step number appended every NTSTAT steps (src/output.f:397-425)
+  subroutine stat_open(path)
+    character(len=*), intent(in) :: path
+    integer :: u
+    open(newunit=u, file=path, status='replace', form='formatted')
+    close(u)
+  end subroutine stat_open
+
+  subroutine stat_step(path, itime)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: itime
+    integer :: u
+    open(newunit=u, file=path, status='old', position='append', form='formatted')
+    write(u, '(''Time step number: '', i7)') itime
+    close(u)
+  end subroutine stat_step
+
+  ! output/outIIIIIPPPP.res on every dump step (src/output.f:241-265): one
+  ! unformatted record of rho, u, v, w, E per unit mass (ERUN/DRUN), T, p,
+  ! Y_k, the reaction rates of the last right-hand side and ETIME.
+  subroutine write_out_res(path, nx, ny, nz, ns, run, t, p, rate, etime, ierr)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: nx, ny, nz, ns
+    real(c_double), intent(in) :: run(nx * ny * nz, 5 + ns), t(nx * ny * nz), p(nx * ny * nz)
+    real(c_double), intent(in) :: rate(nx * ny * nz, ns)
+    real(c_double), intent(in) :: etime
+    integer, intent(out) :: ierr
+    integer :: u, k
+    open(newunit=u, file=path, status='replace', form='unformatted', iostat=ierr)
+    if (ierr /= 0) return
+    write(u) run(:, 1), run(:, 2) / run(:, 1), run(:, 3) / run(:, 1), run(:, 4) / run(:, 1), &
+             run(:, 5) / run(:, 1), t, p, ((run(:, 5 + k) / run(:, 1)), k = 1, ns), rate, etime
+    close(u)
+  end subroutine write_out_res
 
 end module senga_io

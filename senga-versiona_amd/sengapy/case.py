@@ -6,7 +6,8 @@ Two case families match the workloads this library is measured on:
   Arrhenius chemistry (R -> P, Zeldovich number 6, heat-release parameter 0.75,
   unity Lewis number, Pr = 0.7), a planar flame slab in progress variable along
   x (two erf fronts so the profile is periodic) and seeded solenoidal
-  turbulence.  At 32^3 with L = 2*pi mm this is the CPU-runnable workload.
+  turbulence with u'/S_L = 4 and l/delta = 2 from the computed laminar flame.
+  At 32^3 with L = 2*pi mm this is the CPU-runnable workload.
 * ``h2_nscbc``: 9-species 21-step H2-air mechanism (the reference's
   input/chem.dat), NSCBC inflow (type 13, specified density/velocity/species) at
   x-left and partially non-reflecting outflow (type 21) at x-right, periodic in
@@ -143,26 +144,51 @@ def _fmt_roundtrip(m: _chem.Mechanism, path: str) -> _chem.Mechanism:
     return _chem.read_chem(path)
 
 
-def periodic_1step(n=(32, 32, 32), length=2.0e-3 * math.pi, seed=1, uprime=2.0,
+_FLAME_1STEP = {}
+
+
+def one_step_flame(mech=None, Tu=300.0):
+    """(S_L, thermal thickness delta, T_b) of the single-step mechanism's
+    planar laminar flame at Tu and 1 atm (sengapy.flame1d), cached."""
+    mech = mech or one_step_mechanism()
+    key = (tuple(mech.rparam[:, 0]), float(mech.amolcp[5, 0, 1]), Tu)
+    if key not in _FLAME_1STEP:
+        from . import flame1d
+        sl, delta, tb, _ = flame1d.single_step_flame(mech, Tu, npts=2001)
+        _FLAME_1STEP[key] = (sl, delta, tb)
+    return _FLAME_1STEP[key]
+
+
+def periodic_1step(n=(32, 32, 32), length=2.0e-3 * math.pi, seed=1, uprime_over_sl=4.0, l_over_delta=2.0,
                    workdir=None, cfl=0.5, nsteps=10) -> Case:
+    """The single-step flame slab in turbulence with u'/S_L and l/delta given
+    (defaults 4 and 2): S_L and the thermal thickness delta = (T_b - T_u) /
+    max dT/dx come from the mechanism's planar laminar flame (one_step_flame),
+    u' = sqrt(<u^2+v^2+w^2>/3) is set to uprime_over_sl * S_L, the spectrum's
+    peak wavelength so that the expected integral length scale of the discrete
+    field is l_over_delta * delta, and the erf fronts have the laminar
+    flame's thermal thickness (erf width delta / sqrt(pi))."""
     nx, ny, nz = n
     workdir = workdir or os.path.join(DATA, "_gen")
     os.makedirs(workdir, exist_ok=True)
     mech = _fmt_roundtrip(one_step_mechanism(), os.path.join(workdir, f"onestep_{nx}.chem.dat"))
     tab = _chem.derive(mech)
+    Tu, Tb = 300.0, 1200.0
+    sl, dth, _ = one_step_flame(mech, Tu)
     L = (length, length, length)
     d = tuple(L[i] / (n[i] - 1) for i in range(3))
     x = _ic.grid(nx, L[0])
     period = nx * d[0]
-    delta = 3.0 * d[0]
-    c = 0.5 * (np.vectorize(math.erf)((x - 0.3 * period) / delta)
-               - np.vectorize(math.erf)((x - 0.7 * period) / delta))
-    Tu, Tb = 300.0, 1200.0
+    width = dth / math.sqrt(math.pi)
+    c = 0.5 * (np.vectorize(math.erf)((x - 0.3 * period) / width)
+               - np.vectorize(math.erf)((x - 0.7 * period) / width))
     shape = (nz, ny, nx)
     C = np.broadcast_to(c, shape)
     T = Tu + (Tb - Tu) * C
     Y = [1.0 - C, C.copy()]
-    u, v, w = _ic.solenoidal_field(shape, (d[2], d[1], d[0]), uprime, 2.0 * 2.0 * delta, seed,
+    spacing = (d[2], d[1], d[0])
+    k0_len = _ic.k0_len_for_integral_scale(shape, spacing, l_over_delta * dth)
+    u, v, w = _ic.solenoidal_field(shape, spacing, uprime_over_sl * sl, k0_len, seed,
                                    spectrum="batchelor-townsend")
     st = _ic.conservative_state(tab, T, np.full(shape, 1.0e5), Y, u, v, w)
     dt = _f32(cfl * min(d) / sound_speed_max(tab, st, T))
@@ -201,9 +227,18 @@ def _h2_mixtures(mech, phi):
 
 def h2_nscbc(n=(128, 128, 128), length=(5.0e-3, 5.0e-3, 5.0e-3), seed=7, phi=0.7,
              uin=1.25, uprime_over_sl=5.0, sl=1.2, mixavg=True, cfl=0.5, nsteps=100,
-             periodic_x=False, radicals=True, Tu=300.0) -> Case:
+             periodic_x=False, radicals=True, Tu=300.0, falloff_variant=False) -> Case:
+    """falloff_variant: the mechanism of h2_falloff_mechanism() (Troe and SRI
+    steps) instead of the reference's H2-air mechanism."""
     nx, ny, nz = n
-    mech = h2_mechanism()
+    chem_path = os.path.join(DATA, "h2_9sp", "chem.dat")
+    if falloff_variant:
+        gen = os.path.join(DATA, "_gen")
+        os.makedirs(gen, exist_ok=True)
+        chem_path = os.path.join(gen, "h2_falloff.chem.dat")
+        mech = _fmt_roundtrip(h2_falloff_mechanism(), chem_path)
+    else:
+        mech = h2_mechanism()
     tab = _chem.derive(mech)
     dif = _diff.read_diff(os.path.join(DATA, "h2_9sp", "diff.dat"), mech.wmolar) if mixavg else None
     Yu, Yb, idx = _h2_mixtures(mech, phi)
@@ -248,8 +283,9 @@ def h2_nscbc(n=(128, 128, 128), length=(5.0e-3, 5.0e-3, 5.0e-3), seed=7, phi=0.7
                   (21, [0, 0, 0, 0], [1.0e5, 0.287, 1.0e-3, 0.0]), per, per, per, per]
     ctl.nfmavt = 1 if mixavg else 0
     name = f"h2_{'per' if periodic_x else 'nscbc'}_{'mix' if mixavg else 'cle'}_{nx}x{ny}x{nz}"
-    return Case(name, ctl, mech, tab, dif, Dump(st, 0.0, dt),
-                os.path.join(DATA, "h2_9sp", "chem.dat"),
+    if falloff_variant:
+        name = "h2_falloff" + name[2:]
+    return Case(name, ctl, mech, tab, dif, Dump(st, 0.0, dt), chem_path,
                 os.path.join(DATA, "h2_9sp", "diff.dat") if mixavg else None)
 
 
@@ -281,6 +317,96 @@ def h2_kernel(n=(64, 64, 64), length=(4.0e-3, 4.0e-3, 4.0e-3), seed=11, phi=0.8,
     return Case(f"h2_kernel_{nx}x{ny}x{nz}", ctl, mech, tab, dif, Dump(st, 0.0, dt),
                 os.path.join(DATA, "h2_9sp", "chem.dat"),
                 os.path.join(DATA, "h2_9sp", "diff.dat") if mixavg else None)
+
+
+def h2_falloff_mechanism() -> _chem.Mechanism:
+    """The H2-air mechanism with its two pressure-dependent steps moved to the
+    other falloff forms CHRATE supports (src/chrate.f:312-436):
+
+    * step 9, H + O2 (+M) = HO2 (+M), in Troe form with its Lindemann
+      low-pressure Arrhenius data and Troe constants alpha = 0.7346,
+      T*** = 94 K, T** = 5182 K, T* = 1756 K and the standard broadening
+      constants c = -0.4 - 0.67 log Fc, n = 0.75 - 1.27 log Fc, d = 0.14
+      (file layout of src/chemin.f:300-322; conversions src/chemin.f:843-850);
+    * step 15, H2O2 (+M) = 2 OH (+M), in SRI form with a = 0.45, b = 797 K,
+      c = 979 K, d = 1, e = 0.  Its low-pressure pre-exponential is raised by
+      1e6 so that the reduced pressure stays above 10 over 300-2500 K: the
+      reference's SRI exponent is 1/(1 + log10(log10 Pr)) (src/chrate.f:415),
+      which has no real value for Pr < 1.
+
+    Test data for the Troe and SRI branches, not a validated mechanism."""
+    m = h2_mechanism()
+    ist_troe, ist_sri = 8, 14                       # 0-based steps 9 and 15
+    lt, ls = m.mllist[ist_troe] - 1, m.mllist[ist_sri] - 1
+    a0t, n0t, e0t, _ = m.rclind[:, lt]
+    a0s, n0s, e0s, _ = m.rclind[:, ls]
+    m.rctroe = np.array([[a0t, n0t, e0t, 0.7346, 94.0, 5182.0, 1756.0, -0.4, -0.67, 0.75, -1.27, 0.14]]).T
+    m.rcsrif = np.array([[a0s * 1.0e6, n0s, e0s, 0.45, 797.0, 979.0, 1.0, 0.0]]).T
+    m.mllist = [0] * m.nstep
+    m.rclind = np.zeros((4, 0))
+    m.mtlist = [0] * m.nstep
+    m.mtlist[ist_troe] = 1
+    m.mslist = [0] * m.nstep
+    m.mslist[ist_sri] = 1
+    return m
+
+
+def ch4_mechanism() -> _chem.Mechanism:
+    return _chem.read_chem(os.path.join(DATA, "ch4_4step", "chem.dat"))
+
+
+def ch4_flame(n=(24, 24, 24), length=None, seed=3, phi=0.8, Tu=300.0, uprime=1.0,
+              cfl=0.5, nsteps=10, workdir=None) -> Case:
+    """Triply periodic methane-air flame slab with the reference's 4-step
+    mechanism (data/ch4_4step/chem.dat): two erf fronts along x between
+    unburnt CH4 + (2/phi)(O2 + 3.76 N2) at Tu and its complete-combustion
+    products at the adiabatic temperature, Gaussian bumps of H2, H and CO in
+    the fronts, seeded Passot-Pouquet turbulence, constant-Lewis transport
+    with the mechanism's Lewis numbers, 1 atm."""
+    nx, ny, nz = n
+    workdir = workdir or os.path.join(DATA, "_gen")
+    os.makedirs(workdir, exist_ok=True)
+    mech = ch4_mechanism()
+    tab = _chem.derive(mech)
+    idx = {s: i for i, s in enumerate(mech.species)}
+    W = mech.wmolar
+    nO2 = 2.0 / phi
+
+    def massfrac(mol):
+        y = np.zeros(mech.nspec)
+        for sp, nm in mol.items():
+            y[idx[sp]] = nm * W[idx[sp]]
+        return y / y.sum()
+
+    Yu = massfrac({"CH4": 1.0, "O2": nO2, "N2": 3.76 * nO2})
+    Yb = massfrac({"CO2": 1.0, "H2O": 2.0, "O2": max(nO2 - 2.0, 0.0), "N2": 3.76 * nO2})
+    Tb = _ic.adiabatic_temperature(tab, Yu, Yb, Tu)
+    if length is None:
+        length = tuple((m - 1) * H2_SPACING for m in n)
+    d = tuple(length[i] / (n[i] - 1) for i in range(3))
+    x = _ic.grid(nx, length[0])
+    period = nx * d[0]
+    delta = 3.0 * d[0]
+    c = 0.5 * (np.vectorize(math.erf)((x - 0.3 * period) / delta)
+               - np.vectorize(math.erf)((x - 0.7 * period) / delta))
+    shape = (nz, ny, nx)
+    C = np.broadcast_to(c, shape)
+    T = Tu + (Tb - Tu) * C
+    Y = [Yu[i] * (1.0 - C) + Yb[i] * C for i in range(mech.nspec)]
+    bump = np.exp(-((C - 0.5) / 0.2) ** 2)
+    for s, amp in (("H2", 2e-3), ("H", 1e-4), ("CO", 5e-3)):
+        Y[idx[s]] = Y[idx[s]] + amp * bump
+    Y[idx["N2"]] = Y[idx["N2"]] - (sum(Y) - 1.0)
+    u, v, w = _ic.solenoidal_field(shape, (d[2], d[1], d[0]), uprime, 4.0 * delta, seed,
+                                   spectrum="passot-pouquet")
+    st = _ic.conservative_state(tab, T, np.full(shape, 1.0e5), Y, u, v, w)
+    dt = _f32(cfl * min(d) / sound_speed_max(tab, st, T))
+    ctl = Control(length[0], length[1], length[2], nx, ny, nz, dt, 1, nsteps, 0, prin=1.0e5,
+                  trin=Tu, yrin=[float(y) for y in Yu])
+    ctl.bc = [(1, [0, 0, 0, 0], [0.0] * 4)] * 6
+    ctl.nfmavt = 0
+    return Case(f"ch4_4step_{nx}x{ny}x{nz}", ctl, mech, tab, None, Dump(st, 0.0, dt),
+                os.path.join(DATA, "ch4_4step", "chem.dat"), None)
 
 
 # ---------------------------------------------------------------------------

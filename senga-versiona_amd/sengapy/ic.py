@@ -19,31 +19,53 @@ def grid(n: int, length: float) -> np.ndarray:
     return np.arange(n) * (length / float(n - 1))
 
 
-def solenoidal_field(shape, spacing, urms, k0_len, seed, spectrum="passot-pouquet"):
-    """Random solenoidal velocity on a periodic box (numpy FFT, host side).
-
-    shape=(nz,ny,nx); spacing=(dz,dy,dx); the period in each direction is
-    n*d.  Energy spectrum with peak wavenumber k0 = 2*pi/k0_len:
-      passot-pouquet:     E(k) ~ (k/k0)^4 exp(-2 (k/k0)^2)
-      batchelor-townsend: E(k) ~ (k/k0)^4 exp(-2 (k/k0)^2) (1 + (k/k0)^2)^(-1/2)
-    Random phases from numpy's PCG64 with the given seed; the field is scaled so
-    that each component has RMS urms.  Returns (u, v, w) arrays of `shape`.
-    """
-    rng = np.random.default_rng(seed)
+def _wavenumbers(shape, spacing):
     nz, ny, nx = shape
     dz, dy, dx = spacing
     kz = 2 * np.pi * np.fft.fftfreq(nz, d=dz)
     ky = 2 * np.pi * np.fft.fftfreq(ny, d=dy)
     kx = 2 * np.pi * np.fft.fftfreq(nx, d=dx)
-    KZ, KY, KX = np.meshgrid(kz, ky, kx, indexing="ij")
+    return np.meshgrid(kz, ky, kx, indexing="ij")
+
+
+def espect(k, k0):
+    """Energy spectrum shape of the reference's turbulence generator,
+    E(k) ~ (k/k0)^4 exp(-2 (k/k0)^2) (src/espect.f:62-65, which cites
+    Batchelor and Townsend 1948 for it; Passot and Pouquet 1987 use the same
+    form).  Unnormalised: the field is scaled to its r.m.s. afterwards."""
+    r = k / k0
+    return r ** 4 * np.exp(-2.0 * r ** 2)
+
+
+def solenoidal_field(shape, spacing, urms, k0_len, seed, spectrum="passot-pouquet"):
+    """Random solenoidal velocity on a periodic box (numpy FFT, host side).
+
+    shape=(nz,ny,nx); spacing=(dz,dy,dx); the period in each direction is
+    n*d.  Energy spectrum espect() with peak wavenumber k0 = 2*pi/k0_len
+    (spectrum "passot-pouquet" and "batchelor-townsend" name the same form,
+    src/espect.f).  Each Fourier mode gets amplitude sqrt(E(k) / (4 pi k^2))
+    and random complex Gaussian components (numpy PCG64, seed); the
+    dilatational part is projected out, the real part taken (which keeps the
+    field solenoidal: the projection is even in k), and all three components
+    are scaled by ONE factor so that u' = sqrt(<u^2+v^2+w^2>/3) = urms (a
+    per-component factor would reintroduce divergence).  Returns (u, v, w).
+    """
+    if spectrum not in ("passot-pouquet", "batchelor-townsend"):
+        raise ValueError(spectrum)
+    rng = np.random.default_rng(seed)
+    KZ, KY, KX = _wavenumbers(shape, spacing)
     k2 = KX ** 2 + KY ** 2 + KZ ** 2
     k = np.sqrt(k2)
     k0 = 2 * np.pi / k0_len
-    r = np.where(k > 0, k / k0, 0.0)
-    ek = r ** 4 * np.exp(-2.0 * r ** 2)
-    if spectrum == "batchelor-townsend":
-        ek = ek / np.sqrt(1.0 + r ** 2)
+    ek = np.where(k > 0, espect(k, k0), 0.0)
     amp = np.where(k > 0, np.sqrt(ek / np.maximum(4 * np.pi * k2, 1e-300)), 0.0)
+    # no energy in Nyquist modes (even n): a real field's Nyquist mode has no
+    # sine part, so its derivative -- and its divergence -- is undefined there
+    for ax, nn in enumerate(shape):
+        if nn % 2 == 0:
+            sl = [slice(None)] * 3
+            sl[ax] = nn // 2
+            amp[tuple(sl)] = 0.0
     comps = []
     for _ in range(3):
         ph = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
@@ -55,13 +77,46 @@ def solenoidal_field(shape, spacing, urms, k0_len, seed, spectrum="passot-pouque
     ux = ux - KX * dot
     uy = uy - KY * dot
     uz = uz - KZ * dot
-    out = []
-    for c in (ux, uy, uz):
-        f = np.real(np.fft.ifftn(c))
+    out = [np.real(np.fft.ifftn(c)) for c in (ux, uy, uz)]
+    for f in out:
         f -= f.mean()
-        s = math.sqrt(float((f ** 2).mean()))
-        out.append(f * (urms / s if s > 0 else 0.0))
-    return out[0], out[1], out[2]
+    s = math.sqrt(float(sum((f ** 2).mean() for f in out)) / 3.0)
+    return tuple(f * (urms / s if s > 0 else 0.0) for f in out)
+
+
+def integral_scale(u, v, w, spacing):
+    """Longitudinal integral length scale of a periodic field from its
+    discrete energy spectrum: l = (pi / (2 u'^2)) int E(k)/k dk
+    = (3 pi / 4) sum_k e(k)/|k| / sum_k e(k), e(k) the modal energy."""
+    KZ, KY, KX = _wavenumbers(u.shape, spacing)
+    k = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
+    e = sum(np.abs(np.fft.fftn(f)) ** 2 for f in (u, v, w))
+    m = k > 0
+    return 0.75 * np.pi * float(np.sum(e[m] / k[m]) / np.sum(e[m]))
+
+
+def expected_integral_scale(shape, spacing, k0_len):
+    """integral_scale() of the mean modal energy of solenoidal_field's
+    ensemble: e(k) ~ E(k) / (4 pi k^2) (the projection removes the same share
+    of every mode)."""
+    KZ, KY, KX = _wavenumbers(shape, spacing)
+    k = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
+    m = k > 0
+    e = espect(k[m], 2 * np.pi / k0_len) / (4 * np.pi * k[m] ** 2)
+    return 0.75 * np.pi * float(np.sum(e / k[m]) / np.sum(e))
+
+
+def k0_len_for_integral_scale(shape, spacing, target):
+    """Peak wavelength 2 pi / k0 whose expected discrete integral scale is
+    `target` (bisection; on a fine grid k0_len -> target * sqrt(2 pi))."""
+    lo, hi = 1e-3 * target, 1e3 * target
+    for _ in range(200):
+        mid = math.sqrt(lo * hi)
+        if expected_integral_scale(shape, spacing, mid) > target:
+            hi = mid
+        else:
+            lo = mid
+    return math.sqrt(lo * hi)
 
 
 def erf_profile(x, x0, delta):

@@ -20,6 +20,13 @@ def build(name):
         return C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=False)
     if name == "h2_per_mix_11x12x11":
         return C.h2_nscbc(n=(11, 12, 11), length=(1.4e-3, 1.5e-3, 1.4e-3), mixavg=True, periodic_x=True)
+    if name == "h2_falloff_nscbc_mix_12x11x11":
+        # CHRATE's Troe and SRI branches (src/chrate.f:312-436)
+        return C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=True, falloff_variant=True)
+    if name == "ch4_4step_16x12x11":
+        # the reference's 4-step methane mechanism: reactant coefficient lists
+        # with non-integer and negative orders (src/chrate.f:475-516)
+        return C.ch4_flame(n=(16, 12, 11))
     if name == ADAPT:
         c = C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=True)
         c.control.nstpsw = 1
@@ -30,7 +37,8 @@ def build(name):
 ADAPT = "h2_nscbc_mix_adapt_12x11x11"
 
 
-NAMES = ["periodic_1step_16x12x11", "h2_nscbc_mix_12x11x11", "h2_nscbc_cle_12x11x11", "h2_per_mix_11x12x11"]
+NAMES = ["periodic_1step_16x12x11", "h2_nscbc_mix_12x11x11", "h2_nscbc_cle_12x11x11", "h2_per_mix_11x12x11",
+         "h2_falloff_nscbc_mix_12x11x11", "ch4_4step_16x12x11"]
 OPS = {"ops_periodic_16x12x11": "periodic_1step_16x12x11", "ops_boundedx_12x11x11": "h2_nscbc_cle_12x11x11"}
 
 

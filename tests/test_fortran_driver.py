@@ -71,7 +71,7 @@ def test_fortran_driver_run_matches_library_and_oracle(tmp_path):
     # the driver's configuration is the binding's byte for byte, so the H2
     # mechanism it reads from chem.dat is the one compiled into the library
     assert "compiled mechanism" in r.stdout
-    d = read_dump(str(tmp_path / "output" / "dmpi0000000.dat"))
+    d = read_dump(str(tmp_path / "output" / "dmpi0000001.dat"))
     s = Solver(c)
     assert s.mechanism_compiled()
     s.set_timestep(c.dump.tstep)
@@ -113,7 +113,7 @@ def test_fortran_driver_adaptive_restart_keeps_dump_step(tmp_path):
     r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
-    d = read_dump(str(tmp_path / "output" / "dmpi0000000.dat"))
+    d = read_dump(str(tmp_path / "output" / "dmpi0000001.dat"))
     o = Oracle(c)
     o.set_timestep(c.dump.tstep)
     o.set_adaptive(True, c.dump.errold, c.dump.errldr)
@@ -153,3 +153,69 @@ def test_fortran_chem_reader_rejects_out_of_range_indices(tmp_path, what):
     r = subprocess.run([exe, "--config", str(tmp_path), str(tmp_path / "cfg.bin")], capture_output=True, text=True)
     assert r.returncode != 0
     assert "cannot read input/chem.dat" in r.stdout
+
+
+def test_both_readers_read_the_reference_formatted_dump(tmp_path):
+    """A dump the reference itself wrote: src/casefiles/21step/dmpi0000000.dat
+    (formatted, 1001 x 1 x 1 points, 9 species; kept as the fixture
+    tests/golden/ref21step_dmpi0000000.dat).  The Python formatted reader and
+    the Fortran reader (the driver's --convert-dump mode: formatted read as
+    src/indata.f:1449-1473, unformatted write as src/output.f:306-316) give the
+    same values bit for bit, and the state is a 1-D flame: sum_k rho Y_k =
+    rho to the dump's precision, positive density."""
+    from sengapy.dump import read_dump, read_dump_formatted
+    exe = _driver()
+    src = os.path.join(HERE, "golden", "ref21step_dmpi0000000.dat")
+    d = read_dump_formatted(src)
+    assert d.state.shape == (14, 1, 1, 1001)
+    out = tmp_path / "dmpi.bin"
+    r = subprocess.run([exe, "--convert-dump", src, str(out)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    u = read_dump(str(out))
+    assert np.array_equal(u.state, d.state)
+    assert (u.etime, u.tstep, u.errold, u.errldr) == (d.etime, d.tstep, d.errold, d.errldr)
+    rho = d.state[0]
+    assert rho.min() > 0
+    assert np.max(np.abs(d.state[5:].sum(axis=0) / rho - 1.0)) < 1e-11
+    assert d.etime == pytest.approx(1.394146750614353e-2) and d.tstep == pytest.approx(1.413965038003083e-8)
+
+
+@pytest.mark.gpu
+def test_fortran_driver_outputs_match_reference(tmp_path):
+    """The run outputs of src/output.f against the reference's own, written by
+    the reference build with OUTPUT called after every step (fixture
+    output_h2_nscbc_mix_12x11x11.npz: 4 steps, reports every step, dumps and
+    statistics every 2 steps): the report lines OUTPUT appends to rept.res
+    (step number, elapsed and next time step in 1PE12.4, dump file names),
+    stat.res, the out*.res file names, and their arrays -- rho, u, v, w, E, T,
+    p, Y_k, the reaction rates and the elapsed time -- at 1e-10 max-norm."""
+    import golden_cases
+    from sengapy.dump import OUT_FIELDS, read_out_res
+    from conftest import rel_err
+    exe = _driver()
+    fx = dict(np.load(os.path.join(HERE, "golden", "output_h2_nscbc_mix_12x11x11.npz")))
+    c = golden_cases.build("h2_nscbc_mix_12x11x11")
+    c.dump.state = fx["state0"]
+    c.dump.tstep = float(fx["tstep"])
+    c.control.tstep = c.dump.tstep
+    c.control.ntime = int(fx["nsteps"])
+    c.control.ntdump, c.control.ntrept, c.control.ntstat = 2, 1, 2
+    c.write_inputs(str(tmp_path))
+    print("fortran driver outputs", exe, flush=True)
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    od = tmp_path / "output"
+    rept = [ln.rstrip() for ln in (od / "rept.res").read_text().splitlines()]
+    first = next(i for i, ln in enumerate(rept) if ln.startswith("Time step number:"))
+    assert rept[first:] == list(fx["rept_lines"])
+    assert [ln.rstrip() for ln in (od / "stat.res").read_text().splitlines()] == list(fx["stat_lines"])
+    names = sorted(f for f in os.listdir(od) if f.startswith("out"))
+    assert names == list(fx["outres_names"])
+    nx, ny, nz = c.shape
+    for q, nm in enumerate(names):
+        o = read_out_res(str(od / nm), nx, ny, nz, c.nspec)
+        errs = {k: rel_err(o[k], fx[f"out{q}_{k}"]) for k in OUT_FIELDS + ("Y", "rate")}
+        print(nm, " ".join(f"{k} {e:.1e}" for k, e in errs.items()), flush=True)
+        assert max(errs.values()) < 1e-10
+        assert o["etime"] == float(fx[f"out{q}_etime"])

@@ -4,7 +4,8 @@ A batch launches every substage kernel once over all its members, from a
 member table and block map, and replays one captured graph per RK step.  Each
 member evaluates exactly the arithmetic it evaluates alone, so the bar is
 bit-identity with the single-case run (FP64 and mixed), including members
-that join or leave between steps.
+that join or leave between steps; each member is also checked against the
+oracle (FP64 1e-10, mixed 1e-5).
 """
 import numpy as np
 import pytest
@@ -43,8 +44,23 @@ def _alone(spec, steps, precision=0):
 SHAPES = [(32, 16, 16), (24, 20, 16), (40, 16, 24), (16, 16, 16)]
 
 
-@pytest.mark.parametrize("kind,precision", [("nscbc", 0), ("nscbc", 1), ("kernel", 0)])
+def _oracle(spec, steps):
+    from pyoracle import Oracle
+    from sengapy import case as C
+    c = C.make_case(spec)
+    o = Oracle(c)
+    o.set_timestep(c.dump.tstep)
+    o.set_state(c.dump.state, 0.0)
+    o.step(steps)
+    run, T, _, _ = o.get()
+    return run, T
+
+
+@pytest.mark.parametrize("kind,precision", [("nscbc", 0), ("nscbc", 1), ("kernel", 0), ("kernel", 1)])
 def test_batch_matches_alone(kind, precision):
+    """Each member of a batch of uneven cases equals its solo run bit for bit
+    and the oracle within the precision's bar (FP64 1e-10, mixed 1e-5)."""
+    from conftest import assert_state_close
     from sengapy.abi import Batch
     specs = _specs(kind, SHAPES)
     print("batch", kind, precision, [s["n"] for s in specs], flush=True)
@@ -57,11 +73,14 @@ def test_batch_matches_alone(kind, precision):
     assert b.size() == len(sol)
     b.step(steps)
     b.sync()
+    tol, tol_T = (1e-10, 1e-12) if precision == 0 else (1e-5, 1e-5)
     for q, s in enumerate(sol):
         run, T, P, _ = s.download()
         assert np.array_equal(run, ref[q][0]), f"member {q} registers differ"
         assert np.array_equal(T, ref[q][1]), f"member {q} temperature differs"
         assert np.array_equal(P, ref[q][2]), f"member {q} pressure differs"
+        run_o, T_o = _oracle(specs[q], steps)
+        assert_state_close(f"member {q} vs oracle", run, T, run_o, T_o, tol, tol_T)
     b.close()
 
 

@@ -3,10 +3,13 @@
 Requests arrive while others are in flight, join the batch at the next chunk
 boundary and leave it when their own step count is done.  Each result must be
 the one the case gives when solved alone: the batch kernels evaluate every
-member's own arithmetic, so the bar is bit-identity (FP64 and mixed).
+member's own arithmetic, so the bar is bit-identity (FP64 and mixed); each
+result is also checked against the oracle (FP64 1e-10, mixed 1e-5).
 """
 import numpy as np
 import pytest
+
+from conftest import assert_state_close
 
 pytestmark = pytest.mark.gpu
 
@@ -36,6 +39,16 @@ def _alone(req, precision):
     return run, T, P
 
 
+def _oracle(req):
+    from pyoracle import Oracle
+    o = Oracle(req.case)
+    o.set_timestep(req.case.dump.tstep)
+    o.set_state(req.case.dump.state, 0.0)
+    o.step(req.nsteps)
+    run, T, _, _ = o.get()
+    return run, T
+
+
 @pytest.mark.parametrize("precision,point_budget", [(0, 24e6), (1, 24e6), (0, 5000)])
 def test_served_results_equal_alone(precision, point_budget):
     from sengapy.serve import Server, summary
@@ -44,12 +57,15 @@ def test_served_results_equal_alone(precision, point_budget):
     srv = Server(chunk=2, precision=precision, point_budget=point_budget)
     res = srv.run(reqs)
     assert [r.rid for r in res] == list(range(len(reqs)))
+    tol, tol_T = (1e-10, 1e-12) if precision == 0 else (1e-5, 1e-5)
     for r, q in zip(res, reqs):
         run, T, P = _alone(q, precision)
         assert np.array_equal(r.run, run), f"request {q.rid}: registers differ from the solo run"
         assert np.array_equal(r.T, T)
         assert np.array_equal(r.P, P)
         assert r.finished >= r.admitted >= r.arrival - 1e-9
+        run_o, T_o = _oracle(q)
+        assert_state_close(f"request {q.rid} vs oracle", r.run, r.T, run_o, T_o, tol, tol_T)
     if point_budget < 2 * 16 ** 3:
         # a budget below any two cases together: one case in flight at a time
         assert srv.max_points_in_flight == max(r.points for r in res)
