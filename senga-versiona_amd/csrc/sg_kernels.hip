@@ -578,9 +578,11 @@ __device__ __forceinline__ void chem_step64(const M &m, int st, double t, double
   }
   if (mb != 0 && flthrd) s1 = s1 * s3;
   if (m.dense()) {
-    // every species once per list: absent species add s1*0, an exact zero
+    // every species once per list: absent species add s1*0, an exact zero;
+    // with compile-time tables those terms are not formed at all
 #pragma unroll
-    for (int u = 0; u < NS; u++) rate[u] = rate[u] + s1 * m.nuw(st, u);
+    for (int u = 0; u < NS; u++)
+      if (!M::fixed || m.nuw(st, u) != 0.0) rate[u] = rate[u] + s1 * m.nuw(st, u);
   } else {
     for (int q = 0; q < m.nss(st); q++) {
       int sp = m.nssp(st, q);
@@ -984,7 +986,8 @@ __device__ __forceinline__ void chem_stepmx(const M &m, int st, float tf, double
   if (mb != 0 && flthrd) s1 = s1 * s3;
   if (m.dense()) {
 #pragma unroll
-    for (int u = 0; u < NS; u++) rate[u] = rate[u] + s1 * m.nuwf(st, u);
+    for (int u = 0; u < NS; u++)
+      if (!M::fixed || m.nuwf(st, u) != 0.0f) rate[u] = rate[u] + s1 * m.nuwf(st, u);
   } else {
     for (int q = 0; q < m.nss(st); q++) {
       int sp = m.nssp(st, q);
