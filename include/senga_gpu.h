@@ -215,6 +215,19 @@ int sg_rhs(sg_handle *h, int irk, double *rhs);
  * out(nx,ny,nz,9). */
 int sg_derivatives(sg_handle *h, const double *f, double *out);
 
+/* No counterpart in the reference: a variant of sg_derivatives for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same operators, same
+ * results; the device copies of f and of the nine outputs live in a fixed
+ * static device array instead of per-call allocations, so the call makes no
+ * allocation.  Limit: grids of at most SG_NOALLOC_MAX_POINTS points
+ * (SG_EINVAL above it); the array (10 x SG_NOALLOC_MAX_POINTS doubles,
+ * 40 MiB) is held by every GPU the library runs on until the process ends.
+ * All calls on one GPU share that array: calls that overlap on one GPU (from
+ * several host threads) race. */
+#define SG_NOALLOC_MAX_POINTS (1 << 19)
+int sg_derivatives_noalloc(sg_handle *h, const double *f, double *out);
+
 /* Wait for all work of the handle; returns the first pending error. */
 int sg_sync(sg_handle *h);
 

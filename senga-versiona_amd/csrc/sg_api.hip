@@ -26,6 +26,29 @@ const char *kKernelNames[K_COUNT] = {"tempin", "boundt", "prim", "pass_x", "pass
 
 }  // namespace
 
+// Staging of sg_derivatives_noalloc: the input field and its nine
+// derivatives, SG_NOALLOC_MAX_POINTS points each (slots rounded to 32
+// elements), in one static device array.  Every GPU holds its own copy for
+// the life of the process.  Host code never uses the array's name (a
+// host-side stand-in): it uses the address hipGetSymbolAddress returns with
+// the device current, kept per device by noalloc_scratch().
+static constexpr long kNoAllocSlot = ((long)SG_NOALLOC_MAX_POINTS + 31) / 32 * 32;
+static __device__ __align__(256) double scratch[10 * kNoAllocSlot];
+static constexpr int kMaxDevices = 64;
+static double *g_scratch[kMaxDevices];
+
+// The array's address on `device` (which must be current), looked up once
+// per device; null if the lookup fails.
+static double *noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  if (!g_scratch[device]) {
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    g_scratch[device] = (double *)ptr;
+  }
+  return g_scratch[device];
+}
+
 struct sg_batch;
 
 struct sg_handle {
@@ -36,6 +59,7 @@ struct sg_handle {
   Fields F;
   hipStream_t stream;
   int device;
+  double *scratch;     // this device's no-allocation staging (noalloc_scratch), or null
   void *base;          // single device allocation
   size_t bytes;
   double etime, tstep;
@@ -645,6 +669,7 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   }
   hipEventCreate(&h->ev[0]);
   hipEventCreate(&h->ev[1]);
+  h->scratch = noalloc_scratch(h->device);   // once per device; failure only disables the no-allocation call
   h->etime = 0.0;
   const int wide[3] = {0, 0, 0};
   if (init_launch(h, wide) != 0) {
@@ -927,6 +952,22 @@ int sg_derivatives(sg_handle *h, const double *f, double *out) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipFree(df));
   HIPCHK(h, hipFree(dout));
+  return rc;
+}
+
+int sg_derivatives_noalloc(sg_handle *h, const double *f, double *out) {
+  if (!h || !f || !out) return SG_EINVAL;
+  const Params &P = h->P;
+  if (P.npts > SG_NOALLOC_MAX_POINTS)
+    return set_err(h, SG_EINVAL, "grid exceeds SG_NOALLOC_MAX_POINTS points of the no-allocation staging");
+  if (!h->scratch) return set_err(h, SG_EDEVICE, "no-allocation staging unavailable on this device");
+  size_t fb = sizeof(double) * P.npts;
+  double *df = h->scratch, *dout = h->scratch + kNoAllocSlot;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpyAsync(df, f, fb, hipMemcpyHostToDevice, h->stream));
+  int rc = timed(h, K_DERIVS, [&] { launch_derivs(P, df, dout, h->stream); });
+  if (!rc) HIPCHK(h, hipMemcpyAsync(out, dout, 9 * fb, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return rc;
 }
 

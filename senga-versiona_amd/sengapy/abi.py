@@ -198,7 +198,7 @@ ABI_FUNCS = ["sg_abi_version", "sg_config_size", "sg_create", "sg_destroy", "sg_
              "sg_batch_create", "sg_batch_destroy", "sg_batch_add", "sg_batch_remove", "sg_batch_size",
              "sg_batch_member", "sg_batch_step", "sg_batch_sync", "sg_footprint", "sg_set_tiles",
              "sg_get_tiles", "sg_autotune", "sg_batch_set_tiles", "sg_batch_get_tiles", "sg_batch_autotune",
-             "sg_mechanism_compiled"]
+             "sg_mechanism_compiled", "sg_derivatives_noalloc"]
 
 
 def load_library(path=None):
@@ -222,6 +222,7 @@ def load_library(path=None):
     lib.sg_step.argtypes = [vp, C.c_int]
     lib.sg_rhs.argtypes = [vp, C.c_int, dp]
     lib.sg_derivatives.argtypes = [vp, dp, dp]
+    lib.sg_derivatives_noalloc.argtypes = [vp, dp, dp]
     lib.sg_sync.argtypes = [vp]
     lib.sg_last_error.argtypes = [vp, P(C.c_int), P(C.c_int), C.c_char_p, C.c_size_t]
     lib.sg_get_stream.argtypes = [vp, P(vp)]
