@@ -103,7 +103,9 @@ inline __host__ __device__ int face_corr(int ns) { return F_NSCAL + ns; }
 // kernel kind.  The grid of a kind is the concatenation of every member's own
 // grid of that kind: flattened block b belongs to member map[kind][b] and is
 // that member's block b - blk0[kind].  A single case is a launch of one member.
-enum LaunchKind { L_FACE = 0, L_FACE2, L_POINT, L_PASSX, L_PASSY, L_PASSZ, L_FINAL, L_NKIND };   // L_FACE2: both x faces
+// L_FACE2: both x faces; L_PASSXE: the edge tiles of bounded x rows (the x
+// pass of a bounded x direction is two launches, interior and edge tiles)
+enum LaunchKind { L_FACE = 0, L_FACE2, L_POINT, L_PASSX, L_PASSY, L_PASSZ, L_FINAL, L_PASSXE, L_NKIND };
 
 struct Member {
   Params P;

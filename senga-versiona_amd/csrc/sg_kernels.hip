@@ -1300,18 +1300,45 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
 // DIR.  The block owns a tile of points; every field (or product of fields)
 // is staged once into LDS with a 5-deep halo along DIR, then each thread
 // evaluates its stencils from LDS at compile-time offsets.  Fields staged:
-// rho u_d, e*rho u_d, e, T, lambda, u, v, w, the three momentum fluxes,
-// p - p_ref, mu, h_mix and per species Y, Y*rho u_d, rho D, h.
-template <int DIR, int NW = 0>
+// rho u_d (resident for the whole pass), e, T, lambda, u, v, w, p - p_ref,
+// mu, h_mix and per species Y, rho D, h; the products with rho u_d that the
+// split convective forms difference (e rho u_d, Y rho u_d and the momentum
+// fluxes rho u_d u_b) are formed inside the stencils from two tiles.
+// Bounded x direction: the x pass runs as two launches.  Edge tiles, PASS_EW
+// points wide at each end of a row, hold every closure row (i < 5, i > nx - 6)
+// and evaluate all their stencils in the uniform row form; interior tiles
+// cover [PASS_EW, nx - PASS_EW), where every stencil is the interior one and
+// reaches no further than 5 points into the edge tiles.  Neither kernel then
+// carries the other's stencil code (with both in one kernel, and the form
+// chosen per wave, the bounded x pass took 0.345 ms at 128^3 in FP64; the
+// interior form alone 0.298 ms, the row form alone 0.331 ms).  xp: 0 every
+// x tile of a row (periodic x, and the y and z passes); 1 the interior tiles
+// of a bounded row; 2 its edge tiles.  Rows shorter than 2 PASS_EW are all edge
+// tiles.
+constexpr int PASS_EW = 16;
+__host__ __device__ inline int pass_xtiles(int nx, int xp, int tx) {
+  if (xp == 0) return (nx + tx - 1) / tx;
+  if (xp == 1) return nx >= 2 * PASS_EW ? (nx - 2 * PASS_EW + tx - 1) / tx : 0;
+  return nx >= 2 * PASS_EW ? 2 : (nx + PASS_EW - 1) / PASS_EW;
+}
+// first x index of x tile bx
+__host__ __device__ inline int pass_x0(int nx, int xp, int tx, int bx) {
+  if (xp == 0) return bx * tx;
+  if (xp == 1) return PASS_EW + bx * tx;
+  return nx >= 2 * PASS_EW ? (bx == 0 ? 0 : nx - PASS_EW) : bx * PASS_EW;
+}
+
+template <int DIR, int NW = 0, int XP = 0>
 struct PassTile {
   // Wide tiles (NW = 0), x: 32x8 points (halo 42/32 along x); y, z: 32x16
   // points, 512 threads, so that the 10-point halo along the pass direction
   // costs 26/16 re-reads.  Narrow tiles (NW = 1) have rows of 16 points and
   // twice the rows: no idle lanes when nx is an odd multiple of 16, and a
   // cheaper y/z halo (42/32), at half-length coalesced rows.  The autotuner
-  // picks one per direction.
-  static constexpr int TX = NW ? 16 : 32;
-  static constexpr int TY = DIR == 0 ? (NW ? 16 : 8) : (DIR == 1 ? (NW ? 32 : 16) : 1);
+  // picks one per direction.  Edge tiles of a bounded x row (XP = 2):
+  // PASS_EW x 16 points.
+  static constexpr int TX = XP == 2 ? PASS_EW : (NW ? 16 : 32);
+  static constexpr int TY = XP == 2 ? 16 : (DIR == 0 ? (NW ? 16 : 8) : (DIR == 1 ? (NW ? 32 : 16) : 1));
   static constexpr int TZ = DIR == 2 ? (NW ? 32 : 16) : 1;
   static constexpr int NT = TX * TY * TZ;   // threads per block
   static constexpr int LX = TX + (DIR == 0 ? 10 : 0);
@@ -1321,35 +1348,33 @@ struct PassTile {
   static constexpr int LSTR = DIR == 0 ? 1 : (DIR == 1 ? LX : LX * LY);   // LDS stride along DIR
 };
 
-template <int DIR, bool BND, int NS, class R, int NW>
-__global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu_waves_per_eu(4))) k_pass(const Member *__restrict__ mem,
-                                                                                                   const uint32_t *__restrict__ map) {
-  using T_ = PassTile<DIR, NW>;
+template <int DIR, bool BND, int NS, class R, int NW, int XP = 0>
+__global__ void __launch_bounds__((PassTile<DIR, NW, XP>::NT)) __attribute__((amdgpu_waves_per_eu(4))) k_pass(const Member *__restrict__ mem,
+                                                                                                       const uint32_t *__restrict__ map) {
+  static_assert(XP == 0 || (DIR == 0 && (XP == 1) != BND), "interior x tiles take the periodic form, edge tiles the bounded one");
+  using T_ = PassTile<DIR, NW, XP>;
   constexpr int TX = T_::TX, TY = T_::TY, LX = T_::LX, LY = T_::LY, LN = T_::LN, LSTR = T_::LSTR;
   constexpr int NTH = T_::NT;
   // LN staged values per field, padded to whole block-sized rows so that
   // every thread loads and stores unconditionally
   constexpr int NIT = (LN + NTH - 1) / NTH;
-  // Two buffers of up to STW fields each: a stage stages one field, or three
-  // fields under one barrier (a species' Y_k, rho D_k and h_k; grouped
-  // primitive and flux stages below).
-  // The velocity, temperature and flux stages are grouped the same way in the
-  // y and z passes (measured faster there, a few spills in FP64 included).
-  // Grouping changes the order of some sums, so both tiles of a direction
-  // take the same path: the tile choice never changes a result.
-  constexpr bool EMW = DIR != 0;
+  // Two buffers of up to STW fields each: a stage stages one field, or up to
+  // three fields under one barrier (e, T and lambda; u, v and w; p and mu; a
+  // species' Y_k, rho D_k and h_k).  Grouped stages were measured faster in
+  // every direction than one field per barrier (FP64 x pass at 128^3
+  // periodic: 0.279 -> 0.265 ms), a few FP64 spills included.
   constexpr int STW = 3;
   __shared__ R tile[2][STW][NIT * NTH];
   const int t = threadIdx.x;
   const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
   int lb;
-  const Member &M = member_at<L_PASSX + DIR>(mem, map, xcd_block(blockIdx.x, gridDim.x), lb);
+  const Member &M = member_at<XP == 2 ? L_PASSXE : L_PASSX + DIR>(mem, map, xcd_block(blockIdx.x, gridDim.x), lb);
   const Params P = M.P;
   const Fields F = global_fields(M.F);
-  const int nbx = (P.nx + TX - 1) / TX, nby = (P.ny + TY - 1) / TY;
-  const int x0 = (lb % nbx) * TX, y0 = ((lb / nbx) % nby) * TY, z0 = (lb / (nbx * nby)) * T_::TZ;
+  const int nbx = pass_xtiles(P.nx, XP, TX), nby = (P.ny + TY - 1) / TY;
+  const int x0 = pass_x0(P.nx, XP, TX, lb % nbx), y0 = ((lb / nbx) % nby) * TY, z0 = (lb / (nbx * nby)) * T_::TZ;
   const int i = x0 + lx, j = y0 + ly, k = z0 + lz;
-  const bool active = i < P.nx && j < P.ny && k < P.nz;
+  const bool active = i < (XP == 1 ? P.nx - PASS_EW : P.nx) && j < P.ny && k < P.nz;
   const long idx = ((long)k * P.ny + j) * P.nx + i;
   const int m = DIR == 0 ? i : (DIR == 1 ? j : k);
   const int n = DIR == 0 ? P.nx : (DIR == 1 ? P.ny : P.nz);
@@ -1360,19 +1385,17 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
     if (m < 5) S.mode = 1 + m;
     else if (m > n - 6 && m < n) S.mode = 11 + (n - 1 - m);
   }
-  // Bounded direction: waves holding any closure row evaluate all their
-  // stencils in the uniform 5-term row form (weights from an LDS table,
-  // offsets packed per lane); other waves take the interior stencil.
+  // Bounded direction: edge tiles (XP = 2) evaluate every stencil in the
+  // uniform 5-term row form (weights from an LDS table, offsets packed per
+  // lane; interior points there take the interior row, the same sums).
   __shared__ R wtab[BND ? SG_NROW : 1][10];   // in the working precision
   int pk[5] = {0, 0, 0, 0, 0};
   int row = 0;
-  bool edge = false;
   if (BND) {
     if (t < SG_NROW * 10) (&wtab[0][0])[t] = (R)(&F.tab->roww[0][0])[t];
     row = row_of_mode(S.mode);
 #pragma unroll
     for (int q = 0; q < 5; q++) pk[q] = c_row_offsets.w[row][q];
-    edge = __ballot(S.mode != 0) != 0;
   }
   // Row offsets for one stencil.  In FP64 each stencil takes fresh copies, so
   // that its neighbour addresses are formed per stencil rather than held live
@@ -1436,18 +1459,17 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
   auto D1f = [&](int f) {
     auto at = [&](int kk) { return tile[buf][f][lbase + (kk - 8) * LSTR]; };
     auto fq = [&](int q) { return tile[buf][f][lbase + q * LSTR]; };
-    if (BND && edge) { int pq[5]; pkc(pq); return d1_rows<R>(P, pq, DIR, wq, fq); }
+    if (BND) { int pq[5]; pkc(pq); return d1_rows<R>(P, pq, DIR, wq, fq); }
     return d1<false, R>(P, S, DIR, at);
   };
   auto D2f = [&](int f) {
     auto at = [&](int kk) { return tile[buf][f][lbase + (kk - 8) * LSTR]; };
     auto fq = [&](int q) { return tile[buf][f][lbase + q * LSTR]; };
-    if (BND && edge) { int pq[5]; pkc(pq); return d2_rows<R>(P, pq, DIR, wq, fq); }
+    if (BND) { int pq[5]; pkc(pq); return d2_rows<R>(P, pq, DIR, wq, fq); }
     return d2<false, R>(P, S, DIR, at);
   };
   auto ctrf = [&](int f) { return tile[buf][f][lbase]; };
   auto D1 = [&]() { return D1f(0); };
-  auto D2 = [&]() { return D2f(0); };
   auto ctr = [&]() { return ctrf(0); };
   auto next = [&]() { buf ^= 1; };
   const long fs = P.fstride;
@@ -1483,7 +1505,7 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
   auto atm = [&](int kk) { return tmd[lbase + (kk - 8) * LSTR]; };
   auto fqm = [&](int q) { return tmd[lbase + q * LSTR]; };
   auto D1M = [&]() {
-    if (BND && edge) { int pq[5]; pkc(pq); return d1_rows<R>(P, pq, DIR, wq, fqm); }
+    if (BND) { int pq[5]; pkc(pq); return d1_rows<R>(P, pq, DIR, wq, fqm); }
     return d1<false, R>(P, S, DIR, atm);
   };
   // The product stencil reads twice the operands of a plain one; its loads
@@ -1500,17 +1522,18 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
     };
     SG_PHASE();
     R r;
-    if (BND && edge) { int pq[5]; pkc(pq); r = d1_rows<R>(P, pq, DIR, wq, fqp); }
+    if (BND) { int pq[5]; pkc(pq); r = d1_rows<R>(P, pq, DIR, wq, fqp); }
     else r = d1<false, R>(P, S, DIR, atp);
     pin(r);
     SG_PHASE();
     return r;
   };
-  auto D1P = [&]() { return D1Pf(0); };
-  // Momentum fluxes as the reference forms them: rho u*u, rho u*v, rho u*w,
-  // rho v*v, rho v*w, rho w*w.  Those of the form rho u_d * u_b come from the
-  // resident tile in the u_b stage; the others (rho u*v in y; rho u*w and
-  // rho v*w in z) are staged as products.
+  // Momentum fluxes rho u_d u_b of the pass direction d: all three are formed
+  // inside the stencils from the resident rho u_d tile and the staged u_b
+  // tile.  The reference forms the fluxes rho u*v (in y), rho u*w and rho v*w
+  // (in z) as rho u_b * u_d (src/rhsvel.f:348-729); rho u_d * u_b is the same
+  // quantity, rounded differently in its last bit, and needs no staged
+  // product (one barrier and two field loads fewer in y, two and four in z).
   R flux[3];
   R dm, md0, aD, aE, dp, p0, dmu, mu0;
   R du[3], d2u[3], u0[3];
@@ -1524,161 +1547,59 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
   auto f_mu = [&](long g) { return mu[g]; };
   prefetch([&](long g) { return md[g]; });
   stage_md();
-  if constexpr (EMW) {
-    // e, T and lambda under one barrier, u, v and w under the next, then the
-    // fluxes without rho u_d, p - p_ref and mu (z: mu after them)
-    prefetch_to(0, [&](long g) { return ee[g]; });
-    prefetch_to(1, [&](long g) { return TT[g]; });
-    prefetch_to(2, [&](long g) { return la[g]; });
-    dm = D1M(); md0 = tmd[lbase];
-    pin(dm); pin(md0);
-    aD = -dm;
-    stage_n(3);
-    prefetch_to(0, [&](long g) { return uu[g]; });
-    prefetch_to(1, [&](long g) { return vv[g]; });
-    prefetch_to(2, [&](long g) { return ww[g]; });
-    // ---------------- continuity and energy convection (src/rhscal.f:141-415)
-    {
-      R dem = D1Pf(0);
-      const R de = D1f(0);
-      const R e0 = ctrf(0);
-      aE = -R(0.5) * (e0 * dm + dem + md0 * de);
-      pin(aE);
-    }
-    SG_PHASE();
-    // ---------------- heat conduction (src/rhscal.f:427-797)
-    {
-      R dT = D1f(1), d2T = D2f(1);
-      pin(dT); pin(d2T);
-      SG_PHASE();
-      const R dl = D1f(2);
-      const R lam0 = ctrf(2);
-      aE = aE + (zcon ? R(0) : dl) * dT;
-      aE = aE + lam0 * (zcon ? R(0) : d2T);
-      pin(aE);
-    }
-    next();
-    // ---------------- velocity derivatives and rho u_d u_b fluxes
-    stage_n(3);
-    if (DIR == 0) {
-      prefetch_to(0, f_pp);
-      prefetch_to(1, f_mu);
-    } else if (DIR == 1) {
-      prefetch_to(0, [&](long g) { return mx[g] * vv[g]; });
-      prefetch_to(1, f_pp);
-      prefetch_to(2, f_mu);
-    } else {
-      prefetch_to(0, [&](long g) { return mx[g] * ww[g]; });
-      prefetch_to(1, [&](long g) { return my[g] * ww[g]; });
-      prefetch_to(2, f_pp);
-    }
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      du[a] = D1f(a); d2u[a] = D2f(a); u0[a] = ctrf(a);
-      pin(du[a]); pin(d2u[a]); pin(u0[a]);
-      if (a >= DIR) {
-        flux[a] = D1Pf(a);
-        pin(flux[a]);
-      }
-      SG_PHASE();
-    }
-    next();
-    stage_n(DIR == 0 ? 2 : 3);
-    if (DIR == 2) prefetch(f_mu);
-    else prefetch_species(0);
-    if (DIR == 0) {
-      dp = D1f(0); p0 = ctrf(0);
-      dmu = D1f(1); mu0 = ctrf(1);
-    } else if (DIR == 1) {
-      flux[0] = D1f(0);
-      dp = D1f(1); p0 = ctrf(1);
-      dmu = D1f(2); mu0 = ctrf(2);
-    } else {
-      flux[0] = D1f(0); flux[1] = D1f(1);
-      dp = D1f(2); p0 = ctrf(2);
-    }
-    next();
-    if (DIR == 2) {
-      stage();
-      prefetch_species(0);
-      dmu = D1(); mu0 = ctr();
-      next();
-    }
-  } else {
-    prefetch([&](long g) { return ee[g]; });
-    dm = D1M(); md0 = tmd[lbase];
-    pin(dm); pin(md0);
-    aD = -dm;
-    stage();
-    prefetch([&](long g) { return TT[g]; });
-    {
-      R dem = D1P();
-      const R de = D1();
-      const R e0 = ctr();
-      next();
-      aE = -R(0.5) * (e0 * dm + dem + md0 * de);
-      pin(aE);
-    }
-    {
-      stage();
-      prefetch([&](long g) { return la[g]; });
-      R dT = D1(), d2T = D2();
-      pin(dT); pin(d2T);
-      next();
-      stage();
-      prefetch([&](long g) { return uu[g]; });
-      const R dl = D1();
-      const R lam0 = ctr();
-      next();
-      aE = aE + (zcon ? R(0) : dl) * dT;
-      aE = aE + lam0 * (zcon ? R(0) : d2T);
-      pin(aE);
-    }
-    stage();
-    prefetch([&](long g) { return vv[g]; });
-    du[0] = D1(); d2u[0] = D2(); u0[0] = ctr();
-    if (DIR == 0) flux[0] = D1P();
-    pin(du[0]); pin(d2u[0]); pin(u0[0]);
-    next();
-    stage();
-    prefetch([&](long g) { return ww[g]; });
-    du[1] = D1(); d2u[1] = D2(); u0[1] = ctr();
-    if (DIR <= 1) flux[1] = D1P();
-    pin(du[1]); pin(d2u[1]); pin(u0[1]);
-    next();
-    stage();
-    if (DIR == 0) prefetch(f_pp);
-    else if (DIR == 1) prefetch([&](long g) { return mx[g] * vv[g]; });
-    else prefetch([&](long g) { return mx[g] * ww[g]; });
-    du[2] = D1(); d2u[2] = D2(); u0[2] = ctr();
-    flux[2] = D1P();
-    pin(du[2]); pin(d2u[2]); pin(u0[2]);
-    next();
-    if (DIR == 1) {
-      stage();
-      prefetch(f_pp);
-      flux[0] = D1();
-      next();
-    } else if (DIR == 2) {
-      stage();
-      prefetch([&](long g) { return my[g] * ww[g]; });
-      flux[0] = D1();
-      next();
-      stage();
-      prefetch(f_pp);
-      flux[1] = D1();
-      next();
-    }
-    stage();
-    prefetch(f_mu);
-    dp = D1(); p0 = ctr();
-    pin(dp); pin(p0);
-    next();
-    stage();
-    prefetch_species(0);
-    dmu = D1(); mu0 = ctr();
-    next();
+  // e, T and lambda under one barrier, u, v and w under the next, then
+  // p - p_ref and mu
+  prefetch_to(0, [&](long g) { return ee[g]; });
+  prefetch_to(1, [&](long g) { return TT[g]; });
+  prefetch_to(2, [&](long g) { return la[g]; });
+  dm = D1M(); md0 = tmd[lbase];
+  pin(dm); pin(md0);
+  aD = -dm;
+  stage_n(3);
+  prefetch_to(0, [&](long g) { return uu[g]; });
+  prefetch_to(1, [&](long g) { return vv[g]; });
+  prefetch_to(2, [&](long g) { return ww[g]; });
+  // ---------------- continuity and energy convection (src/rhscal.f:141-415)
+  {
+    R dem = D1Pf(0);
+    const R de = D1f(0);
+    const R e0 = ctrf(0);
+    aE = -R(0.5) * (e0 * dm + dem + md0 * de);
+    pin(aE);
   }
+  SG_PHASE();
+  // ---------------- heat conduction (src/rhscal.f:427-797)
+  {
+    R dT = D1f(1), d2T = D2f(1);
+    pin(dT); pin(d2T);
+    SG_PHASE();
+    const R dl = D1f(2);
+    const R lam0 = ctrf(2);
+    aE = aE + (zcon ? R(0) : dl) * dT;
+    aE = aE + lam0 * (zcon ? R(0) : d2T);
+    pin(aE);
+  }
+  next();
+  // ---------------- velocity derivatives and rho u_d u_b fluxes, all three
+  // from the resident rho u_d tile and the staged u_b tiles
+  stage_n(3);
+  prefetch_to(0, f_pp);
+  prefetch_to(1, f_mu);
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    du[a] = D1f(a); d2u[a] = D2f(a); u0[a] = ctrf(a);
+    pin(du[a]); pin(d2u[a]); pin(u0[a]);
+    flux[a] = D1Pf(a);
+    pin(flux[a]);
+    SG_PHASE();
+  }
+  next();
+  // ---------------- p - p_ref and mu under one barrier
+  stage_n(2);
+  prefetch_species(0);
+  dp = D1f(0); p0 = ctrf(0);
+  dmu = D1f(1); mu0 = ctrf(1);
+  next();
   pin(dmu); pin(mu0);
   // ---------------- momentum convection (src/rhsvel.f:348-729, 1312-2110)
   R aM[3];
@@ -1732,7 +1653,7 @@ __global__ void __launch_bounds__((PassTile<DIR, NW>::NT)) __attribute__((amdgpu
     else prefetch([&](long g) { return hm[g]; });
     dY = D1f(0); d2Y = D2f(0); yc = ctrf(0);
     pin(dY); pin(d2Y); pin(yc);
-    dYm = D1P();
+    dYm = D1Pf(0);
     pin(dYm);
     SG_PHASE();
     drD = D1f(1); rdc = ctrf(1);
@@ -2247,8 +2168,14 @@ static void launch_pass_w(int dir, const Launch &L, hipStream_t st) {
   const int k = L_PASSX + dir;
   if (dir == 0) {
     const dim3 b(PassTile<0, NW>::NT);
-    if (L.bndx) hipLaunchKernelGGL((k_pass<0, true, NS, R, NW>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
-    else hipLaunchKernelGGL((k_pass<0, false, NS, R, NW>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
+    if (L.bndx) {
+      // interior tiles of the bounded rows, then their edge tiles
+      if (L.nblk[k] > 0) hipLaunchKernelGGL((k_pass<0, false, NS, R, NW, 1>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
+      hipLaunchKernelGGL((k_pass<0, true, NS, R, 0, 2>), dim3(L.nblk[L_PASSXE]), dim3(PassTile<0, 0, 2>::NT), 0, st,
+                         L.mem, L.map[L_PASSXE]);
+    } else {
+      hipLaunchKernelGGL((k_pass<0, false, NS, R, NW>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
+    }
   } else if (dir == 1) {
     hipLaunchKernelGGL((k_pass<1, false, NS, R, NW>), dim3(L.nblk[k]), dim3(PassTile<1, NW>::NT), 0, st, L.mem,
                        L.map[k]);
@@ -2339,8 +2266,12 @@ void member_blocks(const Params &P, const int nw[3], int nblk[L_NKIND]) {
     return ((P.nx + tx - 1) / tx) * ((P.ny + ty - 1) / ty) * ((P.nz + tz - 1) / tz);
   };
   nblk[L_FINAL] = tiles(FinalTile::FX, FinalTile::FY, FinalTile::FZ);
-  nblk[L_PASSX] = nw[0] ? tiles(PassTile<0, 1>::TX, PassTile<0, 1>::TY, PassTile<0, 1>::TZ)
-                        : tiles(PassTile<0, 0>::TX, PassTile<0, 0>::TY, PassTile<0, 0>::TZ);
+  // x pass: every tile of a row (periodic x), or the interior tiles of the
+  // bounded rows (L_PASSX) and their edge tiles (L_PASSXE)
+  const int xp = P.bnd[0] ? 1 : 0;
+  const int txw = nw[0] ? PassTile<0, 1>::TX : PassTile<0, 0>::TX, tyw = nw[0] ? PassTile<0, 1>::TY : PassTile<0, 0>::TY;
+  nblk[L_PASSX] = pass_xtiles(P.nx, xp, txw) * ((P.ny + tyw - 1) / tyw) * P.nz;
+  nblk[L_PASSXE] = P.bnd[0] ? pass_xtiles(P.nx, 2, PASS_EW) * ((P.ny + PassTile<0, 0, 2>::TY - 1) / PassTile<0, 0, 2>::TY) * P.nz : 0;
   nblk[L_PASSY] = nw[1] ? tiles(PassTile<1, 1>::TX, PassTile<1, 1>::TY, PassTile<1, 1>::TZ)
                         : tiles(PassTile<1, 0>::TX, PassTile<1, 0>::TY, PassTile<1, 0>::TZ);
   nblk[L_PASSZ] = nw[2] ? tiles(PassTile<2, 1>::TX, PassTile<2, 1>::TY, PassTile<2, 1>::TZ)
