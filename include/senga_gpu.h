@@ -82,7 +82,13 @@ typedef struct sg_config {
                                  transport, chemistry, derivatives and RHS
                                  assembly; FP64 RK registers, temperature
                                  Newton iteration, enthalpies and rate
-                                 exponents */
+                                 exponents; 2 = mixed as 1, and at the end of
+                                 every RK step (FINCOM) the species of the
+                                 solution register are renormalised in FP64,
+                                 rho Y_k *= rho / sum_j rho Y_j.  The reference
+                                 has no such renormalisation (src/lincom.f:294-
+                                 333 is commented out) and the FP64 form never
+                                 applies it. */
   int32_t physics;            /* SG_PHYS_* bits of switches that are on (all refused) */
   double xgdlen, ygdlen, zgdlen;  /* domain lengths; spacing L/(N-1) (src/dfinit.f:42-50) */
 

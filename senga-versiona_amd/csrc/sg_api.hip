@@ -288,7 +288,8 @@ static int setup_params(sg_handle *h) {
   P.drin = c.prin / rtin;
   for (int q = 0; q < c.nspec; q++) P.yrin[q] = c.yrin[q];
   P.mixavg = c.mixavg;
-  P.precision = c.precision;
+  P.precision = c.precision ? 1 : 0;   // 2: the mixed kernels with renormalisation
+  P.renorm = c.precision == 2 ? 1 : 0;
   P.pofs = c.precision ? c.prin : 0.0;
   P.alamda = c.alamda; P.rlamda = c.rlamda; P.prantl = c.prantl;
   P.tdifgb = c.tdifgb; P.pdifgb = c.pdifgb;
@@ -549,7 +550,7 @@ static int validate_config(const sg_config *cfg) {
   if (!mechanism_in_range(cfg)) return SG_EINVAL;
   if (cfg->nx < 11 || cfg->ny < 11 || cfg->nz < 11) return SG_EINVAL;
   if ((long)cfg->nx * cfg->ny * 8 >= (1L << 31)) return SG_EINVAL;
-  if (cfg->precision != 0 && cfg->precision != 1) return SG_EINVAL;
+  if (cfg->precision < 0 || cfg->precision > 2) return SG_EINVAL;
   if (cfg->mixavg && (cfg->ncovis < 1 || cfg->ncocon < 1 || cfg->ncodif < 1 || cfg->ncovis > SG_NTRC ||
                       cfg->ncocon > SG_NTRC || cfg->ncodif > SG_NTRC))
     return SG_EINVAL;

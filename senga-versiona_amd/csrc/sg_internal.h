@@ -30,6 +30,7 @@ struct Params {
   double drin, uin, cobcxr, pinfxr;
   double yrin[SG_MAX_SPEC];    // inflow mass fractions (NSBCI3)
   int precision;               // 0: FP64; 1: mixed (FP32 primitives, rates, RHS; FP64 registers, T)
+  int renorm;                  // mixed only: FP64 sum(rho Y_k) = rho renormalisation at FINCOM
   double pofs;                 // stored pressure is p - pofs (mixed: the default pressure, else 0)
   int mixavg;
   double alamda, rlamda, prantl, tdifgb, pdifgb;

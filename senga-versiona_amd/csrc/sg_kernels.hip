@@ -1944,6 +1944,37 @@ __device__ __forceinline__ void final_point(const Params &P, const Fields &F, co
     for (int vv2 = 0; vv2 < 5 + NS; vv2++) out[vv2 * fs + idx] = Rv[vv2];
     return;   // no register update
   }
+  if (sizeof(R) == 4 && mode == 1 && P.renorm) {
+    // mixed form with renormalisation: FINCOM, then the species of the new
+    // solution rescaled in FP64 so that they sum to the new density
+    double nw[5 + NS];
+#pragma unroll
+    for (int vv2 = 0; vv2 < 5 + NS; vv2++) {
+      const int slot = vv2 < 5 ? vv2 : 5;
+      const double old = fld(F.run + vv2 * fs, bo8);
+      if (i >= P.rklo[slot]) {
+        if (F.erk) fld(F.erk + vv2 * fs, bo8) += erc * (double)Rv[vv2];
+        nw[vv2] = old + lhs * (double)Rv[vv2];
+      } else {
+        nw[vv2] = old;
+      }
+    }
+    if (i >= P.rklo[5]) {
+      double sy = 0.0;
+#pragma unroll
+      for (int s = 0; s < NS; s++) sy = sy + nw[5 + s];
+      const double f = nw[0] / sy;
+#pragma unroll
+      for (int s = 0; s < NS; s++) nw[5 + s] = nw[5 + s] * f;
+    }
+#pragma unroll
+    for (int vv2 = 0; vv2 < 5 + NS; vv2++) {
+      const int slot = vv2 < 5 ? vv2 : 5;
+      if (i >= P.rklo[slot]) fld(F.run + vv2 * fs, bo8) = nw[vv2];
+      fld(F.q + vv2 * fs, bo8) = i >= P.rklo[slot] ? nw[vv2] : (double)Rv[vv2];
+    }
+    return;
+  }
 #pragma unroll
   for (int vv2 = 0; vv2 < 5 + NS; vv2++) {
     const int slot = vv2 < 5 ? vv2 : 5;

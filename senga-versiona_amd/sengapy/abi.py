@@ -133,7 +133,8 @@ def physics_bits(ctl) -> int:
 
 def config_from_case(case, precision=0) -> Config:
     """Fill sg_config from a Case (control + derived tables); precision 0 is
-    FP64, 1 the mixed FP32/FP64 mode."""
+    FP64, 1 the mixed FP32/FP64 mode, 2 the mixed mode with the FP64 species
+    renormalisation at the end of every step."""
     c = Config()
     ctl, t, m = case.control, case.tables, case.tables.mech
     ns, nst = t.nspec, t.nstep

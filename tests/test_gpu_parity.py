@@ -254,3 +254,27 @@ def test_mixed_precision_against_fp64(kind, n, steps):
     run_o, T_o = _oracle_after(c, steps)
     assert_state_close("fp64 vs oracle", run64, T64, run_o, T_o, TOL, 1e-12, tol_pw=PW_TOL64)
     assert_state_close("mixed vs oracle", run32, T32, run_o, T_o, 1e-5, 1e-5)
+
+
+def test_mixed_renormalisation():
+    """precision 2: the mixed form with the FP64 species renormalisation at the
+    end of every step.  After 20 steps sum_k rho Y_k equals rho to round-off
+    in the solution register, and the state still meets the mixed bar against
+    the oracle; the plain mixed form (precision 1) drifts further from the
+    constraint, which the reference does not enforce either."""
+    from sengapy import case as C
+    n = (40, 24, 24)
+    c = C.h2_nscbc(n=n, length=(4e-3, 2.4e-3, 2.4e-3), mixavg=True, nsteps=20)
+    print("case", c.name, "20 steps, precision 1 and 2", flush=True)
+    run_o, T_o = _oracle_after(c, 20)
+    drift = {}
+    for prec in (1, 2):
+        run, T = _gpu_after(c, 20, prec)
+        dev = np.abs(run[5:].sum(axis=0) / run[0] - 1.0)
+        drift[prec] = float(dev.max())
+        assert_state_close(f"precision {prec} vs oracle", run, T, run_o, T_o, 1e-5, 1e-5)
+    dev_o = float(np.abs(run_o[5:].sum(axis=0) / run_o[0] - 1.0).max())
+    print("max |sum rho Y / rho - 1|: oracle", f"{dev_o:.1e}", "mixed", f"{drift[1]:.1e}",
+          "mixed + renormalisation", f"{drift[2]:.1e}", flush=True)
+    assert drift[2] < 1e-14
+    assert drift[1] > drift[2]
