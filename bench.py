@@ -64,11 +64,17 @@ def copy_calibration(h, nbytes=1 << 30):
     print(f"copy calibration: {nbytes} bytes device-to-device", file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name, ns, bounded):
+def kernel_bytes(name, ns, bounded, nx=128):
     """Algorithmic bytes per grid point of one launch: each distinct input
-    field read once and each output written once (FP64 fields, 4-byte index)."""
+    field read once and each output written once (FP64 fields, 4-byte index).
+    The face kernels touch the nx = 1 and nx = n planes only: their bytes are
+    spread over the grid (divided by nx)."""
     f = 8
     nprim = 9 + 3 * ns
+    if name in ("boundt", "bountt"):   # inflow values of 5+N fields at x-left
+        return f * (5 + ns) / nx
+    if name == "nscbc":                # face data (6+N) + stage register (5+N) + corrections (5+N), two faces
+        return 2 * f * ((6 + ns) + 2 * (5 + ns)) / nx
     if name == "prim":          # reads stage (5+N), T, index; writes primitives, rate, index
         return f * (5 + ns + 1) + 4 + f * (nprim + ns) + 4
     if name == "pass_x":        # rho, 3 momenta, primitives, rate -> accumulators, 4 gradients
@@ -78,6 +84,73 @@ def kernel_bytes(name, ns, bounded):
     if name == "final":         # accumulators, 12 gradients, u,v,w,mu, run (r/w), stage (w)
         return f * (5 + ns) + f * 12 + f * 4 + 2 * f * (5 + ns) + f * (5 + ns)
     return 0
+
+
+FP64_PEAK = 78.6e12   # MI355X vector FP64, FLOP/s
+
+
+def _profile_json(name, n, prec):
+    """Per-kernel entries of a committed rocprofv3 summary (profiles/NAME.json)
+    for the h2_nscbc_mix n^3 workload and precision tag, or {}."""
+    path = os.path.join(ROOT, "profiles", name + ".json")
+    if not os.path.exists(path):
+        return {}
+    d = json.load(open(path))
+    if d.get("workload") != f"h2_nscbc_mix_{n}^3":
+        return {}
+    return d.get(prec, {})
+
+
+def kernel_table(kern, ns, npts, n, prec):
+    """Per kernel: mean launch time (events on the library's stream),
+    algorithmic bytes per point and the HBM rate and fraction they give,
+    counted FP64 + FP32 flops per point and bytes moved per point (committed
+    rocprofv3 counters of the same workload), and the kernel's floor
+    max(algorithmic bytes / 8 TB/s, counted flops / 78.6 TFLOP/s)."""
+    valu = _profile_json("valu", n, prec)
+    traf = _profile_json("traffic", n, prec)
+    out = {}
+    for k, (tot, cnt) in kern.items():
+        ms = tot / cnt
+        ab = kernel_bytes(k, ns, True, n)
+        e = {"ms": ms, "algorithmic_bytes_per_point": ab,
+             "hbm_achieved_gbs": ab * npts / (ms / 1e3) / 1e9,
+             "hbm_frac": ab * npts / (ms / 1e3) / HBM_PEAK}
+        v = valu.get(k)
+        fl = None
+        if v:
+            fl = v["fp64_flops_per_point"] + v["fp32_flops_per_point"]
+            e["flops_per_point"] = fl
+            e["valu_issue_use"] = v["valu_issue_util"]
+        if k in traf:
+            e["traffic_bytes_per_point"] = traf[k]
+        t_hbm = ab * npts / HBM_PEAK * 1e3
+        t_alu = fl * npts / FP64_PEAK * 1e3 if fl is not None else 0.0
+        e["floor_ms"] = max(t_hbm, t_alu)
+        e["floor_bound"] = "fp64 valu" if t_alu > t_hbm else "hbm"
+        e["floor_frac"] = e["floor_ms"] / ms
+        out[k] = e
+    return out
+
+
+def roofline_of(e, name, npts):
+    """The roofline object of one kernel, against the bound its floor names:
+    HBM (algorithmic bytes per launch over the mean launch time, 8 TB/s) or
+    the FP64 vector ALU (counted flops per launch over the same time,
+    78.6 TFLOP/s; the HBM fraction is given beside it)."""
+    t = e["ms"] / 1e3
+    traffic = e.get("traffic_bytes_per_point")
+    r = {"kernel": name, "mean_launch_ms": e["ms"], "algorithmic_bytes_per_point": e["algorithmic_bytes_per_point"],
+         "traffic": traffic * npts if traffic is not None else None}
+    if e["floor_bound"] == "fp64 valu":
+        ach = e["flops_per_point"] * npts / t
+        r.update({"bound": "fp64 valu", "achieved": ach / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                  "frac": ach / FP64_PEAK, "flops_per_point": e["flops_per_point"], "hbm_frac": e["hbm_frac"],
+                  "valu_issue_use": e.get("valu_issue_use")})
+    else:
+        r.update({"bound": "hbm", "achieved": e["hbm_achieved_gbs"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                  "frac": e["hbm_frac"]})
+    return r
 
 
 def _baseline_case():
@@ -185,7 +258,41 @@ def mixed_run(abi, c, a, h, s64):
             "max_rel_err_conserved": max(rel(r32[v], r64[v]) for v in range(5 + ns)),
             "steps_compared": a.warmup + a.steps + a.profile_steps,
             "tiles": {"narrow_pass_tiles": list(tuned[0]), "autotuned": not a.no_autotune},
-            "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in prof.items() if v[1] > 0}}
+            "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in prof.items() if v[1] > 0},
+            "kernels": kernel_table({k: v for k, v in prof.items() if v[1] > 0 and kernel_bytes(k, ns, True, c.shape[0]) > 0},
+                                    ns, n3, c.shape[0], "mixed")}
+
+
+def large_run(abi, h, c, steps, warmup=2):
+    """FP64 and mixed throughput on a larger grid of the same physics (state
+    resident, tiles autotuned), timed with events on the library's stream."""
+    n3 = c.shape[0] * c.shape[1] * c.shape[2]
+    out = {"grid": list(c.shape), "steps": steps}
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    h.hipEventCreate(C.byref(ev0))
+    h.hipEventCreate(C.byref(ev1))
+    for tag, prec in (("fp64", 0), ("mixed", 1)):
+        s = abi.Solver(c, precision=prec)
+        s.set_timestep(c.dump.tstep)
+        s.upload(c.dump.state, 0.0)
+        tiles, _ = s.autotune(3)
+        s.step(warmup)
+        s.sync()
+        stream = C.c_void_p(s.stream())
+        h.hipEventRecord(ev0, stream)
+        s.step(steps)
+        h.hipEventRecord(ev1, stream)
+        h.hipEventSynchronize(ev1)
+        s.sync()
+        ms = C.c_float(0)
+        h.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+        out[tag] = {"value": round(n3 * 5 * steps / (ms.value / 1e3) / 1e6, 3), "unit": "Mpoint-RK-substages/s",
+                    "ms_per_step": round(ms.value / steps, 4), "device_bytes": s.memory(),
+                    "narrow_pass_tiles": list(tiles)}
+        s.close()
+        print(f"{c.shape} {tag}: {ms.value:.2f} ms for {steps} steps", file=sys.stderr, flush=True)
+    out["speedup_mixed"] = out["mixed"]["value"] / out["fp64"]["value"]
+    return out
 
 
 def _make(spec):
@@ -287,7 +394,7 @@ def precision_study(abi, h, c, steps):
     n3 = c.shape[0] * c.shape[1] * c.shape[2]
     res = {}
     states = {}
-    for tag, prec in (("fp64", 0), ("mixed", 1)):
+    for tag, prec in (("fp64", 0), ("mixed", 1), ("mixed_renorm", 2)):
         s = abi.Solver(c, precision=prec)
         s.set_timestep(c.dump.tstep)
         s.upload(c.dump.state, 0.0)
@@ -306,29 +413,33 @@ def precision_study(abi, h, c, steps):
         states[tag] = s.download()[:3]
         s.close()
     names = ["rho", "rho_u", "rho_v", "rho_w", "rho_E"] + [f"rho_Y_{k}" for k in range(c.nspec)]
-    err = {}
     r64, T64, P64 = states["fp64"]
-    r32, T32, P32 = states["mixed"]
 
     def both(x, y):
         d = x - y
         return {"max_rel": float(np.max(np.abs(d)) / max(float(np.max(np.abs(y))), 1e-300)),
                 "l2_rel": float(np.sqrt(np.sum(d * d)) / max(float(np.sqrt(np.sum(y * y))), 1e-300))}
 
-    for v, nm in enumerate(names):
-        err[nm] = both(r32[v], r64[v])
-    err["T"] = both(T32, T64)
-    err["p"] = both(P32, P64)
-    # mass fractions Y_k = rho Y_k / rho
-    ymax = 0.0
-    for k in range(c.nspec):
-        e = both(r32[5 + k] / r32[0], r64[5 + k] / r64[0])
-        err[f"Y_{k}"] = e
-        ymax = max(ymax, e["max_rel"])
-    res["speedup_mixed"] = res["mixed"]["mpoint_substages_per_s"] / res["fp64"]["mpoint_substages_per_s"]
-    res["errors_mixed_vs_fp64"] = err
-    res["max_rel_T"] = err["T"]["max_rel"]
-    res["max_rel_Y"] = ymax
+    for tag in ("mixed", "mixed_renorm"):
+        r32, T32, P32 = states[tag]
+        err = {}
+        for v, nm in enumerate(names):
+            err[nm] = both(r32[v], r64[v])
+        err["T"] = both(T32, T64)
+        err["p"] = both(P32, P64)
+        # mass fractions Y_k = rho Y_k / rho
+        ymax = 0.0
+        for k in range(c.nspec):
+            e = both(r32[5 + k] / r32[0], r64[5 + k] / r64[0])
+            err[f"Y_{k}"] = e
+            ymax = max(ymax, e["max_rel"])
+        sfx = "" if tag == "mixed" else "_renorm"
+        res["speedup_mixed" + sfx] = res[tag]["mpoint_substages_per_s"] / res["fp64"]["mpoint_substages_per_s"]
+        res["errors_mixed" + sfx + "_vs_fp64"] = err
+        res["max_rel_T" + sfx] = err["T"]["max_rel"]
+        res["max_rel_Y" + sfx] = ymax
+        res["species_sum_drift" + sfx] = float(np.abs(r32[5:].sum(axis=0) / r32[0] - 1.0).max())
+    res["species_sum_drift_fp64"] = float(np.abs(r64[5:].sum(axis=0) / r64[0] - 1.0).max())
     return res
 
 
@@ -354,6 +465,9 @@ def main():
     ap.add_argument("--serve-point-budget", type=float, default=6e6,
                     help="grid points in flight beyond which the serving loop queues requests")
     ap.add_argument("--serve-chunk", type=int, default=5, help="RK steps between admission points")
+    ap.add_argument("--large-n", type=int, default=256,
+                    help="grid of the second FP64/mixed throughput figure (0: skip)")
+    ap.add_argument("--large-steps", type=int, default=5)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tiles (profiling runs: no tuning launches in the trace)")
     ap.add_argument("--study-steps", type=int, default=0,
@@ -379,7 +493,10 @@ def main():
         return
     bspecs = Cs.batch_specs(a.batch_cases) if a.batch_cases > 0 else []
     sspecs = Cs.kernel_specs(a.serve_requests, rate=a.serve_rate, nsteps=a.serve_steps) if a.serve_requests > 0 else []
-    built = make_cases(bspecs + sspecs) if bspecs or sspecs else []
+    lspecs = ([dict(kind="nscbc", n=(a.large_n,) * 3, phi=0.7, uprime_over_sl=5.0, Tu=300.0, seed=7, nsteps=100)]
+              if a.large_n > 0 else [])
+    built = make_cases(lspecs + bspecs + sspecs) if bspecs or sspecs or lspecs else []
+    lcases, built = built[:len(lspecs)], built[len(lspecs):]
     bcases, scases = built[:len(bspecs)], built[len(bspecs):]
     n = a.n
     t0 = time.perf_counter()
@@ -422,42 +539,27 @@ def main():
     prof = s.profile_read()
     s.profile(False)
     ns = c.nspec
-    kern = {k: v for k, v in prof.items() if v[1] > 0 and kernel_bytes(k, ns, True) > 0}
-    dom = max(kern, key=lambda k: kern[k][0])
-    dom_ms = kern[dom][0] / kern[dom][1]
-    dom_bytes = kernel_bytes(dom, ns, True) * npts
-    achieved = dom_bytes / (dom_ms / 1e3)
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tfile):
-        t = json.load(open(tfile))
-        if t.get("workload") == f"h2_nscbc_mix_{n}^3" and dom in t.get("kernels", {}):
-            traffic = t["kernels"][dom]
-    # The point-physics kernel is bound by the vector ALU, not HBM: its FP64
-    # rate and VALU issue use from the per-type instruction counters of the
-    # same workload (committed rocprofv3 summary), beside the HBM figure.
-    valu = None
-    vfile = os.path.join(ROOT, "profiles", "valu.json")
-    if os.path.exists(vfile):
-        vj = json.load(open(vfile))
-        which = "true" if s.mechanism_compiled() else "false"   # compiled or table-driven step loop
-        k = vj.get("kernels", {}).get(f"k_prim<{ns},{which}>")
-        if vj.get("workload") == f"h2_nscbc_mix_{n}^3" and k:
-            valu = {"kernel": "prim", "bound": "fp64 valu", "achieved": k["fp64_tflops"], "peak": 78.6,
-                    "unit": "TFLOP/s", "frac": k["fp64_tflops"] / 78.6, "valu_issue_use": k["valu_issue_util"],
-                    "source": "profiles/valu_h2_nscbc_mix_128.md (rocprofv3 --pmc)"}
+    kern = {k: v for k, v in prof.items() if v[1] > 0 and kernel_bytes(k, ns, True, n) > 0}
+    table = kernel_table(kern, ns, npts, n, "fp64")
+    dom = max(table, key=lambda k: table[k]["ms"])
+    roof = roofline_of(table[dom], dom, npts)
     substage_bytes = 32 * (5 + ns) + 16      # compulsory state traffic of one fused substage
-    # HBM bytes one substage actually moves (the per-kernel rocprofv3 FETCH_SIZE
-    # + WRITE_SIZE of the same workload) over the measured substage time
-    hbm_sub = None
-    if os.path.exists(tfile):
-        t = json.load(open(tfile))
-        if t.get("workload") == f"h2_nscbc_mix_{n}^3":
-            per = sum(t["kernels"].get(k, 0.0) for k in ("prim", "pass_x", "pass_y", "pass_z", "nscbc", "final"))
-            sub_s = ms.value / a.steps / 5 / 1e3
-            hbm_sub = {"traffic_bytes": per, "bytes_per_point": per / npts, "achieved": per / sub_s / 1e9,
-                       "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": per / sub_s / HBM_PEAK,
-                       "source": "profiles/traffic_h2_nscbc_mix_128.md (rocprofv3 --pmc, default tiles)"}
+    sub_ms = ms.value / a.steps / 5
+    floor_ms = sum(v["floor_ms"] for v in table.values())
+    meas_ms = sum(v["ms"] for v in table.values())
+    traffic = [v.get("traffic_bytes_per_point") for v in table.values()]
+    substage = {
+        "measured_ms": sub_ms,
+        "kernel_ms_sum": meas_ms,
+        "floor_ms": floor_ms,
+        "floor_frac": floor_ms / sub_ms,
+        "compulsory_bytes_per_point": substage_bytes,
+        "compulsory_hbm_frac": substage_bytes * value * 1e6 / HBM_PEAK,
+        "moved_bytes_per_point": sum(traffic) if all(t is not None for t in traffic) else None,
+        "note": "floor_ms: sum over kernels of max(algorithmic bytes / 8 TB/s, counted flops / 78.6 TFLOP/s)",
+    }
+    if substage["moved_bytes_per_point"] is not None:
+        substage["moved_hbm_frac"] = substage["moved_bytes_per_point"] * npts / (sub_ms / 1e3) / HBM_PEAK
     out = {
         "metric": "Mpoint-RK-substages/s",
         "value": round(value, 3),
@@ -474,18 +576,19 @@ def main():
         "config": {"workload": f"H2-air 9-species 21-step, {n}^3, NSCBC x inflow/outflow, periodic y/z, "
                                "mixture-averaged transport, FP64, RK4(3)5",
                    "grid": [n, n, n], "nspec": ns, "nstep_chem": c.tables.nstep, "tstep": c.dump.tstep},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "algorithmic_bytes_per_point": kernel_bytes(dom, ns, True),
-                     "mean_launch_ms": dom_ms},
-        "valu_roofline": valu,
-        "substage_roofline_frac": substage_bytes * value * 1e6 / HBM_PEAK,
-        "substage_hbm": hbm_sub,
-        "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kern.items()},
+        "roofline": roof,
+        "substage": substage,
+        "kernels": table,
+        "kernel_ms_per_launch": {k: round(v["ms"], 4) for k, v in table.items()},
         "tiles": {"narrow_pass_tiles": list(tuned[0]), "autotuned": not a.no_autotune},
     }
     if not a.no_mixed:
         out["mixed"] = mixed_run(abi, c, a, h, s)
+    s.close()
+    if lcases:
+        out["large"] = large_run(abi, h, lcases[0], a.large_steps)
+        out["large"]["workload"] = (f"the same H2-air NSCBC physics on {a.large_n}^3 (grid spacing of the 128^3 "
+                                    f"/ 5 mm case), {a.large_steps} RK steps timed after 2, tiles autotuned")
     if bcases:
         out["batched"] = batched_run(abi, h, bcases, a.batch_steps)
         out["batched"]["workload"] = (f"{len(bcases)} H2-air NSCBC cases, n^3 with n from {{48,64,96,128}}, "

@@ -296,11 +296,29 @@ __device__ __forceinline__ float fma_u(float a, float b, float c) {
   return r;
 }
 
-// Horner over the zero-padded SG_NCPMAX-term polynomial of interval it.  Both
-// intervals are evaluated with uniform (scalar) coefficients and the result
-// is chosen per point, which is cheaper than choosing every coefficient.
+// The temperature interval it when every active lane of the wave holds the
+// same one, else -1; wave-uniform (from ballots), so the polynomials below
+// then run once with scalar coefficients.
+__device__ __forceinline__ int wave_interval(int it) {
+  const unsigned long long on = __ballot(it != 0), all = __ballot(1);
+  return __builtin_amdgcn_readfirstlane(on == 0ull ? 0 : (on == all ? 1 : -1));
+}
+
+// Horner over the zero-padded SG_NCPMAX-term polynomial of interval it.
+// When the wave disagrees on it, both intervals are evaluated with uniform
+// (scalar) coefficients and the result is chosen per point, which is cheaper
+// than choosing every coefficient.  Both forms compute the same sums.  (The
+// enthalpy and Gibbs polynomials below keep the two-interval form only: with
+// the wave-uniform branch there too the FP64 point kernel spilled.)
 __device__ __forceinline__ double horner(const double (&a)[SG_NTINT][SG_NCPMAX], int it, double x) {
   static_assert(SG_NTINT == 2, "two temperature intervals");
+  const int u = wave_interval(it);
+  if (u >= 0) {
+    double f = a[u][SG_NCPMAX - 1];
+#pragma unroll
+    for (int m = SG_NCPMAX - 2; m >= 0; m--) f = fma_u(f, x, a[u][m]);
+    return f;
+  }
   double f0 = a[0][SG_NCPMAX - 1], f1 = a[1][SG_NCPMAX - 1];
 #pragma unroll
   for (int m = SG_NCPMAX - 2; m >= 0; m--) {
@@ -339,6 +357,43 @@ __device__ __forceinline__ double gibbs_of(const Tables *tb, int s, int it, doub
   return it ? g1 : g0;
 }
 
+// TEMPER's polynomial sums (src/temper.f:92-128 with the coefficients of
+// src/chemin.f:906-916): species s of mass density y in interval it adds its
+// terms to tc (E(T) - E) and td (dE/dT).  A wave that agrees on it adds them
+// with that interval's scalar coefficients; otherwise y * c(interval) is
+// formed as y0 * c(0) + y1 * c(1) with exactly one of y0, y1 zero: the zero
+// term adds an exact zero, so every sum is the one formed with the chosen
+// interval alone, and the coefficients stay uniform.  Terms above the
+// interval's order add y * 0, which leaves the sums unchanged.
+__device__ __forceinline__ void temper_add(const Tables *tb, int s, int it, double y, double (&tc)[6],
+                                           double (&td)[6]) {
+  const int u = wave_interval(it);
+  if (u >= 0) {
+    tc[0] = tc[0] + y * tb->hcp[s][u];
+    tc[1] = tc[1] + y * tb->ct[s][u][0];
+    td[1] = tc[1];
+#pragma unroll
+    for (int m = 2; m <= SG_NCPMAX; m++) {
+      tc[m] = tc[m] + y * tb->ct[s][u][m - 1];
+      td[m] = td[m] + y * tb->cp[s][u][m - 1];
+    }
+    return;
+  }
+  const double y0 = it ? 0.0 : y, y1 = it ? y : 0.0;
+  tc[0] = tc[0] + y0 * tb->hcp[s][0];
+  tc[0] = tc[0] + y1 * tb->hcp[s][1];
+  tc[1] = tc[1] + y0 * tb->ct[s][0][0];
+  tc[1] = tc[1] + y1 * tb->ct[s][1][0];
+  td[1] = tc[1];
+#pragma unroll
+  for (int m = 2; m <= SG_NCPMAX; m++) {
+    tc[m] = tc[m] + y0 * tb->ct[s][0][m - 1];
+    tc[m] = tc[m] + y1 * tb->ct[s][1][m - 1];
+    td[m] = td[m] + y0 * tb->cp[s][0][m - 1];
+    td[m] = td[m] + y1 * tb->cp[s][1][m - 1];
+  }
+}
+
 // Transport fits, zero-padded to SG_NTRC terms.
 __device__ __forceinline__ double trfit(const double *a, double x) {
   double f = a[SG_NTRC - 1];
@@ -351,8 +406,9 @@ __device__ __forceinline__ double trfit(const double *a, double x) {
 // exp(x) in FP64 by Tang's table-driven method (P. T. P. Tang, "Table-driven
 // implementation of the exponential function in IEEE floating-point
 // arithmetic", ACM TOMS 15(2), 1989): x = (64 m + j) ln2/64 + r, |r| <= ln2/128,
-// exp(x) = 2^m 2^(j/64) exp(r), with exp(r) - 1 from its degree-6 Taylor
-// polynomial (truncation < 3e-20).  n ln2/64 is split so that n * L1 is exact.
+// exp(x) = 2^m 2^(j/64) exp(r), with exp(r) - 1 from its degree-5 Taylor
+// polynomial (truncation < 4e-17, a third of an ulp).  n ln2/64 is split so
+// that n * L1 is exact.
 // xt is the 2^(j/64) table (kExp2j) staged in LDS.  Domain: finite x with
 // |x| < 2^31 ln2/64 (about 2.3e7), where n fits the 32-bit exponent split.
 // There it is within 1 ulp of a correctly rounded exp (measured against glibc
@@ -360,17 +416,20 @@ __device__ __forceinline__ double trfit(const double *a, double x) {
 // -745.2; infinite arguments (which would give NaN) are outside the domain,
 // and the kernels never pass them: every call's argument is a rate exponent,
 // a transport fit or a falloff term of a temperature that TEMPER bounds.
-// NaN propagates.  15 VALU instructions and one LDS read, against ~22 for the
+// NaN propagates.  13 VALU instructions and one LDS read, against ~22 for the
 // library exp.
 static constexpr double kExp2j[64] = {0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0, 0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0, 0x1.172b83c7d517bp+0, 0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0, 0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0, 0x1.2d285a6e4030bp+0, 0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0, 0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0, 0x1.3dea64c123422p+0, 0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0, 0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0, 0x1.56f4736b527dap+0, 0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0, 0x1.6247eb03a5585p+0, 0x1.6623882552225p+0, 0x1.6a09e667f3bcdp+0, 0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0, 0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0, 0x1.868d99b4492edp+0, 0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0, 0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0, 0x1.9c49182a3f090p+0, 0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0, 0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0, 0x1.bcc1e904bc1d2p+0, 0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0, 0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0, 0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0, 0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0};
 __device__ __forceinline__ double exp_t(double x, const double *xt) {
   const double INV = 0x1.71547652b82fep+6, L1 = 0x1.62e42ff000000p-7, L2 = -0x1.718432a1b0e26p-41;
-  const double n = __builtin_rint(x * INV);
-  const int ni = (int)n;
+  // n = nearest integer to x*64/ln2 by the shifter 1.5*2^52: the fused sum
+  // rounds to an integer, whose low 32 bits are n (two's complement)
+  const double SH = 0x1.8p52;
+  const double tsh = __builtin_fma(x, INV, SH);
+  const double n = tsh - SH;
+  const int ni = (int)(unsigned)__builtin_bit_cast(unsigned long long, tsh);
   double r = __builtin_fma(-n, L1, x);
   r = __builtin_fma(-n, L2, r);
-  double q = __builtin_fma(r, 1.0 / 720.0, 1.0 / 120.0);
-  q = __builtin_fma(q, r, 1.0 / 24.0);
+  double q = __builtin_fma(r, 1.0 / 120.0, 1.0 / 24.0);
   q = __builtin_fma(q, r, 1.0 / 6.0);
   q = __builtin_fma(q, r, 0.5);
   const double p = __builtin_fma(q, r * r, r);
@@ -605,7 +664,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   __shared__ double xt[64];
   if (threadIdx.x < 64) xt[threadIdx.x] = kExp2j[threadIdx.x];
   __syncthreads();
-  long idx = (long)lb * BLK + threadIdx.x;
+  const long idx = (long)lb * BLK + threadIdx.x;
   if (idx >= P.npts) return;
   const uint32_t bo = (uint32_t)idx * 8u;
   const long fs = P.fstride;
@@ -627,24 +686,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   uint32_t idxo = fld(F.itndex, bo / 2u);
 #pragma unroll
   for (int s = 0; s < NS; s++) {
-    // y * c(interval) as y0 * c(0) + y1 * c(1) with exactly one of y0, y1
-    // zero: the zero term adds an exact zero, so every sum is the one formed
-    // with the chosen interval alone, and the coefficients stay uniform
-    const int it = itint_of(idxo, s);
-    const double y0 = it ? 0.0 : ry[s], y1 = it ? ry[s] : 0.0;
-    tc[0] = tc[0] + y0 * tb->hcp[s][0];
-    tc[0] = tc[0] + y1 * tb->hcp[s][1];
-    tc[1] = tc[1] + y0 * tb->ct[s][0][0];
-    tc[1] = tc[1] + y1 * tb->ct[s][1][0];
-    td[1] = tc[1];
-    // terms above the interval's order add y*0, which leaves the sums unchanged
-#pragma unroll
-    for (int m = 2; m <= SG_NCPMAX; m++) {
-      tc[m] = tc[m] + y0 * tb->ct[s][0][m - 1];
-      tc[m] = tc[m] + y1 * tb->ct[s][1][m - 1];
-      td[m] = td[m] + y0 * tb->cp[s][0][m - 1];
-      td[m] = td[m] + y1 * tb->cp[s][1][m - 1];
-    }
+    temper_add(tb, s, itint_of(idxo, s), ry[s], tc, td);
     s7 = s7 + ry[s] * c->rgspec[s];
   }
   const double toltmp = (double)1.0e-10f;  // REAL literal, src/temper.f:33-34
@@ -1012,7 +1054,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   const Member &M = member_at<L_POINT>(mem, map, blockIdx.x, lb);
   const Params P = M.P;
   const Fields F = global_fields(M.F);
-  long idx = (long)lb * BLK + threadIdx.x;
+  const long idx = (long)lb * BLK + threadIdx.x;
   if (idx >= P.npts) return;
   const uint32_t bo = (uint32_t)idx * 8u, b4 = (uint32_t)idx * 4u;
   const long fs = P.fstride;
@@ -1034,24 +1076,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   uint32_t idxo = fld(F.itndex, bo / 2u);
 #pragma unroll
   for (int s = 0; s < NS; s++) {
-    // y * c(interval) as y0 * c(0) + y1 * c(1) with exactly one of y0, y1
-    // zero: the zero term adds an exact zero, so every sum is the one formed
-    // with the chosen interval alone, and the coefficients stay uniform
-    const int it = itint_of(idxo, s);
-    const double y0 = it ? 0.0 : ry[s], y1 = it ? ry[s] : 0.0;
-    tc[0] = tc[0] + y0 * tb->hcp[s][0];
-    tc[0] = tc[0] + y1 * tb->hcp[s][1];
-    tc[1] = tc[1] + y0 * tb->ct[s][0][0];
-    tc[1] = tc[1] + y1 * tb->ct[s][1][0];
-    td[1] = tc[1];
-    // terms above the interval's order add y*0, which leaves the sums unchanged
-#pragma unroll
-    for (int m = 2; m <= SG_NCPMAX; m++) {
-      tc[m] = tc[m] + y0 * tb->ct[s][0][m - 1];
-      tc[m] = tc[m] + y1 * tb->ct[s][1][m - 1];
-      td[m] = td[m] + y0 * tb->cp[s][0][m - 1];
-      td[m] = td[m] + y1 * tb->cp[s][1][m - 1];
-    }
+    temper_add(tb, s, itint_of(idxo, s), ry[s], tc, td);
     s7 = s7 + ry[s] * c->rgspec[s];
   }
   const double toltmp = (double)1.0e-10f;  // REAL literal, src/temper.f:33-34

@@ -21,12 +21,15 @@ PROGRAM = r"""
 static const double TAB[64] = {%(tab)s};
 static double exp_t(double x) {
   const double INV = %(inv)s, L1 = %(l1)s, L2 = %(l2)s;
-  double n = rint(x * INV);
-  int ni = (int)n;
+  const double SH = 0x1.8p52;
+  double tsh = fma(x, INV, SH);
+  double n = tsh - SH;
+  uint64_t bits;
+  memcpy(&bits, &tsh, 8);
+  int ni = (int)(uint32_t)bits;
   double r = fma(-n, L1, x);
   r = fma(-n, L2, r);
-  double q = fma(r, 1.0 / 720.0, 1.0 / 120.0);
-  q = fma(q, r, 1.0 / 24.0);
+  double q = fma(r, 1.0 / 120.0, 1.0 / 24.0);
   q = fma(q, r, 1.0 / 6.0);
   q = fma(q, r, 0.5);
   double p = fma(q, r * r, r);
