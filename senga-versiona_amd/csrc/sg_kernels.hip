@@ -394,6 +394,24 @@ __device__ __forceinline__ void temper_add(const Tables *tb, int s, int it, doub
   }
 }
 
+// Reciprocals of a[0..N-1] in place with one division (prefix products,
+// Montgomery's trick): 3 (N - 1) products instead of N - 1 more divisions.
+template <int N>
+__device__ __forceinline__ void batch_recip(double (&a)[N]) {
+  double pre[N];
+  pre[0] = a[0];
+#pragma unroll
+  for (int k = 1; k < N; k++) pre[k] = pre[k - 1] * a[k];
+  double inv = 1.0 / pre[N - 1];
+#pragma unroll
+  for (int k = N - 1; k > 0; k--) {
+    const double ak = a[k];
+    a[k] = inv * pre[k - 1];
+    inv = inv * ak;
+  }
+  a[0] = inv;
+}
+
 // Transport fits, zero-padded to SG_NTRC terms.
 __device__ __forceinline__ double trfit(const double *a, double x) {
   double f = a[SG_NTRC - 1];
@@ -834,12 +852,16 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
     for (int js = 0; js < NS; js++) c1t = c1t + yt[js];
     // The reference's term (Y_j + tol) W_j^-1 / CTRANS, CTRANS = exp(f) p_g / p,
-    // is formed as (Y_j + tol) W_j^-1 * exp(-f) * (p / p_g): no division per
-    // pair (a few ulp from the reference's operation order).
+    // is formed as ((Y_j + tol) W_j^-1 (p / p_g)) * exp(-f): no division or
+    // pressure product per pair (a few ulp from the reference's operation
+    // order), and the species' final divisions share one (batch_recip).
     const double ppd = pres / P.pdifgb;
     double yw2[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) yw2[s] = yt[s] * c->ovwmol[s];
+    double yw2p[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) yw2p[s] = yw2[s] * ppd;
     if (tb->diffsym) {
       // D_ij = D_ji: one fit per unordered pair, its term added to both
       // sums.  Visiting pairs (a, b >= a) row by row delivers the terms of
@@ -851,24 +873,23 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       for (int a = 0; a < NS; a++) {
 #pragma unroll
         for (int b = a; b < NS; b++) {
-          const double ictd = exp_t(-trfit(c->diffco[a][b], transp), xt) * ppd;
-          const double tab = yw2[b] * ictd;
+          const double ictd = exp_t(-trfit(c->diffco[a][b], transp), xt);
+          const double tab = yw2p[b] * ictd;
           cs[a] = cs[a] + tab;
           if (b == a) {
             self[a] = tab;
           } else {
-            const double tba = yw2[a] * ictd;
+            const double tba = yw2p[a] * ictd;
             cs[b] = cs[b] + tba;
           }
         }
       }
+      double c2[NS];
 #pragma unroll
-      for (int s = 0; s < NS; s++) {
-        double c1 = c1t - yt[s];
-        double c2 = cs[s] - self[s];
-        c2 = c2 * wmomix;
-        fld(pr + prim_rd(P, s), bo) = d * c1 / c2;
-      }
+      for (int s = 0; s < NS; s++) c2[s] = (cs[s] - self[s]) * wmomix;
+      batch_recip<NS>(c2);
+#pragma unroll
+      for (int s = 0; s < NS; s++) fld(pr + prim_rd(P, s), bo) = d * (c1t - yt[s]) * c2[s];
     } else {
 #pragma unroll
       for (int s = 0; s < NS; s++) {
@@ -887,16 +908,20 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     SG_PHASE();
     // Wilke viscosity (src/rhscal.f:3854-3891).  sqrt(mu_s / mu_j) is formed
-    // as sqrt(mu_s) / sqrt(mu_j), one square root and reciprocal per species.
+    // as sqrt(mu_s) / sqrt(mu_j), with sqrt(mu_s) = exp(fit_s / 2) and its
+    // reciprocal exp(-fit_s / 2) (two exps instead of exp, square root and
+    // division), mu_s = sqrt(mu_s)^2; the divisions mu_s / sum_s share one
+    // (batch_recip).
     double mus[NS], sq[NS], rsq[NS], yw[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      mus[s] = exp_t(trfit(c->viscco[s], transp), xt);
-      sq[s] = sqrt(mus[s]);
-      rsq[s] = 1.0 / sq[s];
+      const double hf = 0.5 * trfit(c->viscco[s], transp);
+      sq[s] = exp_t(hf, xt);
+      rsq[s] = exp_t(-hf, xt);
+      mus[s] = sq[s] * sq[s];
       yw[s] = y[s] * c->ovwmol[s];
     }
-    double c1 = 0.0;
+    double c1 = 0.0, wc2[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       double c2 = 0.0;
@@ -907,9 +932,11 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
         fornow = c->wilko1[s][js] * fornow * fornow;
         c2 = c2 + yw[js] * fornow;
       }
-      double fornow = mus[s] / c2;
-      c1 = c1 + yw[s] * fornow;
+      wc2[s] = c2;
     }
+    batch_recip<NS>(wc2);
+#pragma unroll
+    for (int s = 0; s < NS; s++) c1 = c1 + yw[s] * (mus[s] * wc2[s]);
     fld(pr + P_MU * fs, bo) = c1;
   }
   // face data for NSCBC: cp and mixture gas constant (src/rhscal.f:72-131)
@@ -1373,8 +1400,15 @@ struct PassTile {
   static constexpr int LSTR = DIR == 0 ? 1 : (DIR == 1 ? LX : LX * LY);   // LDS stride along DIR
 };
 
+// Occupancy: FP64 passes need their 128 registers (4 waves per SIMD; at 5
+// the spills made the y and z passes 40% slower).  The FP32 passes are held
+// to 80 registers for 6 waves per SIMD (3 blocks per CU), which the
+// latency-bound staging pipeline turns into 7-10% shorter y and z passes and
+// 2% shorter interior x passes; the bounded-row edge tiles, which would spill
+// heavily at 80, keep 4.
 template <int DIR, bool BND, int NS, class R, int NW, int XP = 0>
-__global__ void __launch_bounds__((PassTile<DIR, NW, XP>::NT)) __attribute__((amdgpu_waves_per_eu(4))) k_pass(const Member *__restrict__ mem,
+__global__ void __launch_bounds__((PassTile<DIR, NW, XP>::NT))
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 ? 6 : 4))) k_pass(const Member *__restrict__ mem,
                                                                                                        const uint32_t *__restrict__ map) {
   static_assert(XP == 0 || (DIR == 0 && (XP == 1) != BND), "interior x tiles take the periodic form, edge tiles the bounded one");
   using T_ = PassTile<DIR, NW, XP>;
