@@ -64,9 +64,11 @@ extern "C" {
 /* sg_config.physics bits: the reference's optional physics switches from
  * cont.dat (src/contin.f:194-209), set when the switch is on -- the four
  * molecular-transport options only together with mixture-averaged transport,
- * as src/diffin.f:66,384-407 applies them.  This library computes none of
- * them: sg_create and sg_footprint return SG_EINVAL when any bit is set, so
- * that such a case is refused instead of run without that physics. */
+ * as src/diffin.f:66,384-407 applies them.  The molar-mass and pressure
+ * diffusion terms are computed (with mixture-averaged transport; without it
+ * they are inactive, as in the reference); Soret, Dufour and radiation are
+ * not: sg_create and sg_footprint return SG_EINVAL when one of those bits is
+ * set, so that such a case is refused instead of run without that physics. */
 #define SG_PHYS_MIXW 1    /* NFMIXW: molar-mass diffusion term (src/rhscal.f:951,1970) */
 #define SG_PHYS_MIXP 2    /* NFMIXP: pressure diffusion term (src/rhscal.f:973,2365) */
 #define SG_PHYS_SORET 4   /* NFMSOR: thermal diffusion of light species (src/rhscal.f:2760) */
@@ -89,7 +91,7 @@ typedef struct sg_config {
                                  has no such renormalisation (src/lincom.f:294-
                                  333 is commented out) and the FP64 form never
                                  applies it. */
-  int32_t physics;            /* SG_PHYS_* bits of switches that are on (all refused) */
+  int32_t physics;            /* SG_PHYS_* bits of switches that are on (MIXW, MIXP computed; others refused) */
   double xgdlen, ygdlen, zgdlen;  /* domain lengths; spacing L/(N-1) (src/dfinit.f:42-50) */
 
   /* boundary conditions, faces xl, xr, yl, yr, zl, zr (cont.dat lines 51-56) */
@@ -168,7 +170,8 @@ size_t sg_config_size(void);
  * part of INDATA/DFINIT/BCINIT.  Returns SG_EINVAL for a species count that is
  * not compiled in (2..SG_MAX_SPEC are), nx/ny/nz < 11, more than 2^29 points
  * per field, cp fits of more than 5 coefficients (src/temper.f NCTMAX),
- * transport fits of more than SG_NTRC coefficients, any SG_PHYS_* bit, or a
+ * transport fits of more than SG_NTRC coefficients, SG_PHYS_SORET, _DUFOUR or
+ * _RADN, or a
  * step list, species index, third-body or falloff index out of range. */
 int sg_create(const sg_config *cfg, sg_handle **out);
 int sg_destroy(sg_handle *h);

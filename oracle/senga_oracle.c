@@ -74,6 +74,10 @@ struct orc {
   double *trun, *prun, *transp, *store7, *utmp, *vtmp, *wtmp, *wmomix, *difmix;
   uint32_t *itndex;   /* ITNDEX: NINTMX = 2 words per point (see itint_get) */
   double *st1, *st2, *st3, *st4, *st5, *st6, *divm, *ucor, *vcor, *wcor;
+  /* first and second derivatives of ln(mixture molar mass) and ln(pressure)
+     (WD1X..WD2Z, PD1X..PD2Z), allocated when NFMIXW / NFMIXP are on */
+  double *wd1[3], *wd2[3], *pd1[3], *pd2[3];
+  int mixw, mixp;
   double *rate, *rrte;
   /* face arrays (ny*nz, and *ns for species) */
   double *fxl, *fxr;
@@ -917,6 +921,18 @@ static int rhscal(orc_t *o) {
     o->vtmp[BI(o, i, j, k)] = ZERO;   /* VTMP is a big array; interior zeroed */
   }
   FOR_B(o) o->wtmp[BI(o, i, j, k)] = ZERO;
+  /* first and second derivatives of ln(mixture molar mass) and ln(pressure),
+     rhscal.f:945-992 */
+  for (int term = 0; term < 2; term++) {
+    if (!(term ? o->mixp : o->mixw)) continue;
+    double **d1 = term ? o->pd1 : o->wd1, **d2 = term ? o->pd2 : o->wd2;
+    FOR_B(o) {
+      size_t b = BI(o, i, j, k);
+      s7[b] = log(term ? o->prun[b] : o->wmomix[b]);
+    }
+    for (int d = 0; d < 3; d++) dfbyd(o, d, s7, d1[d]);
+    for (int d = 0; d < 3; d++) d2fd2(o, d, s7, d2[d]);
+  }
   /* Y: convert, rhscal.f:1013-1028 */
   for (int s = 0; s < ns; s++) {
     double *Y = R(o, 5 + s);
@@ -1046,6 +1062,64 @@ static int rhscal(orc_t *o) {
       size_t p = II(o, i, j, k), b = BI(o, i, j, k);
       double fornow = (s1[p] + s2[p] + s3[p]) * s4[p];
       E[b] = E[b] + fornow * o->utmp[b];
+    }
+    /* molar-mass diffusion (rhscal.f:1970-2364) and pressure diffusion
+       (rhscal.f:2365-2759): the two blocks differ only in the logarithm
+       differentiated and in the factor of DIFMIX*Y.  They find STORE4 holding
+       DIFMIX (moved there for the second-derivative terms above) and STORE5,
+       STORE6 the zeroed y and z species enthalpy gradients, and their
+       "species enthalpy gradient" heat-flux term uses all three as found:
+       DIFMIX in the x component.  The wall terms (FXLDFW ...) need wall
+       boundaries, which this path does not take. */
+    for (int term = 0; term < 2; term++) {
+      if (!(term ? o->mixp : o->mixw)) continue;
+      double **d1 = term ? o->pd1 : o->wd1, **d2 = term ? o->pd2 : o->wd2;
+      FOR_B(o) {
+        size_t b = BI(o, i, j, k);
+        s7[b] = term ? o->difmix[b] * Y[b] * (ONE - c->wmolar[s] / o->wmomix[b]) : o->difmix[b] * Y[b];
+      }
+      FOR_I(o) {
+        size_t p = II(o, i, j, k), b = BI(o, i, j, k);
+        o->ucor[p] = o->ucor[p] + s7[b] * d1[0][p];
+        o->vcor[p] = o->vcor[p] + s7[b] * d1[1][p];
+        o->wcor[p] = o->wcor[p] + s7[b] * d1[2][p];
+      }
+      dfbyd(o, 0, s7, s1);
+      dfbyd(o, 1, s7, s2);
+      dfbyd(o, 2, s7, s3);
+      zero_flags(o, o->fdif, s1, s2, s3);
+      FOR_I(o) {
+        size_t p = II(o, i, j, k);
+        double fornow = s1[p] * d1[0][p] + s2[p] * d1[1][p] + s3[p] * d1[2][p];
+        rt[p] = rt[p] + fornow;
+        vt[BI(o, i, j, k)] = vt[BI(o, i, j, k)] + fornow;
+      }
+      zero_flags(o, o->fadb, s1, s2, s3);
+      FOR_I(o) {
+        size_t p = II(o, i, j, k), b = BI(o, i, j, k);
+        double fornow = s1[p] * d1[0][p] + s2[p] * d1[1][p] + s3[p] * d1[2][p];
+        E[b] = E[b] + fornow * o->utmp[b];
+      }
+      zero_flags(o, o->fdif, s4, s5, s6);
+      zero_flags(o, o->fadb, s4, s5, s6);
+      FOR_I(o) {
+        size_t p = II(o, i, j, k), b = BI(o, i, j, k);
+        double fornow = s4[p] * d1[0][p] + s5[p] * d1[1][p] + s6[p] * d1[2][p];
+        E[b] = E[b] + fornow * s7[b];
+      }
+      zero_flags(o, o->fdif, d2[0], d2[1], d2[2]);
+      FOR_I(o) {
+        size_t p = II(o, i, j, k);
+        double fornow = (d2[0][p] + d2[1][p] + d2[2][p]) * s7[BI(o, i, j, k)];
+        rt[p] = rt[p] + fornow;
+        vt[BI(o, i, j, k)] = vt[BI(o, i, j, k)] + fornow;
+      }
+      zero_flags(o, o->fadb, d2[0], d2[1], d2[2]);
+      FOR_I(o) {
+        size_t p = II(o, i, j, k), b = BI(o, i, j, k);
+        double fornow = (d2[0][p] + d2[1][p] + d2[2][p]) * s7[b];
+        E[b] = E[b] + fornow * o->utmp[b];
+      }
     }
   }
   /* correction velocity terms, rhscal.f:3782-3843 */
@@ -1560,6 +1634,11 @@ orc_t *orc_create(const sg_config *cfg, int *err) {
     *err = SG_EINVAL;
     return NULL;
   }
+  /* Soret, Dufour and radiation are not restated */
+  if (cfg->physics & ~(SG_PHYS_MIXW | SG_PHYS_MIXP)) {
+    *err = SG_EINVAL;
+    return NULL;
+  }
   orc_t *o = (orc_t *)calloc(1, sizeof(orc_t));
   o->c = *cfg;
   o->nx = cfg->nx; o->ny = cfg->ny; o->nz = cfg->nz; o->ns = cfg->nspec;
@@ -1587,6 +1666,14 @@ orc_t *orc_create(const sg_config *cfg, int *err) {
   for (size_t q = 0; q < sizeof(in) / sizeof(in[0]); q++) *in[q] = (double *)calloc(o->ni, sizeof(double));
   o->rate = (double *)calloc(o->ni * o->ns, sizeof(double));
   o->rrte = (double *)calloc(o->ni * o->ns, sizeof(double));
+  /* molar-mass and pressure diffusion act with mixture-averaged transport
+     only (src/diffin.f:384-386) */
+  o->mixw = cfg->mixavg && (cfg->physics & SG_PHYS_MIXW);
+  o->mixp = cfg->mixavg && (cfg->physics & SG_PHYS_MIXP);
+  for (int d = 0; d < 3; d++) {
+    if (o->mixw) { o->wd1[d] = (double *)calloc(o->ni, sizeof(double)); o->wd2[d] = (double *)calloc(o->ni, sizeof(double)); }
+    if (o->mixp) { o->pd1[d] = (double *)calloc(o->ni, sizeof(double)); o->pd2[d] = (double *)calloc(o->ni, sizeof(double)); }
+  }
   size_t fsz = ((size_t)F_NSCAL + (size_t)F_NSPEC * o->ns) * o->ny * o->nz;
   o->fxl = (double *)calloc(fsz, sizeof(double));
   o->fxr = (double *)calloc(fsz, sizeof(double));
@@ -1602,6 +1689,7 @@ void orc_destroy(orc_t *o) {
                   o->wtmp, o->wmomix, o->difmix, o->itndex, o->st1, o->st2, o->st3, o->st4, o->st5,
                   o->st6, o->divm, o->ucor, o->vcor, o->wcor, o->rate, o->rrte, o->fxl, o->fxr};
   for (size_t q = 0; q < sizeof(ptrs) / sizeof(ptrs[0]); q++) free(ptrs[q]);
+  for (int d = 0; d < 3; d++) { free(o->wd1[d]); free(o->wd2[d]); free(o->pd1[d]); free(o->pd2[d]); }
   free(o);
 }
 

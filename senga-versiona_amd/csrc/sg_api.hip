@@ -142,6 +142,7 @@ static void build_tables(sg_config &c, Tables &t) {
   }
   for (int s = 0; s < SG_MAX_SPEC; s++) {
     t.ovwf[s] = (float)c.ovwmol[s];
+    t.wmol[s] = c.wmolar[s];
     t.olewf[s] = (float)c.olewis[s];
     for (int m = 0; m < SG_NTRC; m++) {
       t.condf[s][m] = (float)c.condco[s][m];
@@ -288,6 +289,9 @@ static int setup_params(sg_handle *h) {
   P.drin = c.prin / rtin;
   for (int q = 0; q < c.nspec; q++) P.yrin[q] = c.yrin[q];
   P.mixavg = c.mixavg;
+  // the molecular-transport options act with mixture-averaged transport only (src/diffin.f:384-386)
+  P.mixw = c.mixavg && (c.physics & SG_PHYS_MIXW) ? 1 : 0;
+  P.mixp = c.mixavg && (c.physics & SG_PHYS_MIXP) ? 1 : 0;
   P.precision = c.precision ? 1 : 0;   // 2: the mixed kernels with renormalisation
   P.renorm = c.precision == 2 ? 1 : 0;
   P.pofs = c.precision ? c.prin : 0.0;
@@ -353,6 +357,7 @@ static int init_launch(sg_handle *h, const int nw[3]) {
   L.ns = h->P.ns;
   L.precision = h->P.precision;
   L.bndx = h->P.bnd[0];
+  L.ext = h->P.mixw || h->P.mixp;
   L.inflow = h->P.inflow_xl;
   L.mechfix = h->mechfix;
   return 0;
@@ -471,6 +476,7 @@ static int batch_prepare(sg_batch *b) {
     b->L.ns = h0->P.ns;
     b->L.precision = h0->P.precision;
     b->L.bndx = h0->P.bnd[0];
+    b->L.ext = h0->P.mixw || h0->P.mixp;
     b->L.mechfix = h0->mechfix;   // members share one mechanism
     b->L.inflow = 0;
     for (int m = 0; m < nm; m++) b->L.inflow |= b->members[m]->P.inflow_xl;
@@ -545,7 +551,8 @@ static bool mechanism_in_range(const sg_config *c) {
 
 static int validate_config(const sg_config *cfg) {
   if (!species_supported(cfg->nspec)) return SG_EINVAL;   // compiled species counts
-  if (cfg->physics != 0) return SG_EINVAL;                // optional physics not computed here
+  // optional physics not computed here: Soret, Dufour, radiation
+  if (cfg->physics & ~(SG_PHYS_MIXW | SG_PHYS_MIXP)) return SG_EINVAL;
   if (cfg->nstep < 0 || cfg->nstep > SG_MAX_STEP) return SG_EINVAL;
   if (!mechanism_in_range(cfg)) return SG_EINVAL;
   if (cfg->nx < 11 || cfg->ny < 11 || cfg->nz < 11) return SG_EINVAL;
@@ -1084,8 +1091,10 @@ int sg_batch_add(sg_batch *b, sg_handle *h) {
   if (b->members.size() >= 65535) return set_err(h, SG_EINVAL, "batch is full");
   if (!b->members.empty()) {
     const sg_handle *h0 = b->members[0];
-    if (h->P.ns != h0->P.ns || h->P.precision != h0->P.precision || h->P.bnd[0] != h0->P.bnd[0])
-      return set_err(h, SG_EINVAL, "batch members share species count, precision and x boundary kind");
+    if (h->P.ns != h0->P.ns || h->P.precision != h0->P.precision || h->P.bnd[0] != h0->P.bnd[0] ||
+        (h->P.mixw || h->P.mixp) != (h0->P.mixw || h0->P.mixp))
+      return set_err(h, SG_EINVAL,
+                     "batch members share species count, precision, x boundary kind and the transport terms");
     if (!same_mechanism(h, h0)) return set_err(h, SG_EINVAL, "batch members share one mechanism");
   }
   // the batch stream runs after the handle's queued work (upload, TEMPIN)

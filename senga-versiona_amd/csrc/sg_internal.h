@@ -33,6 +33,7 @@ struct Params {
   int renorm;                  // mixed only: FP64 sum(rho Y_k) = rho renormalisation at FINCOM
   double pofs;                 // stored pressure is p - pofs (mixed: the default pressure, else 0)
   int mixavg;
+  int mixw, mixp;              // molar-mass / pressure diffusion (mixture-averaged transport only)
   double alamda, rlamda, prantl, tdifgb, pdifgb;
 };
 
@@ -57,6 +58,7 @@ struct Tables {
   // dense per-step rate coefficients: nuw[st][k] = diffmw of species k in
   // step st's list (0 if absent); valid when no species repeats in a list
   double nuw[SG_MAX_STEP][SG_MAX_SPEC];
+  double wmol[SG_MAX_SPEC];                      // molar masses (WMOLAR), pressure diffusion
   int diffsym;                                   // binary diffusion fits symmetric in (i, j)
   int ratedense;                                 // nuw is usable
   // FP32 copies of the tables the mixed-precision kernels read, so that
@@ -87,7 +89,9 @@ struct Fields {
   unsigned long long *emax;   // global error-norm maximum (bits of a non-negative double)
 };
 
-enum PrimSlot { P_U = 0, P_V, P_W, P_E, P_T, P_P, P_LAM, P_MU, P_HMIX, P_NSCAL };
+// P_LNW, P_LNP, P_OVW: ln(mixture molar mass), ln(pressure), 1/(mixture molar
+// mass), written when molar-mass or pressure diffusion is on
+enum PrimSlot { P_U = 0, P_V, P_W, P_E, P_T, P_P, P_LAM, P_MU, P_HMIX, P_LNW, P_LNP, P_OVW, P_NSCAL };
 // face slots: scalars then per-species
 enum FaceSlot { F_CP = 0, F_RR, F_DPDX, F_DPDY, F_DPDZ, F_DRDX, F_NSCAL };
 // per species: F_DYDX at F_NSCAL + s; then the NSCBC corrections of the
@@ -126,6 +130,7 @@ struct Launch {
   const sg_config *c;
   const Tables *tab;
   int ns, precision, bndx, inflow;
+  int ext;          // molar-mass or pressure diffusion on: the passes with those terms
   int pass_nw[3];   // pass tile per direction: 0 wide (rows of 32), 1 narrow (rows of 16)
   int mechfix;      // the chemistry is the compiled mechanism (MechH2): unrolled step loop
 };

@@ -780,6 +780,13 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
     wmomix = d * c3;
   }
   fld(pr + P_LAM * fs, bo) = lam;
+  if (P.mixw || P.mixp) {
+    // ln(mixture molar mass), ln(pressure) and the reciprocal molar mass for
+    // the molar-mass and pressure diffusion terms (src/rhscal.f:951-985, 2374)
+    fld(pr + P_LNW * fs, bo) = log(wmomix);
+    fld(pr + P_LNP * fs, bo) = log(pres);
+    fld(pr + P_OVW * fs, bo) = 1.0 / wmomix;
+  }
   SG_PHASE();
   // --- chemistry (src/chrate.f:146-760) on rho Y ---
   {
@@ -1172,8 +1179,6 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
     float c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      const float *cc = nullptr;
-      (void)cc;
       float f = tb->condf[s][SG_NTRC - 1];
 #pragma unroll
       for (int m = SG_NTRC - 2; m >= 0; m--) f = fma_u(f, transp, tb->condf[s][m]);
@@ -1190,6 +1195,18 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
     wmomix = df * c3;
   }
   fld(pr + P_LAM * fs, b4) = lam;
+  if (P.mixw || P.mixp) {
+    // ln(mixture molar mass), ln(pressure) and the reciprocal molar mass for
+    // the molar-mass and pressure diffusion terms (src/rhscal.f:951-985, 2374),
+    // formed in FP64
+    double sw = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) sw = sw + ry[s] * c->ovwmol[s];
+    const double wm = d * (1.0 / sw);
+    fld(pr + P_LNW * fs, b4) = (float)log(wm);
+    fld(pr + P_LNP * fs, b4) = (float)log(pres);
+    fld(pr + P_OVW * fs, b4) = (float)(1.0 / wm);
+  }
   SG_PHASE();
   // --- chemistry (src/chrate.f:146-760) on rho Y ---
   {
@@ -1405,10 +1422,11 @@ struct PassTile {
 // to 80 registers for 6 waves per SIMD (3 blocks per CU), which the
 // latency-bound staging pipeline turns into 7-10% shorter y and z passes and
 // 2% shorter interior x passes; the bounded-row edge tiles, which would spill
-// heavily at 80, keep 4.
-template <int DIR, bool BND, int NS, class R, int NW, int XP = 0>
+// heavily at 80, keep 4, and so do the passes with the molar-mass and
+// pressure diffusion terms (EXT), which need more.
+template <int DIR, bool BND, int NS, class R, int NW, int XP = 0, bool EXT = false>
 __global__ void __launch_bounds__((PassTile<DIR, NW, XP>::NT))
-__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 ? 6 : 4))) k_pass(const Member *__restrict__ mem,
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) k_pass(const Member *__restrict__ mem,
                                                                                                        const uint32_t *__restrict__ map) {
   static_assert(XP == 0 || (DIR == 0 && (XP == 1) != BND), "interior x tiles take the periodic form, edge tiles the bounded one");
   using T_ = PassTile<DIR, NW, XP>;
@@ -1556,6 +1574,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 ? 6 : 4))) k_pass(c
   // formed inside the stencils from two staged tiles instead of being staged
   // from memory, which is the same multiplication of the same two values.
   __shared__ R tmd[NIT * NTH];
+  // EXT: 1/(mixture molar mass), resident for the species' pressure diffusion
+  __shared__ R tow[EXT ? NIT * NTH : 1];
   auto stage_md = [&]() {
 #pragma unroll
     for (int it = 0; it < NIT; it++) tmd[t + it * NTH] = pf[0][it];
@@ -1579,6 +1599,26 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 ? 6 : 4))) k_pass(c
       const int l = lbase + q * LSTR;
       return tile[buf][f][l] * tmd[l];
     };
+    SG_PHASE();
+    R r;
+    if (BND) { int pq[5]; pkc(pq); r = d1_rows<R>(P, pq, DIR, wq, fqp); }
+    else r = d1<false, R>(P, S, DIR, atp);
+    pin(r);
+    SG_PHASE();
+    return r;
+  };
+  // EXT: rho D_k Y_k (pw: times 1 - W_k / W) differentiated along this
+  // direction, from the staged Y_k (slot 0) and rho D_k (slot 1) tiles and
+  // the resident 1/W tile, formed at every stencil point as the reference
+  // forms STORE7 (src/rhscal.f:1977, 2374)
+  auto D1PY = [&](bool pw, R wk) {
+    auto val = [&](int l) {
+      R v = tile[buf][1][l] * tile[buf][0][l];
+      if (pw) v = v * (R(1) - wk * tow[l]);
+      return v;
+    };
+    auto atp = [&](int kk) { return val(lbase + (kk - 8) * LSTR); };
+    auto fqp = [&](int q) { return val(lbase + q * LSTR); };
     SG_PHASE();
     R r;
     if (BND) { int pq[5]; pkc(pq); r = d1_rows<R>(P, pq, DIR, wq, fqp); }
@@ -1655,11 +1695,35 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 ? 6 : 4))) k_pass(c
   next();
   // ---------------- p - p_ref and mu under one barrier
   stage_n(2);
-  prefetch_species(0);
+  const R *lw = pr + P_LNW * fs, *lp = pr + P_LNP * fs, *ow = pr + P_OVW * fs;
+  if constexpr (EXT) {
+    prefetch_to(0, [&](long g) { return lw[g]; });
+    prefetch_to(1, [&](long g) { return lp[g]; });
+    prefetch_to(2, [&](long g) { return ow[g]; });
+  } else {
+    prefetch_species(0);
+  }
   dp = D1f(0); p0 = ctrf(0);
   dmu = D1f(1); mu0 = ctrf(1);
   next();
   pin(dmu); pin(mu0);
+  // ---------------- EXT: ln W and ln p under one barrier, 1/W into its
+  // resident tile (src/rhscal.f:945-992)
+  R dlw = R(0), d2lw = R(0), dlp = R(0), d2lp = R(0);
+  if constexpr (EXT) {
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      tile[buf][0][t + it * NTH] = pf[0][it];
+      tile[buf][1][t + it * NTH] = pf[1][it];
+      tow[t + it * NTH] = pf[2][it];
+    }
+    __syncthreads();
+    prefetch_species(0);
+    dlw = D1f(0); d2lw = D2f(0);
+    dlp = D1f(1); d2lp = D2f(1);
+    pin(dlw); pin(d2lw); pin(dlp); pin(d2lp);
+    next();
+  }
   // ---------------- momentum convection (src/rhsvel.f:348-729, 1312-2110)
   R aM[3];
   {
@@ -1720,6 +1784,15 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 ? 6 : 4))) k_pass(c
     SG_PHASE();
     dh = D1f(2);
     hc = ctrf(2);
+    R ds7w = R(0), ds7p = R(0), fw = R(0);
+    if constexpr (EXT) {
+      if (P.mixw) ds7w = D1PY(false, R(0));
+      if (P.mixp) {
+        const R wk = (R)F.tab->wmol[sp];
+        fw = R(1) - wk * tow[lbase];
+        ds7p = D1PY(true, wk);
+      }
+    }
     next();
     R ys = -R(0.5) * yc * dm - R(0.5) * dYm - R(0.5) * md0 * dY;
     const R g1 = (zdif ? R(0) : drD) * dY;
@@ -1730,6 +1803,29 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 ? 6 : 4))) k_pass(c
     aE = aE + ((zdifadb ? R(0) : dh) * dY) * rdc;
     aE = aE + (rdc * (zdifadb ? R(0) : d2Y)) * hc;
     ucor = ucor + rdc * dY;
+    if constexpr (EXT) {
+      // molar-mass (src/rhscal.f:1970-2364) and pressure (2365-2759)
+      // diffusion terms of this direction.  The heat-flux term with the
+      // species enthalpy gradient takes rho D_k in its x component, where the
+      // reference's STORE4 holds it at that point instead of dh/dx.
+      const R s7w = rdc * yc;
+      const R dhx = DIR == 0 ? rdc : dh;
+      auto term = [&](R s7, R ds7, R d1l, R d2l) {
+        ucor = ucor + s7 * d1l;
+        const R f1 = (zdif ? R(0) : ds7) * d1l;
+        ys = ys + f1;
+        vt = vt + f1;
+        aE = aE + ((zdifadb ? R(0) : ds7) * d1l) * hc;
+        aE = aE + ((zdifadb ? R(0) : dhx) * d1l) * s7;
+        const R f3 = (zdif ? R(0) : d2l) * s7;
+        ys = ys + f3;
+        vt = vt + f3;
+        aE = aE + ((zdifadb ? R(0) : d2l) * s7) * hc;
+      };
+      if (P.mixw) term(s7w, ds7w, dlw, d2lw);
+      if (P.mixp) term(s7w * fw, ds7p, dlp, d2lp);
+      pin(ys); pin(vt); pin(ucor); pin(aE);
+    }
     aY[sp] = ys;
     dYz[sp] = zdif ? R(0) : dY;
     yv[sp] = yc;
@@ -2253,31 +2349,38 @@ void launch_prim_t(const Launch &L, hipStream_t st) {
   if (L.precision) hipLaunchKernelGGL((k_prim_mx<NS, false>), g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
   else hipLaunchKernelGGL((k_prim<NS, false>), g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
 }
-template <int NS, class R, int NW>
+template <int NS, class R, int NW, bool EXT>
 static void launch_pass_w(int dir, const Launch &L, hipStream_t st) {
   const int k = L_PASSX + dir;
   if (dir == 0) {
     const dim3 b(PassTile<0, NW>::NT);
     if (L.bndx) {
       // interior tiles of the bounded rows, then their edge tiles
-      if (L.nblk[k] > 0) hipLaunchKernelGGL((k_pass<0, false, NS, R, NW, 1>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
-      hipLaunchKernelGGL((k_pass<0, true, NS, R, 0, 2>), dim3(L.nblk[L_PASSXE]), dim3(PassTile<0, 0, 2>::NT), 0, st,
-                         L.mem, L.map[L_PASSXE]);
+      if (L.nblk[k] > 0)
+        hipLaunchKernelGGL((k_pass<0, false, NS, R, NW, 1, EXT>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
+      hipLaunchKernelGGL((k_pass<0, true, NS, R, 0, 2, EXT>), dim3(L.nblk[L_PASSXE]), dim3(PassTile<0, 0, 2>::NT), 0,
+                         st, L.mem, L.map[L_PASSXE]);
     } else {
-      hipLaunchKernelGGL((k_pass<0, false, NS, R, NW>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
+      hipLaunchKernelGGL((k_pass<0, false, NS, R, NW, 0, EXT>), dim3(L.nblk[k]), b, 0, st, L.mem, L.map[k]);
     }
   } else if (dir == 1) {
-    hipLaunchKernelGGL((k_pass<1, false, NS, R, NW>), dim3(L.nblk[k]), dim3(PassTile<1, NW>::NT), 0, st, L.mem,
-                       L.map[k]);
+    hipLaunchKernelGGL((k_pass<1, false, NS, R, NW, 0, EXT>), dim3(L.nblk[k]), dim3(PassTile<1, NW>::NT), 0, st,
+                       L.mem, L.map[k]);
   } else {
-    hipLaunchKernelGGL((k_pass<2, false, NS, R, NW>), dim3(L.nblk[k]), dim3(PassTile<2, NW>::NT), 0, st, L.mem,
-                       L.map[k]);
+    hipLaunchKernelGGL((k_pass<2, false, NS, R, NW, 0, EXT>), dim3(L.nblk[k]), dim3(PassTile<2, NW>::NT), 0, st,
+                       L.mem, L.map[k]);
   }
 }
+// EXT: the passes with the molar-mass and pressure diffusion terms
 template <int NS, class R>
 static void launch_pass_r(int dir, const Launch &L, hipStream_t st) {
-  if (L.pass_nw[dir]) launch_pass_w<NS, R, 1>(dir, L, st);
-  else launch_pass_w<NS, R, 0>(dir, L, st);
+  if (L.ext) {
+    if (L.pass_nw[dir]) launch_pass_w<NS, R, 1, true>(dir, L, st);
+    else launch_pass_w<NS, R, 0, true>(dir, L, st);
+  } else {
+    if (L.pass_nw[dir]) launch_pass_w<NS, R, 1, false>(dir, L, st);
+    else launch_pass_w<NS, R, 0, false>(dir, L, st);
+  }
 }
 template <int NS>
 void launch_pass_t(int dir, const Launch &L, hipStream_t st) {

@@ -27,6 +27,15 @@ def build(name):
         # the reference's 4-step methane mechanism: reactant coefficient lists
         # with non-integer and negative orders (src/chrate.f:475-516)
         return C.ch4_flame(n=(16, 12, 11))
+    if name == "h2_nscbc_mixwp_12x11x11":
+        # molar-mass and pressure diffusion (src/rhscal.f:945-992,1970-2759)
+        c = C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=True)
+        c.control.nfmixw = c.control.nfmixp = 1
+        return c
+    if name == "h2_per_mixwp_11x12x11":
+        c = C.h2_nscbc(n=(11, 12, 11), length=(1.4e-3, 1.5e-3, 1.4e-3), mixavg=True, periodic_x=True)
+        c.control.nfmixw = c.control.nfmixp = 1
+        return c
     if name == ADAPT:
         c = C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=True)
         c.control.nstpsw = 1
@@ -38,7 +47,7 @@ ADAPT = "h2_nscbc_mix_adapt_12x11x11"
 
 
 NAMES = ["periodic_1step_16x12x11", "h2_nscbc_mix_12x11x11", "h2_nscbc_cle_12x11x11", "h2_per_mix_11x12x11",
-         "h2_falloff_nscbc_mix_12x11x11", "ch4_4step_16x12x11"]
+         "h2_falloff_nscbc_mix_12x11x11", "ch4_4step_16x12x11", "h2_nscbc_mixwp_12x11x11", "h2_per_mixwp_11x12x11"]
 OPS = {"ops_periodic_16x12x11": "periodic_1step_16x12x11", "ops_boundedx_12x11x11": "h2_nscbc_cle_12x11x11"}
 
 

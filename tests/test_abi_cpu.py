@@ -59,14 +59,13 @@ def test_rejects_unsupported_configs(lib):
     assert lib.sg_create(None, C.byref(h)) == 1
 
 
-@pytest.mark.parametrize("switch,bit", [("nfmixw", abi.PHYS_MIXW), ("nfmixp", abi.PHYS_MIXP),
-                                        ("nfmsor", abi.PHYS_SORET), ("nfmduf", abi.PHYS_DUFOUR),
+@pytest.mark.parametrize("switch,bit", [("nfmsor", abi.PHYS_SORET), ("nfmduf", abi.PHYS_DUFOUR),
                                         ("nfradn", abi.PHYS_RADN)])
 def test_rejects_each_optional_physics_switch(lib, switch, bit, tmp_path):
-    """Soret, Dufour, the molar-mass and pressure diffusion terms and radiation
-    (cont.dat switches, src/contin.f:194-209) are not computed here: a case
-    that switches one on is refused by sg_create and sg_footprint instead of
-    run without it.  The switch reaches sg_config through both readers."""
+    """Soret, Dufour and radiation (cont.dat switches, src/contin.f:194-209)
+    are not computed here: a case that switches one on is refused by
+    sg_create and sg_footprint instead of run without it.  The switch reaches
+    sg_config through both readers."""
     c = CS.h2_nscbc(n=(12, 11, 11), length=(1e-3, 1e-3, 1e-3), mixavg=True)
     setattr(c.control, switch, 1)
     p = tmp_path / "cont.dat"
@@ -218,3 +217,20 @@ def test_serving_summary():
     assert abs(s["p50_s"] - np.percentile(lat, 50)) < 1e-12
     assert abs(s["p99_s"] - np.percentile(lat, 99)) < 1e-12
     assert abs(s["mpoint_substages_per_s"] - 10 * 1000 * 50 / 6.4 / 1e6) < 1e-9
+
+
+@pytest.mark.parametrize("switch,bit", [("nfmixw", abi.PHYS_MIXW), ("nfmixp", abi.PHYS_MIXP)])
+def test_accepts_molar_mass_and_pressure_diffusion(lib, switch, bit, tmp_path):
+    """The molar-mass and pressure diffusion terms are computed: the switch
+    reaches sg_config through the reader, and sg_footprint accepts it."""
+    c = CS.h2_nscbc(n=(12, 11, 11), length=(1e-3, 1e-3, 1e-3), mixavg=True)
+    setattr(c.control, switch, 1)
+    p = tmp_path / "cont.dat"
+    c.control.write(str(p))
+    assert getattr(cont.read_cont(str(p)), switch) == 1
+    cfg = abi.config_from_case(c)
+    assert cfg.physics == bit
+    out = C.c_size_t(0)
+    assert lib.sg_footprint(C.byref(cfg), C.byref(out)) == 0
+    cfg.physics = bit | abi.PHYS_SORET
+    assert lib.sg_footprint(C.byref(cfg), C.byref(out)) == 1

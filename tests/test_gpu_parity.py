@@ -132,6 +132,25 @@ def _gpu_after(c, steps, precision):
     return run, T
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_transport_terms_against_oracle(periodic):
+    """Molar-mass and pressure diffusion (cont.dat NFMIXW, NFMIXP): the passes
+    with those terms against the oracle (itself bit-exact against the
+    reference's fixtures) on larger grids than the fixtures, after 6 steps:
+    FP64 within the bar, the mixed form within 1e-5.  A duct with NSCBC x
+    faces and a triply periodic box."""
+    from sengapy import case as C
+    n = (40, 24, 24)
+    c = C.h2_nscbc(n=n, length=(4e-3, 2.4e-3, 2.4e-3), mixavg=True, periodic_x=periodic, nsteps=6)
+    c.control.nfmixw = c.control.nfmixp = 1
+    print("case", c.name, "molar-mass and pressure diffusion, 6 steps", flush=True)
+    run_o, T_o = _oracle_after(c, 6)
+    run, T = _gpu_after(c, 6, 0)
+    assert_state_close("fp64 vs oracle", run, T, run_o, T_o, TOL, 1e-12, tol_pw=PW_TOL64)
+    run, T = _gpu_after(c, 6, 1)
+    assert_state_close("mixed vs oracle", run, T, run_o, T_o, 1e-5, 1e-5)
+
+
 def test_h2_workload_full_size_one_step():
     """The measured workload itself -- 128^3 H2-air, NSCBC inflow/outflow,
     mixture-averaged transport -- one RK step against the oracle: FP64 at

@@ -58,3 +58,18 @@ def test_adaptive_step_bitexact(oracle_built):
     assert tstep != float(fx["tstep"])
     assert np.array_equal(run, fx["run2"])
     assert np.array_equal(T2, fx["T2"])
+
+
+@pytest.mark.parametrize("flag", ["nfmixw", "nfmixp"])
+def test_transport_fixture_exercises_each_term(flag, oracle_built):
+    """The molar-mass and pressure diffusion fixture pins both terms: with
+    either switch off the oracle no longer reproduces the reference's
+    right-hand side (the density and momentum are untouched by them)."""
+    c, fx = golden_cases.load("h2_nscbc_mixwp_12x11x11")
+    setattr(c.control, flag, 0)
+    o = Oracle(c)
+    o.set_timestep(c.dump.tstep)
+    o.set_state(c.dump.state, 0.0)
+    rhs = o.rhs(1)
+    assert np.array_equal(rhs[:4], fx["rhs"][:4])
+    assert not np.array_equal(rhs[4:], fx["rhs"][4:])
