@@ -1807,7 +1807,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
       // molar-mass (src/rhscal.f:1970-2364) and pressure (2365-2759)
       // diffusion terms of this direction.  The heat-flux term with the
       // species enthalpy gradient takes rho D_k in its x component, where the
-      // reference's STORE4 holds it at that point instead of dh/dx.
+      // reference's STORE4 holds it at that point instead of dh/dx
+      // (src/rhscal.f:1887, 2071-2103, 2467-2499).
       const R s7w = rdc * yc;
       const R dhx = DIR == 0 ? rdc : dh;
       auto term = [&](R s7, R ds7, R d1l, R d2l) {
