@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define SG_ABI_VERSION 2
+#define SG_ABI_VERSION 3
 
 #define SG_MAX_SPEC 18   /* the reference's largest shipped mechanism (casefiles/68step: NSPCMX=18) */
 #define SG_MAX_STEP 128
@@ -64,11 +64,11 @@ extern "C" {
 /* sg_config.physics bits: the reference's optional physics switches from
  * cont.dat (src/contin.f:194-209), set when the switch is on -- the four
  * molecular-transport options only together with mixture-averaged transport,
- * as src/diffin.f:66,384-407 applies them.  The molar-mass and pressure
- * diffusion terms are computed (with mixture-averaged transport; without it
- * they are inactive, as in the reference); Soret, Dufour and radiation are
- * not: sg_create and sg_footprint return SG_EINVAL when one of those bits is
- * set, so that such a case is refused instead of run without that physics. */
+ * as src/diffin.f:66,384-407 applies them.  The four molecular-transport
+ * terms are computed (with mixture-averaged transport; without it they are
+ * inactive, as in the reference); radiation is not: sg_create and
+ * sg_footprint return SG_EINVAL when its bit is set, so that such a case is
+ * refused instead of run without it. */
 #define SG_PHYS_MIXW 1    /* NFMIXW: molar-mass diffusion term (src/rhscal.f:951,1970) */
 #define SG_PHYS_MIXP 2    /* NFMIXP: pressure diffusion term (src/rhscal.f:973,2365) */
 #define SG_PHYS_SORET 4   /* NFMSOR: thermal diffusion of light species (src/rhscal.f:2760) */
@@ -91,7 +91,7 @@ typedef struct sg_config {
                                  has no such renormalisation (src/lincom.f:294-
                                  333 is commented out) and the FP64 form never
                                  applies it. */
-  int32_t physics;            /* SG_PHYS_* bits of switches that are on (MIXW, MIXP computed; others refused) */
+  int32_t physics;            /* SG_PHYS_* bits of switches that are on (RADN refused) */
   double xgdlen, ygdlen, zgdlen;  /* domain lengths; spacing L/(N-1) (src/dfinit.f:42-50) */
 
   /* boundary conditions, faces xl, xr, yl, yr, zl, zr (cont.dat lines 51-56) */
@@ -156,6 +156,15 @@ typedef struct sg_config {
   double diffco[SG_MAX_SPEC][SG_MAX_SPEC][SG_NTRC];  /* [i][j][coef] = DIFFCO(coef,j,i) */
   double wilko1[SG_MAX_SPEC][SG_MAX_SPEC];             /* [i][j] = WILKO1(j,i) */
   double wilko2[SG_MAX_SPEC][SG_MAX_SPEC];
+
+  /* thermal diffusion ratios for the Soret and Dufour terms (diff.dat,
+   * src/diffin.f:180-195), with the reference's antisymmetric fill
+   * (src/diffin.f:345-368): [i][j][coef] = TDRCCO(coef,j,i), a polynomial in
+   * T/TDIFGB.  They act for the light species, molar mass <= wmltdr + 1e-3
+   * (cont.dat WMLTDR, src/diffin.f:388-407). */
+  int32_t ncotdr, reserved2;
+  double wmltdr;
+  double tdrcco[SG_MAX_SPEC][SG_MAX_SPEC][SG_NTRC];
 } sg_config;
 
 typedef struct sg_handle sg_handle;
@@ -170,8 +179,8 @@ size_t sg_config_size(void);
  * part of INDATA/DFINIT/BCINIT.  Returns SG_EINVAL for a species count that is
  * not compiled in (2..SG_MAX_SPEC are), nx/ny/nz < 11, more than 2^29 points
  * per field, cp fits of more than 5 coefficients (src/temper.f NCTMAX),
- * transport fits of more than SG_NTRC coefficients, SG_PHYS_SORET, _DUFOUR or
- * _RADN, or a
+ * transport fits or thermal diffusion ratios of more than SG_NTRC
+ * coefficients, SG_PHYS_RADN, or a
  * step list, species index, third-body or falloff index out of range. */
 int sg_create(const sg_config *cfg, sg_handle **out);
 int sg_destroy(sg_handle *h);

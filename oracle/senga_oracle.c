@@ -78,6 +78,12 @@ struct orc {
      (WD1X..WD2Z, PD1X..PD2Z), allocated when NFMIXW / NFMIXP are on */
   double *wd1[3], *wd2[3], *pd1[3], *pd2[3];
   int mixw, mixp;
+  /* Soret and Dufour: first and second derivatives of ln(T/TDIFGB) (TD1X..
+     TD2Z), the thermal diffusion ratio of the current species (TDRMIX, big)
+     and the light species they act for (FLMTDR, bit per species) */
+  double *td1[3], *td2[3], *tdrmix;
+  int soret, dufour;
+  unsigned ltmask;
   double *rate, *rrte;
   /* face arrays (ny*nz, and *ns for species) */
   double *fxl, *fxr;
@@ -933,6 +939,11 @@ static int rhscal(orc_t *o) {
     for (int d = 0; d < 3; d++) dfbyd(o, d, s7, d1[d]);
     for (int d = 0; d < 3; d++) d2fd2(o, d, s7, d2[d]);
   }
+  /* TRANSP holds ln(T/TDIFGB), rhscal.f:994-1004 (FLMIXT) */
+  if (o->ltmask) {
+    for (int d = 0; d < 3; d++) dfbyd(o, d, o->transp, o->td1[d]);
+    for (int d = 0; d < 3; d++) d2fd2(o, d, o->transp, o->td2[d]);
+  }
   /* Y: convert, rhscal.f:1013-1028 */
   for (int s = 0; s < ns; s++) {
     double *Y = R(o, 5 + s);
@@ -998,6 +1009,26 @@ static int rhscal(orc_t *o) {
         s7[b] = o->difmix[b];
       }
     }
+    /* thermal diffusion ratio of species s relative to the other light
+       species, rhscal.f:1353-1410 */
+    const int flmtdr = (o->ltmask >> s) & 1u;
+    const int flmsor = flmtdr && o->soret, flmduf = flmtdr && o->dufour;
+    if (flmtdr) {
+      FOR_B(o) {
+        size_t b = BI(o, i, j, k);
+        double tdr = ZERO;
+        for (int js = 0; js < ns; js++) {
+          if (!((o->ltmask >> js) & 1u) || js == s) continue;
+          double combo2 = o->trun[b] / c->tdifgb;
+          double fornow = c->tdrcco[s][js][c->ncotdr - 1];
+          for (int q = c->ncotdr - 2; q >= 0; q--) fornow = fornow * combo2 + c->tdrcco[s][js][q];
+          double ctrans = fornow;
+          fornow = R(o, 5 + js)[b] * c->ovwmol[js];
+          tdr = tdr + fornow * o->wmomix[b] * ctrans;
+        }
+        o->tdrmix[b] = tdr * Y[b] * o->wmomix[b] * c->ovwmol[s];
+      }
+    }
     FOR_I(o) {
       size_t p = II(o, i, j, k), b = BI(o, i, j, k);
       o->ucor[p] = o->ucor[p] + s7[b] * s1[p];
@@ -1016,6 +1047,12 @@ static int rhscal(orc_t *o) {
     }
     for (int f = 0; f < 2; f++)
       if (o->fcnv[f]) FOR_FACE(o) FSP(o, face[f], FS_STRH, s)[(size_t)k * o->ny + j] = o->utmp[BI(o, xf[f], j, k)];
+    /* Dufour: the species enthalpy carries R_k T TDRMIX through the heat-flux
+       terms below, rhscal.f:1549-1566 */
+    if (flmduf) FOR_B(o) {
+        size_t b = BI(o, i, j, k);
+        o->utmp[b] = o->utmp[b] + c->rgspec[s] * o->trun[b] * o->tdrmix[b];
+      }
     /* diffusivity gradient terms, rhscal.f:1575-1632 */
     dfbyd(o, 0, s7, s4);
     dfbyd(o, 1, s7, s5);
@@ -1118,6 +1155,64 @@ static int rhscal(orc_t *o) {
       FOR_I(o) {
         size_t p = II(o, i, j, k), b = BI(o, i, j, k);
         double fornow = (d2[0][p] + d2[1][p] + d2[2][p]) * s7[b];
+        E[b] = E[b] + fornow * o->utmp[b];
+      }
+    }
+    /* Soret (thermal diffusion) terms, rhscal.f:2757-3758; the Dufour
+       enthalpy is taken out again before their heat-flux terms (2832-2845).
+       The wall terms (FXLDFW, FXLADW ...) need wall boundaries. */
+    if (flmsor) {
+      FOR_B(o) {
+        size_t b = BI(o, i, j, k);
+        s7[b] = o->difmix[b] * Y[b] * o->tdrmix[b];
+      }
+      FOR_I(o) {
+        size_t p = II(o, i, j, k), b = BI(o, i, j, k);
+        o->ucor[p] = o->ucor[p] + s7[b] * o->td1[0][p];
+        o->vcor[p] = o->vcor[p] + s7[b] * o->td1[1][p];
+        o->wcor[p] = o->wcor[p] + s7[b] * o->td1[2][p];
+      }
+      dfbyd(o, 0, s7, s1);
+      dfbyd(o, 1, s7, s2);
+      dfbyd(o, 2, s7, s3);
+      zero_flags(o, o->fdif, s1, s2, s3);
+      FOR_I(o) {
+        size_t p = II(o, i, j, k);
+        double fornow = s1[p] * o->td1[0][p] + s2[p] * o->td1[1][p] + s3[p] * o->td1[2][p];
+        rt[p] = rt[p] + fornow;
+        vt[BI(o, i, j, k)] = vt[BI(o, i, j, k)] + fornow;
+      }
+      if (flmduf) FOR_B(o) {
+          size_t b = BI(o, i, j, k);
+          o->utmp[b] = o->utmp[b] - c->rgspec[s] * o->trun[b] * o->tdrmix[b];
+        }
+      zero_flags(o, o->fadb, s1, s2, s3);
+      FOR_I(o) {
+        size_t p = II(o, i, j, k), b = BI(o, i, j, k);
+        double fornow = s1[p] * o->td1[0][p] + s2[p] * o->td1[1][p] + s3[p] * o->td1[2][p];
+        E[b] = E[b] + fornow * o->utmp[b];
+      }
+      dfbyd(o, 0, o->utmp, s4);
+      dfbyd(o, 1, o->utmp, s5);
+      dfbyd(o, 2, o->utmp, s6);
+      zero_flags(o, o->fdif, s4, s5, s6);
+      zero_flags(o, o->fadb, s4, s5, s6);
+      FOR_I(o) {
+        size_t p = II(o, i, j, k), b = BI(o, i, j, k);
+        double fornow = s4[p] * o->td1[0][p] + s5[p] * o->td1[1][p] + s6[p] * o->td1[2][p];
+        E[b] = E[b] + fornow * s7[b];
+      }
+      zero_flags(o, o->fdif, o->td2[0], o->td2[1], o->td2[2]);
+      FOR_I(o) {
+        size_t p = II(o, i, j, k);
+        double fornow = (o->td2[0][p] + o->td2[1][p] + o->td2[2][p]) * s7[BI(o, i, j, k)];
+        rt[p] = rt[p] + fornow;
+        vt[BI(o, i, j, k)] = vt[BI(o, i, j, k)] + fornow;
+      }
+      zero_flags(o, o->fadb, o->td2[0], o->td2[1], o->td2[2]);
+      FOR_I(o) {
+        size_t p = II(o, i, j, k), b = BI(o, i, j, k);
+        double fornow = (o->td2[0][p] + o->td2[1][p] + o->td2[2][p]) * s7[b];
         E[b] = E[b] + fornow * o->utmp[b];
       }
     }
@@ -1634,8 +1729,8 @@ orc_t *orc_create(const sg_config *cfg, int *err) {
     *err = SG_EINVAL;
     return NULL;
   }
-  /* Soret, Dufour and radiation are not restated */
-  if (cfg->physics & ~(SG_PHYS_MIXW | SG_PHYS_MIXP)) {
+  /* radiation is not restated */
+  if (cfg->physics & SG_PHYS_RADN) {
     *err = SG_EINVAL;
     return NULL;
   }
@@ -1670,10 +1765,18 @@ orc_t *orc_create(const sg_config *cfg, int *err) {
      only (src/diffin.f:384-386) */
   o->mixw = cfg->mixavg && (cfg->physics & SG_PHYS_MIXW);
   o->mixp = cfg->mixavg && (cfg->physics & SG_PHYS_MIXP);
+  o->soret = cfg->mixavg && (cfg->physics & SG_PHYS_SORET);
+  o->dufour = cfg->mixavg && (cfg->physics & SG_PHYS_DUFOUR);
+  o->ltmask = 0;
+  if (o->soret || o->dufour)   /* src/diffin.f:388-407, WMLTOL = 1.0D-3 */
+    for (int s = 0; s < o->ns; s++)
+      if (cfg->wmolar[s] <= cfg->wmltdr + 1.0e-3) o->ltmask |= 1u << s;
   for (int d = 0; d < 3; d++) {
     if (o->mixw) { o->wd1[d] = (double *)calloc(o->ni, sizeof(double)); o->wd2[d] = (double *)calloc(o->ni, sizeof(double)); }
     if (o->mixp) { o->pd1[d] = (double *)calloc(o->ni, sizeof(double)); o->pd2[d] = (double *)calloc(o->ni, sizeof(double)); }
+    if (o->ltmask) { o->td1[d] = (double *)calloc(o->ni, sizeof(double)); o->td2[d] = (double *)calloc(o->ni, sizeof(double)); }
   }
+  if (o->ltmask) o->tdrmix = (double *)calloc(o->nb, sizeof(double));
   size_t fsz = ((size_t)F_NSCAL + (size_t)F_NSPEC * o->ns) * o->ny * o->nz;
   o->fxl = (double *)calloc(fsz, sizeof(double));
   o->fxr = (double *)calloc(fsz, sizeof(double));
@@ -1689,7 +1792,10 @@ void orc_destroy(orc_t *o) {
                   o->wtmp, o->wmomix, o->difmix, o->itndex, o->st1, o->st2, o->st3, o->st4, o->st5,
                   o->st6, o->divm, o->ucor, o->vcor, o->wcor, o->rate, o->rrte, o->fxl, o->fxr};
   for (size_t q = 0; q < sizeof(ptrs) / sizeof(ptrs[0]); q++) free(ptrs[q]);
-  for (int d = 0; d < 3; d++) { free(o->wd1[d]); free(o->wd2[d]); free(o->pd1[d]); free(o->pd2[d]); }
+  for (int d = 0; d < 3; d++) {
+    free(o->wd1[d]); free(o->wd2[d]); free(o->pd1[d]); free(o->pd2[d]); free(o->td1[d]); free(o->td2[d]);
+  }
+  free(o->tdrmix);
   free(o);
 }
 

@@ -143,6 +143,7 @@ static void build_tables(sg_config &c, Tables &t) {
   for (int s = 0; s < SG_MAX_SPEC; s++) {
     t.ovwf[s] = (float)c.ovwmol[s];
     t.wmol[s] = c.wmolar[s];
+    t.rgs[s] = c.rgspec[s];
     t.olewf[s] = (float)c.olewis[s];
     for (int m = 0; m < SG_NTRC; m++) {
       t.condf[s][m] = (float)c.condco[s][m];
@@ -292,6 +293,13 @@ static int setup_params(sg_handle *h) {
   // the molecular-transport options act with mixture-averaged transport only (src/diffin.f:384-386)
   P.mixw = c.mixavg && (c.physics & SG_PHYS_MIXW) ? 1 : 0;
   P.mixp = c.mixavg && (c.physics & SG_PHYS_MIXP) ? 1 : 0;
+  P.soret = c.mixavg && (c.physics & SG_PHYS_SORET) ? 1 : 0;
+  P.dufour = c.mixavg && (c.physics & SG_PHYS_DUFOUR) ? 1 : 0;
+  // the thermal diffusion flags FLMTDR/FLMSOR/FLMDUF of src/diffin.f:388-407
+  P.ltmask = 0;
+  if (P.soret || P.dufour)
+    for (int s = 0; s < c.nspec; s++)
+      if (c.wmolar[s] <= c.wmltdr + 1.0e-3) P.ltmask |= 1u << s;
   P.precision = c.precision ? 1 : 0;   // 2: the mixed kernels with renormalisation
   P.renorm = c.precision == 2 ? 1 : 0;
   P.pofs = c.precision ? c.prin : 0.0;
@@ -357,7 +365,7 @@ static int init_launch(sg_handle *h, const int nw[3]) {
   L.ns = h->P.ns;
   L.precision = h->P.precision;
   L.bndx = h->P.bnd[0];
-  L.ext = h->P.mixw || h->P.mixp;
+  L.ext = h->P.mixw || h->P.mixp || h->P.soret || h->P.dufour;
   L.inflow = h->P.inflow_xl;
   L.mechfix = h->mechfix;
   return 0;
@@ -476,7 +484,7 @@ static int batch_prepare(sg_batch *b) {
     b->L.ns = h0->P.ns;
     b->L.precision = h0->P.precision;
     b->L.bndx = h0->P.bnd[0];
-    b->L.ext = h0->P.mixw || h0->P.mixp;
+    b->L.ext = h0->P.mixw || h0->P.mixp || h0->P.soret || h0->P.dufour;
     b->L.mechfix = h0->mechfix;   // members share one mechanism
     b->L.inflow = 0;
     for (int m = 0; m < nm; m++) b->L.inflow |= b->members[m]->P.inflow_xl;
@@ -551,8 +559,10 @@ static bool mechanism_in_range(const sg_config *c) {
 
 static int validate_config(const sg_config *cfg) {
   if (!species_supported(cfg->nspec)) return SG_EINVAL;   // compiled species counts
-  // optional physics not computed here: Soret, Dufour, radiation
-  if (cfg->physics & ~(SG_PHYS_MIXW | SG_PHYS_MIXP)) return SG_EINVAL;
+  // optional physics not computed here: radiation
+  if (cfg->physics & SG_PHYS_RADN) return SG_EINVAL;
+  if (cfg->mixavg && (cfg->physics & (SG_PHYS_SORET | SG_PHYS_DUFOUR)) && (cfg->ncotdr < 1 || cfg->ncotdr > SG_NTRC))
+    return SG_EINVAL;
   if (cfg->nstep < 0 || cfg->nstep > SG_MAX_STEP) return SG_EINVAL;
   if (!mechanism_in_range(cfg)) return SG_EINVAL;
   if (cfg->nx < 11 || cfg->ny < 11 || cfg->nz < 11) return SG_EINVAL;
@@ -580,7 +590,7 @@ static size_t case_bytes(const Params &P, size_t *main_block) {
   // FP64 registers; primitives, rates, accumulators and gradients in the
   // working precision (FP32 in the mixed mode)
   const size_t esz = P.precision ? sizeof(float) : sizeof(double);
-  const long nreg = (5 + ns) * 2, nwork = nprim(ns) + ns + (5 + ns) + 12;
+  const long nreg = (5 + ns) * 2, nwork = nprim(ns, P.soret || P.dufour) + ns + (5 + ns) + 12;
   size_t bytes = sizeof(double) * (size_t)fs * nreg + esz * (size_t)fs * nwork;
   bytes = ((bytes + 255) / 256) * 256;
   const size_t itn = ((sizeof(uint32_t) * (size_t)fs + 255) / 256) * 256;
@@ -640,7 +650,8 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   const size_t esz = P.precision ? sizeof(float) : sizeof(double);
   const size_t total = case_bytes(P, &h->bytes);
   h->lbytes = total - h->bytes;
-  size_t bytes = sizeof(double) * (size_t)fs * (5 + ns) * 2 + esz * (size_t)fs * (nprim(ns) + ns + (5 + ns) + 12);
+  size_t bytes = sizeof(double) * (size_t)fs * (5 + ns) * 2 +
+                 esz * (size_t)fs * (nprim(ns, P.soret || P.dufour) + ns + (5 + ns) + 12);
   bytes = ((bytes + 255) / 256) * 256;
   const size_t itn = ((sizeof(uint32_t) * (size_t)fs + 255) / 256) * 256;
   const size_t facesz = sizeof(double) * 2 * (size_t)nface(ns) * P.ny * P.nz;
@@ -655,7 +666,7 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   Fields &F = h->F;
   F.q = (double *)b; b += sizeof(double) * (5 + ns) * fs;
   F.run = (double *)b; b += sizeof(double) * (5 + ns) * fs;
-  F.prim = (double *)b; b += esz * nprim(ns) * fs;   // element type per precision
+  F.prim = (double *)b; b += esz * nprim(ns, P.soret || P.dufour) * fs;   // element type per precision
   F.rate = (double *)b; b += esz * ns * fs;
   F.acc = (double *)b; b += esz * (5 + ns) * fs;
   F.grad = (double *)b; b += esz * 12 * fs;
@@ -1092,7 +1103,8 @@ int sg_batch_add(sg_batch *b, sg_handle *h) {
   if (!b->members.empty()) {
     const sg_handle *h0 = b->members[0];
     if (h->P.ns != h0->P.ns || h->P.precision != h0->P.precision || h->P.bnd[0] != h0->P.bnd[0] ||
-        (h->P.mixw || h->P.mixp) != (h0->P.mixw || h0->P.mixp))
+        (h->P.mixw || h->P.mixp || h->P.soret || h->P.dufour) !=
+            (h0->P.mixw || h0->P.mixp || h0->P.soret || h0->P.dufour))
       return set_err(h, SG_EINVAL,
                      "batch members share species count, precision, x boundary kind and the transport terms");
     if (!same_mechanism(h, h0)) return set_err(h, SG_EINVAL, "batch members share one mechanism");

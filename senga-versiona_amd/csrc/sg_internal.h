@@ -34,6 +34,8 @@ struct Params {
   double pofs;                 // stored pressure is p - pofs (mixed: the default pressure, else 0)
   int mixavg;
   int mixw, mixp;              // molar-mass / pressure diffusion (mixture-averaged transport only)
+  int soret, dufour;           // thermal diffusion and its heat flux, for the species in ltmask
+  uint32_t ltmask;             // light species (molar mass <= WMLTDR + 1e-3) when soret or dufour
   double alamda, rlamda, prantl, tdifgb, pdifgb;
 };
 
@@ -59,6 +61,7 @@ struct Tables {
   // step st's list (0 if absent); valid when no species repeats in a list
   double nuw[SG_MAX_STEP][SG_MAX_SPEC];
   double wmol[SG_MAX_SPEC];                      // molar masses (WMOLAR), pressure diffusion
+  double rgs[SG_MAX_SPEC];                       // species gas constants (RGSPEC), Dufour
   int diffsym;                                   // binary diffusion fits symmetric in (i, j)
   int ratedense;                                 // nuw is usable
   // FP32 copies of the tables the mixed-precision kernels read, so that
@@ -90,8 +93,9 @@ struct Fields {
 };
 
 // P_LNW, P_LNP, P_OVW: ln(mixture molar mass), ln(pressure), 1/(mixture molar
-// mass), written when molar-mass or pressure diffusion is on
-enum PrimSlot { P_U = 0, P_V, P_W, P_E, P_T, P_P, P_LAM, P_MU, P_HMIX, P_LNW, P_LNP, P_OVW, P_NSCAL };
+// mass), written when molar-mass or pressure diffusion is on; P_LNT:
+// ln(T/TDIFGB), written when Soret or Dufour is on
+enum PrimSlot { P_U = 0, P_V, P_W, P_E, P_T, P_P, P_LAM, P_MU, P_HMIX, P_LNW, P_LNP, P_OVW, P_LNT, P_NSCAL };
 // face slots: scalars then per-species
 enum FaceSlot { F_CP = 0, F_RR, F_DPDX, F_DPDY, F_DPDZ, F_DRDX, F_NSCAL };
 // per species: F_DYDX at F_NSCAL + s; then the NSCBC corrections of the
@@ -100,7 +104,11 @@ enum FaceSlot { F_CP = 0, F_RR, F_DPDX, F_DPDY, F_DPDZ, F_DRDX, F_NSCAL };
 inline __host__ __device__ long prim_y(const Params &P, int s) { return (long)(P_NSCAL + s) * P.fstride; }
 inline __host__ __device__ long prim_rd(const Params &P, int s) { return (long)(P_NSCAL + P.ns + s) * P.fstride; }
 inline __host__ __device__ long prim_h(const Params &P, int s) { return (long)(P_NSCAL + 2 * P.ns + s) * P.fstride; }
-inline __host__ __device__ int nprim(int ns) { return P_NSCAL + 3 * ns; }
+// with Soret or Dufour on (tdr): per species the thermal diffusion ratio
+// TDRMIX and T * TDRMIX follow h
+inline __host__ __device__ long prim_tdr(const Params &P, int s) { return (long)(P_NSCAL + 3 * P.ns + s) * P.fstride; }
+inline __host__ __device__ long prim_tt(const Params &P, int s) { return (long)(P_NSCAL + 4 * P.ns + s) * P.fstride; }
+inline __host__ __device__ int nprim(int ns, int tdr) { return P_NSCAL + (tdr ? 5 : 3) * ns; }
 inline __host__ __device__ int nface(int ns) { return F_NSCAL + ns + 5 + ns; }
 inline __host__ __device__ int face_corr(int ns) { return F_NSCAL + ns; }
 
@@ -130,7 +138,7 @@ struct Launch {
   const sg_config *c;
   const Tables *tab;
   int ns, precision, bndx, inflow;
-  int ext;          // molar-mass or pressure diffusion on: the passes with those terms
+  int ext;          // a molecular-transport option on: the passes with those terms
   int pass_nw[3];   // pass tile per direction: 0 wide (rows of 32), 1 narrow (rows of 16)
   int mechfix;      // the chemistry is the compiled mechanism (MechH2): unrolled step loop
 };

@@ -960,6 +960,50 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
   }
 }
 
+// Soret / Dufour, after the point kernel: ln(T/TDIFGB), the thermal
+// diffusion ratio TDRMIX of each light species relative to the other light
+// species (src/rhscal.f:1353-1410) and T * TDRMIX, from the stage register
+// and the stored temperature (rounded to FP32 in the mixed form).  A kernel
+// of its own, so that the point kernels keep their registers.
+template <int NS, class R>
+__global__ void __launch_bounds__(BLK) k_tdr(const Member *__restrict__ mem, const uint32_t *__restrict__ map,
+                                             const sg_config *__restrict__ c) {
+  int lb;
+  const Member &M = member_at<L_POINT>(mem, map, blockIdx.x, lb);
+  const Params P = M.P;
+  if (!P.ltmask) return;
+  const Fields F = global_fields(M.F);
+  const long idx = (long)lb * BLK + threadIdx.x;
+  if (idx >= P.npts) return;
+  const long fs = P.fstride;
+  R *pr = (R *)F.prim;
+  const double d = F.q[idx], ovd = 1.0 / d;
+  double ry[NS];
+  double c3 = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    ry[s] = F.q[(5 + s) * fs + idx];
+    c3 = c3 + ry[s] * c->ovwmol[s];
+  }
+  const double wmomix = d * (1.0 / c3);
+  const double t = (double)pr[P_T * fs + idx];
+  pr[P_LNT * fs + idx] = (R)log(t / P.tdifgb);
+  const double combo2 = t / P.tdifgb;
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    if (!((P.ltmask >> k) & 1u)) continue;
+    double tdr = 0.0;
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
+      if (j == k || !((P.ltmask >> j) & 1u)) continue;
+      tdr = tdr + (ry[j] * ovd) * c->ovwmol[j] * wmomix * trfit(c->tdrcco[k][j], combo2);
+    }
+    tdr = tdr * (ry[k] * ovd) * wmomix * c->ovwmol[k];
+    pr[prim_tdr(P, k) + idx] = (R)tdr;
+    pr[prim_tt(P, k) + idx] = (R)(t * tdr);
+  }
+}
+
 // FP32 quotient and reciprocal from the hardware reciprocal (v_rcp_f32, about
 // 1 ulp; the quotient within 2 ulp) instead of the IEEE-exact FP32 division
 // sequence (~10 instructions each), for the FP32 transport and chemistry of
@@ -1578,7 +1622,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
   __shared__ R tow[EXT ? NIT * NTH : 1];
   auto stage_md = [&]() {
 #pragma unroll
-    for (int it = 0; it < NIT; it++) tmd[t + it * NTH] = pf[0][it];
+    for (int it = 0; it < NIT; it++) {
+      tmd[t + it * NTH] = pf[0][it];
+      if constexpr (EXT) tow[t + it * NTH] = pf[1][it];
+    }
     __syncthreads();
   };
   auto atm = [&](int kk) { return tmd[lbase + (kk - 8) * LSTR]; };
@@ -1644,7 +1691,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
   };
   auto f_pp = [&](long g) { return pp[g] - pref; };
   auto f_mu = [&](long g) { return mu[g]; };
+  const R *lw = pr + P_LNW * fs, *lp = pr + P_LNP * fs, *ow = pr + P_OVW * fs, *lt = pr + P_LNT * fs;
   prefetch([&](long g) { return md[g]; });
+  if constexpr (EXT) prefetch_to(1, [&](long g) { return ow[g]; });
   stage_md();
   // e, T and lambda under one barrier, u, v and w under the next, then
   // p - p_ref and mu
@@ -1695,11 +1744,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
   next();
   // ---------------- p - p_ref and mu under one barrier
   stage_n(2);
-  const R *lw = pr + P_LNW * fs, *lp = pr + P_LNP * fs, *ow = pr + P_OVW * fs;
   if constexpr (EXT) {
     prefetch_to(0, [&](long g) { return lw[g]; });
     prefetch_to(1, [&](long g) { return lp[g]; });
-    prefetch_to(2, [&](long g) { return ow[g]; });
+    prefetch_to(2, [&](long g) { return lt[g]; });
   } else {
     prefetch_species(0);
   }
@@ -1707,21 +1755,16 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
   dmu = D1f(1); mu0 = ctrf(1);
   next();
   pin(dmu); pin(mu0);
-  // ---------------- EXT: ln W and ln p under one barrier, 1/W into its
-  // resident tile (src/rhscal.f:945-992)
-  R dlw = R(0), d2lw = R(0), dlp = R(0), d2lp = R(0);
+  // ---------------- EXT: ln W, ln p and ln(T/TDIFGB) under one barrier
+  // (src/rhscal.f:945-1004); 1/W went into its resident tile with rho u_d
+  R dlw = R(0), d2lw = R(0), dlp = R(0), d2lp = R(0), dlt = R(0), d2lt = R(0);
   if constexpr (EXT) {
-#pragma unroll
-    for (int it = 0; it < NIT; it++) {
-      tile[buf][0][t + it * NTH] = pf[0][it];
-      tile[buf][1][t + it * NTH] = pf[1][it];
-      tow[t + it * NTH] = pf[2][it];
-    }
-    __syncthreads();
+    stage_n(3);
     prefetch_species(0);
     dlw = D1f(0); d2lw = D2f(0);
     dlp = D1f(1); d2lp = D2f(1);
-    pin(dlw); pin(d2lw); pin(dlp); pin(d2lp);
+    dlt = D1f(2); d2lt = D2f(2);
+    pin(dlw); pin(d2lw); pin(dlp); pin(d2lp); pin(dlt); pin(d2lt);
     next();
   }
   // ---------------- momentum convection (src/rhsvel.f:348-729, 1312-2110)
@@ -1772,8 +1815,19 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
   for (int sp = 0; sp < NS; sp++) {
     R dY, d2Y, yc, dYm, drD, rdc, dh, hc;
     stage_n(3);
-    if (sp + 1 < NS) prefetch_species(sp + 1);
-    else prefetch([&](long g) { return hm[g]; });
+    // a light species under Soret or Dufour stages TDRMIX and T * TDRMIX next
+    const bool ltsp = EXT && ((P.ltmask >> sp) & 1u);
+    auto prefetch_next = [&]() {
+      if (sp + 1 < NS) prefetch_species(sp + 1);
+      else prefetch([&](long g) { return hm[g]; });
+    };
+    if (ltsp) {
+      const R *Ts = pr + prim_tdr(P, sp), *TTs = pr + prim_tt(P, sp);
+      prefetch_to(0, [Ts](long g) { return Ts[g]; });
+      prefetch_to(1, [TTs](long g) { return TTs[g]; });
+    } else {
+      prefetch_next();
+    }
     dY = D1f(0); d2Y = D2f(0); yc = ctrf(0);
     pin(dY); pin(d2Y); pin(yc);
     dYm = D1Pf(0);
@@ -1794,14 +1848,46 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
       }
     }
     next();
+    // EXT, light species: TDRMIX (slot 0) and T * TDRMIX (slot 1) of this
+    // species under one more barrier, the Y_k and rho D_k tiles still in the
+    // other buffer
+    R tdr0 = R(0), ds7s = R(0), dtt = R(0), ttc = R(0);
+    if constexpr (EXT) {
+      if (ltsp) {
+        stage_n(2);
+        prefetch_next();
+        const int pb = buf ^ 1;
+        tdr0 = ctrf(0);
+        ttc = ctrf(1);
+        if (P.dufour) dtt = D1f(1);
+        if (P.soret) {
+          // d(rho D_k Y_k TDRMIX)/dx_d as the reference forms STORE7 (src/rhscal.f:2768)
+          auto val = [&](int l) { return tile[pb][1][l] * tile[pb][0][l] * tile[buf][0][l]; };
+          auto atp = [&](int kk) { return val(lbase + (kk - 8) * LSTR); };
+          auto fqp = [&](int q) { return val(lbase + q * LSTR); };
+          SG_PHASE();
+          if (BND) { int pq[5]; pkc(pq); ds7s = d1_rows<R>(P, pq, DIR, wq, fqp); }
+          else ds7s = d1<false, R>(P, S, DIR, atp);
+          SG_PHASE();
+        }
+        pin(tdr0); pin(ttc); pin(dtt); pin(ds7s);
+        next();
+      }
+    }
+    // Dufour: the species enthalpy carries R_k T TDRMIX through the heat-flux
+    // terms up to the Soret terms (src/rhscal.f:1549-1566, 2832-2845)
+    const bool dfsp = EXT && ltsp && P.dufour;
+    const R rgk = dfsp ? (R)F.tab->rgs[sp] : R(0);
+    const R hcu = dfsp ? hc + rgk * ttc : hc;
+    const R dhu = dfsp ? dh + rgk * dtt : dh;
     R ys = -R(0.5) * yc * dm - R(0.5) * dYm - R(0.5) * md0 * dY;
     const R g1 = (zdif ? R(0) : drD) * dY;
     const R g2 = rdc * (zdif ? R(0) : d2Y);
     ys = ys + g1 + g2;
     vt = vt + g1 + g2;
-    aE = aE + hc * ((zdifadb ? R(0) : drD) * dY);
-    aE = aE + ((zdifadb ? R(0) : dh) * dY) * rdc;
-    aE = aE + (rdc * (zdifadb ? R(0) : d2Y)) * hc;
+    aE = aE + hcu * ((zdifadb ? R(0) : drD) * dY);
+    aE = aE + ((zdifadb ? R(0) : dhu) * dY) * rdc;
+    aE = aE + (rdc * (zdifadb ? R(0) : d2Y)) * hcu;
     ucor = ucor + rdc * dY;
     if constexpr (EXT) {
       // molar-mass (src/rhscal.f:1970-2364) and pressure (2365-2759)
@@ -1810,21 +1896,25 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
       // reference's STORE4 holds it at that point instead of dh/dx
       // (src/rhscal.f:1887, 2071-2103, 2467-2499).
       const R s7w = rdc * yc;
-      const R dhx = DIR == 0 ? rdc : dh;
-      auto term = [&](R s7, R ds7, R d1l, R d2l) {
+      auto term = [&](R s7, R ds7, R d1l, R d2l, R ut, R dut) {
         ucor = ucor + s7 * d1l;
         const R f1 = (zdif ? R(0) : ds7) * d1l;
         ys = ys + f1;
         vt = vt + f1;
-        aE = aE + ((zdifadb ? R(0) : ds7) * d1l) * hc;
-        aE = aE + ((zdifadb ? R(0) : dhx) * d1l) * s7;
+        aE = aE + ((zdifadb ? R(0) : ds7) * d1l) * ut;
+        aE = aE + ((zdifadb ? R(0) : dut) * d1l) * s7;
         const R f3 = (zdif ? R(0) : d2l) * s7;
         ys = ys + f3;
         vt = vt + f3;
-        aE = aE + ((zdifadb ? R(0) : d2l) * s7) * hc;
+        aE = aE + ((zdifadb ? R(0) : d2l) * s7) * ut;
       };
-      if (P.mixw) term(s7w, ds7w, dlw, d2lw);
-      if (P.mixp) term(s7w * fw, ds7p, dlp, d2lp);
+      const R dhx = DIR == 0 ? rdc : dhu;
+      if (P.mixw) term(s7w, ds7w, dlw, d2lw, hcu, dhx);
+      if (P.mixp) term(s7w * fw, ds7p, dlp, d2lp, hcu, dhx);
+      // Soret (src/rhscal.f:2757-3758): rho D_k Y_k TDRMIX with ln(T/TDIFGB);
+      // its heat-flux terms take the species enthalpy without the Dufour part
+      // and its gradient afresh
+      if (ltsp && P.soret) term(s7w * tdr0, ds7s, dlt, d2lt, hc, dh);
       pin(ys); pin(vt); pin(ucor); pin(aE);
     }
     aY[sp] = ys;
@@ -2338,7 +2428,7 @@ void launch_bountt_t(const Launch &L, hipStream_t st) {
   hipLaunchKernelGGL(k_bountt<NS>, dim3(L.nblk[L_FACE]), dim3(BLK), 0, st, L.mem, L.map[L_FACE]);
 }
 template <int NS>
-void launch_prim_t(const Launch &L, hipStream_t st) {
+static void launch_prim_k(const Launch &L, hipStream_t st) {
   const dim3 g(L.nblk[L_POINT]), b(BLK);
   if constexpr (NS == MechH2::nspec) {
     if (L.mechfix) {
@@ -2349,6 +2439,15 @@ void launch_prim_t(const Launch &L, hipStream_t st) {
   }
   if (L.precision) hipLaunchKernelGGL((k_prim_mx<NS, false>), g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
   else hipLaunchKernelGGL((k_prim<NS, false>), g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
+}
+template <int NS>
+void launch_prim_t(const Launch &L, hipStream_t st) {
+  launch_prim_k<NS>(L, st);
+  if (L.ext) {   // Soret / Dufour quantities of the members that need them
+    const dim3 g(L.nblk[L_POINT]), b(BLK);
+    if (L.precision) hipLaunchKernelGGL((k_tdr<NS, float>), g, b, 0, st, L.mem, L.map[L_POINT], L.c);
+    else hipLaunchKernelGGL((k_tdr<NS, double>), g, b, 0, st, L.mem, L.map[L_POINT], L.c);
+  }
 }
 template <int NS, class R, int NW, bool EXT>
 static void launch_pass_w(int dir, const Launch &L, hipStream_t st) {

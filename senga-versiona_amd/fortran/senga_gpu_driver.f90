@@ -84,7 +84,7 @@ contains
 
   subroutine run_case()
 
-  if (sg_abi_version() /= 2 .or. sg_config_size() /= c_sizeof(cfg)) call fail('library ABI mismatch')
+  if (sg_abi_version() /= 3 .or. sg_config_size() /= c_sizeof(cfg)) call fail('library ABI mismatch')
   ns = cfg%nspec
   npts = ctl%nx * ctl%ny * ctl%nz
   allocate(run(npts, 5 + ns))

@@ -18,7 +18,7 @@ module senga_gpu
   integer, parameter, public :: SG_OK = 0, SG_EINVAL = 1, SG_ENOMEM = 2, SG_EDEVICE = 3
   integer, parameter, public :: SG_ETEMPER = 4, SG_ETEMPIN = 5, SG_ESTATE = 6
   integer, parameter, public :: SG_BC_PERIODIC = 1, SG_BC_INFLOW3 = 13, SG_BC_OUTFLOW1 = 21
-  ! sg_config%physics bits: optional physics switched on in cont.dat (all refused)
+  ! sg_config%physics bits: optional physics switched on in cont.dat (radiation refused)
   integer, parameter, public :: SG_PHYS_MIXW = 1, SG_PHYS_MIXP = 2, SG_PHYS_SORET = 4
   integer, parameter, public :: SG_PHYS_DUFOUR = 8, SG_PHYS_RADN = 16
 
@@ -78,6 +78,9 @@ module senga_gpu
     real(c_double) :: diffco(SG_NTRC, SG_MAX_SPEC, SG_MAX_SPEC)
     real(c_double) :: wilko1(SG_MAX_SPEC, SG_MAX_SPEC)
     real(c_double) :: wilko2(SG_MAX_SPEC, SG_MAX_SPEC)
+    integer(c_int32_t) :: ncotdr, reserved2
+    real(c_double) :: wmltdr
+    real(c_double) :: tdrcco(SG_NTRC, SG_MAX_SPEC, SG_MAX_SPEC)
   end type sg_config_t
 
   public :: sg_abi_version, sg_config_size, sg_create, sg_destroy

@@ -420,7 +420,7 @@ contains
     character(len=*), intent(in) :: path
     type(sg_config_t), intent(inout) :: cfg
     integer, intent(out) :: ierr
-    integer :: u, ns, i, j, k, n, jsp, ncv, ncc, ncd
+    integer :: u, ns, i, j, k, n, jsp, ncv, ncc, ncd, nct
     character(len=32) :: sym
     real(c_double) :: fornow
     open(newunit=u, file=path, status='old', form='formatted', iostat=ierr)
@@ -469,10 +469,34 @@ contains
       end do
       if (i == 1) cfg%ncodif = ncd
     end do
+    ! thermal diffusion ratios: species i lists j < i (src/diffin.f:180-195)
+    read(u, *)
+    do i = 1, ns
+      read(u, *) jsp, nct
+      if (nct > SG_NTRC) then
+        ierr = 1
+        close(u)
+        return
+      end if
+      do j = 1, i - 1
+        read(u, *) k, cfg%tdrcco(1:nct, j, i)
+      end do
+      if (i == 1) cfg%ncotdr = nct
+    end do
     close(u)
     do i = 1, ns
       do j = i + 1, ns
         cfg%diffco(:, j, i) = cfg%diffco(:, i, j)
+      end do
+    end do
+    ! antisymmetric fill in the reference's order (src/diffin.f:345-368)
+    do i = 1, ns
+      cfg%tdrcco(:, i, i) = 0.0d0
+      do j = 1, i - 1
+        cfg%tdrcco(:, j, i) = -cfg%tdrcco(:, j, i)
+      end do
+      do j = i + 1, ns
+        cfg%tdrcco(:, j, i) = cfg%tdrcco(:, i, j)
       end do
     end do
     do i = 1, ns
@@ -497,7 +521,7 @@ contains
     cfg%mixavg = merge(1, 0, ctl%nfmavt == 1)
     ! optional physics that is switched on: the molecular-transport options
     ! act only with mixture-averaged transport (src/diffin.f:66,384-407).
-    ! sg_create refuses any of them.
+    ! sg_create refuses radiation.
     cfg%physics = 0
     if (ctl%nfmavt == 1) then
       if (ctl%nfmixw == 1) cfg%physics = ior(cfg%physics, SG_PHYS_MIXW)
@@ -507,6 +531,7 @@ contains
     end if
     if (ctl%nfradn == 1) cfg%physics = ior(cfg%physics, SG_PHYS_RADN)
     cfg%precision = 0
+    cfg%wmltdr = ctl%wmltdr
     cfg%xgdlen = ctl%xgdlen
     cfg%ygdlen = ctl%ygdlen
     cfg%zgdlen = ctl%zgdlen

@@ -64,6 +64,8 @@ class Config(C.Structure):
         ("viscco", (d * NTRC) * MAX_SPEC), ("condco", (d * NTRC) * MAX_SPEC),
         ("diffco", ((d * NTRC) * MAX_SPEC) * MAX_SPEC),
         ("wilko1", (d * MAX_SPEC) * MAX_SPEC), ("wilko2", (d * MAX_SPEC) * MAX_SPEC),
+        ("ncotdr", i32), ("reserved2", i32), ("wmltdr", d),
+        ("tdrcco", ((d * NTRC) * MAX_SPEC) * MAX_SPEC),
     ]
 
 
@@ -185,6 +187,10 @@ def config_from_case(case, precision=0) -> Config:
                     c.diffco[s][j][q] = dt.diffco[q, j, s]
                 c.wilko1[s][j] = dt.wilko1[j, s]
                 c.wilko2[s][j] = dt.wilko2[j, s]
+                for q in range(dt.ncotdr):
+                    c.tdrcco[s][j][q] = dt.tdrcco[q, j, s]
+        c.ncotdr = dt.ncotdr
+    c.wmltdr = ctl.wmltdr
     return c
 
 

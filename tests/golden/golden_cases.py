@@ -36,6 +36,21 @@ def build(name):
         c = C.h2_nscbc(n=(11, 12, 11), length=(1.4e-3, 1.5e-3, 1.4e-3), mixavg=True, periodic_x=True)
         c.control.nfmixw = c.control.nfmixp = 1
         return c
+    if name == "h2_nscbc_sordf_12x11x11":
+        # Soret and Dufour for the light species H2, H (src/rhscal.f:1353-1410,
+        # 1549-1566, 2757-3758)
+        c = C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=True)
+        c.control.nfmsor = c.control.nfmduf = 1
+        return c
+    if name == "h2_per_duf_11x12x11":
+        c = C.h2_nscbc(n=(11, 12, 11), length=(1.4e-3, 1.5e-3, 1.4e-3), mixavg=True, periodic_x=True)
+        c.control.nfmduf = 1
+        return c
+    if name == "h2_nscbc_alltr_12x11x11":
+        # all four molecular-transport options together
+        c = C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=True)
+        c.control.nfmixw = c.control.nfmixp = c.control.nfmsor = c.control.nfmduf = 1
+        return c
     if name == ADAPT:
         c = C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=True)
         c.control.nstpsw = 1
@@ -47,7 +62,8 @@ ADAPT = "h2_nscbc_mix_adapt_12x11x11"
 
 
 NAMES = ["periodic_1step_16x12x11", "h2_nscbc_mix_12x11x11", "h2_nscbc_cle_12x11x11", "h2_per_mix_11x12x11",
-         "h2_falloff_nscbc_mix_12x11x11", "ch4_4step_16x12x11", "h2_nscbc_mixwp_12x11x11", "h2_per_mixwp_11x12x11"]
+         "h2_falloff_nscbc_mix_12x11x11", "ch4_4step_16x12x11", "h2_nscbc_mixwp_12x11x11", "h2_per_mixwp_11x12x11",
+         "h2_nscbc_sordf_12x11x11", "h2_per_duf_11x12x11", "h2_nscbc_alltr_12x11x11"]
 OPS = {"ops_periodic_16x12x11": "periodic_1step_16x12x11", "ops_boundedx_12x11x11": "h2_nscbc_cle_12x11x11"}
 
 

@@ -36,7 +36,7 @@ def test_exports_every_declared_symbol(lib):
 
 
 def test_abi_version_and_layout(lib):
-    assert lib.sg_abi_version() == 2
+    assert lib.sg_abi_version() == 3
     assert lib.sg_config_size() == C.sizeof(abi.Config)
 
 
@@ -59,13 +59,12 @@ def test_rejects_unsupported_configs(lib):
     assert lib.sg_create(None, C.byref(h)) == 1
 
 
-@pytest.mark.parametrize("switch,bit", [("nfmsor", abi.PHYS_SORET), ("nfmduf", abi.PHYS_DUFOUR),
-                                        ("nfradn", abi.PHYS_RADN)])
+@pytest.mark.parametrize("switch,bit", [("nfradn", abi.PHYS_RADN)])
 def test_rejects_each_optional_physics_switch(lib, switch, bit, tmp_path):
-    """Soret, Dufour and radiation (cont.dat switches, src/contin.f:194-209)
-    are not computed here: a case that switches one on is refused by
-    sg_create and sg_footprint instead of run without it.  The switch reaches
-    sg_config through both readers."""
+    """Radiation (cont.dat switch, src/contin.f:194-209) is not computed
+    here: a case that switches it on is refused by sg_create and sg_footprint
+    instead of run without it.  The switch reaches sg_config through both
+    readers."""
     c = CS.h2_nscbc(n=(12, 11, 11), length=(1e-3, 1e-3, 1e-3), mixavg=True)
     setattr(c.control, switch, 1)
     p = tmp_path / "cont.dat"
@@ -219,10 +218,12 @@ def test_serving_summary():
     assert abs(s["mpoint_substages_per_s"] - 10 * 1000 * 50 / 6.4 / 1e6) < 1e-9
 
 
-@pytest.mark.parametrize("switch,bit", [("nfmixw", abi.PHYS_MIXW), ("nfmixp", abi.PHYS_MIXP)])
-def test_accepts_molar_mass_and_pressure_diffusion(lib, switch, bit, tmp_path):
-    """The molar-mass and pressure diffusion terms are computed: the switch
-    reaches sg_config through the reader, and sg_footprint accepts it."""
+@pytest.mark.parametrize("switch,bit", [("nfmixw", abi.PHYS_MIXW), ("nfmixp", abi.PHYS_MIXP),
+                                        ("nfmsor", abi.PHYS_SORET), ("nfmduf", abi.PHYS_DUFOUR)])
+def test_accepts_molecular_transport_options(lib, switch, bit, tmp_path):
+    """The four molecular-transport options are computed: the switch reaches
+    sg_config through the reader, and sg_footprint accepts it (not with
+    radiation).  Soret and Dufour need the thermal diffusion ratio fits."""
     c = CS.h2_nscbc(n=(12, 11, 11), length=(1e-3, 1e-3, 1e-3), mixavg=True)
     setattr(c.control, switch, 1)
     p = tmp_path / "cont.dat"
@@ -232,5 +233,10 @@ def test_accepts_molar_mass_and_pressure_diffusion(lib, switch, bit, tmp_path):
     assert cfg.physics == bit
     out = C.c_size_t(0)
     assert lib.sg_footprint(C.byref(cfg), C.byref(out)) == 0
-    cfg.physics = bit | abi.PHYS_SORET
+    cfg.physics = bit | abi.PHYS_RADN
     assert lib.sg_footprint(C.byref(cfg), C.byref(out)) == 1
+    if bit in (abi.PHYS_SORET, abi.PHYS_DUFOUR):
+        assert cfg.ncotdr == 4
+        cfg.physics = bit
+        cfg.ncotdr = 0
+        assert lib.sg_footprint(C.byref(cfg), C.byref(out)) == 1

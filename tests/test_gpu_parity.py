@@ -132,18 +132,23 @@ def _gpu_after(c, steps, precision):
     return run, T
 
 
+@pytest.mark.parametrize("options", ["mixwp", "sordf", "all"])
 @pytest.mark.parametrize("periodic", [False, True])
-def test_transport_terms_against_oracle(periodic):
-    """Molar-mass and pressure diffusion (cont.dat NFMIXW, NFMIXP): the passes
-    with those terms against the oracle (itself bit-exact against the
-    reference's fixtures) on larger grids than the fixtures, after 6 steps:
-    FP64 within the bar, the mixed form within 1e-5.  A duct with NSCBC x
-    faces and a triply periodic box."""
+def test_transport_terms_against_oracle(periodic, options):
+    """The molecular-transport options (cont.dat NFMIXW, NFMIXP, NFMSOR,
+    NFMDUF): molar-mass and pressure diffusion, Soret and Dufour, and all
+    four, through the passes with those terms, against the oracle (itself
+    bit-exact against the reference's fixtures) on larger grids than the
+    fixtures, after 6 steps: FP64 within the bar, the mixed form within
+    1e-5.  A duct with NSCBC x faces and a triply periodic box."""
     from sengapy import case as C
     n = (40, 24, 24)
     c = C.h2_nscbc(n=n, length=(4e-3, 2.4e-3, 2.4e-3), mixavg=True, periodic_x=periodic, nsteps=6)
-    c.control.nfmixw = c.control.nfmixp = 1
-    print("case", c.name, "molar-mass and pressure diffusion, 6 steps", flush=True)
+    if options in ("mixwp", "all"):
+        c.control.nfmixw = c.control.nfmixp = 1
+    if options in ("sordf", "all"):
+        c.control.nfmsor = c.control.nfmduf = 1
+    print("case", c.name, options, "6 steps", flush=True)
     run_o, T_o = _oracle_after(c, 6)
     run, T = _gpu_after(c, 6, 0)
     assert_state_close("fp64 vs oracle", run, T, run_o, T_o, TOL, 1e-12, tol_pw=PW_TOL64)

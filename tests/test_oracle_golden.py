@@ -60,12 +60,15 @@ def test_adaptive_step_bitexact(oracle_built):
     assert np.array_equal(T2, fx["T2"])
 
 
-@pytest.mark.parametrize("flag", ["nfmixw", "nfmixp"])
-def test_transport_fixture_exercises_each_term(flag, oracle_built):
-    """The molar-mass and pressure diffusion fixture pins both terms: with
-    either switch off the oracle no longer reproduces the reference's
+@pytest.mark.parametrize("name,flag", [("h2_nscbc_mixwp_12x11x11", "nfmixw"), ("h2_nscbc_mixwp_12x11x11", "nfmixp"),
+                                       ("h2_nscbc_sordf_12x11x11", "nfmsor"), ("h2_nscbc_sordf_12x11x11", "nfmduf"),
+                                       ("h2_per_duf_11x12x11", "nfmduf"), ("h2_nscbc_alltr_12x11x11", "nfmixw"),
+                                       ("h2_nscbc_alltr_12x11x11", "nfmsor")])
+def test_transport_fixture_exercises_each_term(name, flag, oracle_built):
+    """The molecular-transport fixtures pin each term they switch on: with
+    that switch off the oracle no longer reproduces the reference's
     right-hand side (the density and momentum are untouched by them)."""
-    c, fx = golden_cases.load("h2_nscbc_mixwp_12x11x11")
+    c, fx = golden_cases.load(name)
     setattr(c.control, flag, 0)
     o = Oracle(c)
     o.set_timestep(c.dump.tstep)
