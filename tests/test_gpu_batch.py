@@ -193,3 +193,46 @@ def test_batch_autotune_keeps_results():
     b.set_tiles((1, 1, 1))
     assert b.tiles() == (1, 1, 1)
     b.close()
+
+
+def test_batch_with_transport_options():
+    """Cases with the four molecular-transport options (the passes with those
+    terms and the thermal-diffusion kernel) batched: each member equals its
+    solo run bit for bit; a member without the options is refused."""
+    from sengapy import case as C
+    from sengapy.abi import Batch, SgError, Solver
+    cases = []
+    for n, L in (((24, 16, 16), (3e-3, 2e-3, 2e-3)), ((32, 20, 16), (4e-3, 2.5e-3, 2e-3))):
+        c = C.h2_nscbc(n=n, length=L, mixavg=True, nsteps=3)
+        c.control.nfmixw = c.control.nfmixp = c.control.nfmsor = c.control.nfmduf = 1
+        cases.append(c)
+    print("batch with transport options", [c.shape for c in cases], flush=True)
+
+    def solver(c):
+        s = Solver(c)
+        s.set_timestep(c.dump.tstep)
+        s.upload(c.dump.state, 0.0)
+        return s
+    ref = []
+    for c in cases:
+        s = solver(c)
+        s.step(3)
+        ref.append(s.download())
+        s.close()
+    b = Batch()
+    sol = [solver(c) for c in cases]
+    for s in sol:
+        b.add(s)
+    plain = C.h2_nscbc(n=(24, 16, 16), length=(3e-3, 2e-3, 2e-3), mixavg=True, nsteps=1)
+    p = solver(plain)
+    with pytest.raises(SgError) as e:
+        b.add(p)
+    assert e.value.code == SG_EINVAL
+    b.step(3)
+    b.sync()
+    for q, s in enumerate(sol):
+        run, T, P, _ = s.download()
+        assert np.array_equal(run, ref[q][0]), f"member {q} registers differ"
+        assert np.array_equal(T, ref[q][1]), f"member {q} temperature differs"
+    b.close()
+    p.close()
