@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define SG_ABI_VERSION 3
+#define SG_ABI_VERSION 4
 
 #define SG_MAX_SPEC 18   /* the reference's largest shipped mechanism (casefiles/68step: NSPCMX=18) */
 #define SG_MAX_STEP 128
@@ -46,6 +46,7 @@ extern "C" {
 #define SG_NCOF 7        /* NASA-7 coefficients per interval */
 #define SG_NTINT 2       /* temperature intervals per species */
 #define SG_NTRC 4        /* coefficients of the transport fits (diff.dat) */
+#define SG_NKPRAD 6      /* coefficients of a Planck mean absorption fit (radn.dat, src/com_senga2.h NCFRMX) */
 
 /* error codes */
 #define SG_OK 0
@@ -64,11 +65,9 @@ extern "C" {
 /* sg_config.physics bits: the reference's optional physics switches from
  * cont.dat (src/contin.f:194-209), set when the switch is on -- the four
  * molecular-transport options only together with mixture-averaged transport,
- * as src/diffin.f:66,384-407 applies them.  The four molecular-transport
- * terms are computed (with mixture-averaged transport; without it they are
- * inactive, as in the reference); radiation is not: sg_create and
- * sg_footprint return SG_EINVAL when its bit is set, so that such a case is
- * refused instead of run without it. */
+ * as src/diffin.f:66,384-407 applies them.  All are computed (the transport
+ * options with mixture-averaged transport; without it they are inactive, as
+ * in the reference). */
 #define SG_PHYS_MIXW 1    /* NFMIXW: molar-mass diffusion term (src/rhscal.f:951,1970) */
 #define SG_PHYS_MIXP 2    /* NFMIXP: pressure diffusion term (src/rhscal.f:973,2365) */
 #define SG_PHYS_SORET 4   /* NFMSOR: thermal diffusion of light species (src/rhscal.f:2760) */
@@ -91,7 +90,7 @@ typedef struct sg_config {
                                  has no such renormalisation (src/lincom.f:294-
                                  333 is commented out) and the FP64 form never
                                  applies it. */
-  int32_t physics;            /* SG_PHYS_* bits of switches that are on (RADN refused) */
+  int32_t physics;            /* SG_PHYS_* bits of switches that are on */
   double xgdlen, ygdlen, zgdlen;  /* domain lengths; spacing L/(N-1) (src/dfinit.f:42-50) */
 
   /* boundary conditions, faces xl, xr, yl, yr, zl, zr (cont.dat lines 51-56) */
@@ -165,6 +164,16 @@ typedef struct sg_config {
   int32_t ncotdr, reserved2;
   double wmltdr;
   double tdrcco[SG_MAX_SPEC][SG_MAX_SPEC][SG_NTRC];
+
+  /* optically thin radiation (NFRADN; radn.dat, src/radcin.f:76-126): nsprad
+   * radiating species, species nsprid[r] (0-based) with the Planck mean
+   * absorption coefficient sum_c akprad[r][c] T^c over nkprad[r] terms;
+   * trefrn: the ambient temperature (cont.dat TREFRN) */
+  int32_t nsprad, reserved3;
+  int32_t nsprid[SG_MAX_SPEC];
+  int32_t nkprad[SG_MAX_SPEC];
+  double trefrn;
+  double akprad[SG_MAX_SPEC][SG_NKPRAD];
 } sg_config;
 
 typedef struct sg_handle sg_handle;
@@ -180,7 +189,8 @@ size_t sg_config_size(void);
  * not compiled in (2..SG_MAX_SPEC are), nx/ny/nz < 11, more than 2^29 points
  * per field, cp fits of more than 5 coefficients (src/temper.f NCTMAX),
  * transport fits or thermal diffusion ratios of more than SG_NTRC
- * coefficients, SG_PHYS_RADN, or a
+ * coefficients, radiation data out of range (species index, more than
+ * SG_NKPRAD coefficients), or a
  * step list, species index, third-body or falloff index out of range. */
 int sg_create(const sg_config *cfg, sg_handle **out);
 int sg_destroy(sg_handle *h);

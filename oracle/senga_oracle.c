@@ -83,6 +83,7 @@ struct orc {
      and the light species they act for (FLMTDR, bit per species) */
   double *td1[3], *td2[3], *tdrmix;
   int soret, dufour;
+  int radn;                /* optically thin radiation (FLRADN) */
   unsigned ltmask;
   double *rate, *rrte;
   /* face arrays (ny*nz, and *ns for species) */
@@ -913,6 +914,28 @@ static int rhscal(orc_t *o) {
     size_t p = II(o, i, j, k), b = BI(o, i, j, k);
     E[b] = E[b] + (s1[p] + s2[p] + s3[p]) * s7[b];
   }
+  /* optically thin radiation, rhscal.f:810 (src/radcal.f:36-111): the
+     species are still the conserved rho Y_k here */
+  if (o->radn) {
+    const double foursb = 4.0 * 5.67037321e-08;
+    const double trfrth = c->trefrn * c->trefrn * c->trefrn * c->trefrn;
+    FOR_I(o) {
+      size_t b = BI(o, i, j, k);
+      double store1 = ZERO;
+      for (int r = 0; r < c->nsprad; r++) {
+        double fornow = o->trun[b];
+        int nk = c->nkprad[r];
+        double plspec = c->akprad[r][nk - 1];
+        for (int q = nk - 2; q >= 0; q--) plspec = plspec * fornow + c->akprad[r][q];
+        int ks = c->nsprid[r];
+        fornow = R(o, 5 + ks)[b] * c->rgspec[ks] * o->trun[b];
+        store1 = store1 + plspec * fornow;
+      }
+      double fornow = o->trun[b];
+      fornow = fornow * fornow * fornow * fornow;
+      E[b] = E[b] - foursb * store1 * (fornow - trfrth);
+    }
+  }
   /* chemistry, rhscal.f:822-832 */
   chrate(o);
   memcpy(o->rrte, o->rate, sizeof(double) * ni * ns);
@@ -1729,10 +1752,14 @@ orc_t *orc_create(const sg_config *cfg, int *err) {
     *err = SG_EINVAL;
     return NULL;
   }
-  /* radiation is not restated */
   if (cfg->physics & SG_PHYS_RADN) {
-    *err = SG_EINVAL;
-    return NULL;
+    int bad = cfg->nsprad < 0 || cfg->nsprad > cfg->nspec;
+    for (int r = 0; !bad && r < cfg->nsprad; r++)
+      bad = cfg->nsprid[r] < 0 || cfg->nsprid[r] >= cfg->nspec || cfg->nkprad[r] < 1 || cfg->nkprad[r] > SG_NKPRAD;
+    if (bad) {
+      *err = SG_EINVAL;
+      return NULL;
+    }
   }
   orc_t *o = (orc_t *)calloc(1, sizeof(orc_t));
   o->c = *cfg;
@@ -1765,6 +1792,7 @@ orc_t *orc_create(const sg_config *cfg, int *err) {
      only (src/diffin.f:384-386) */
   o->mixw = cfg->mixavg && (cfg->physics & SG_PHYS_MIXW);
   o->mixp = cfg->mixavg && (cfg->physics & SG_PHYS_MIXP);
+  o->radn = (cfg->physics & SG_PHYS_RADN) != 0;
   o->soret = cfg->mixavg && (cfg->physics & SG_PHYS_SORET);
   o->dufour = cfg->mixavg && (cfg->physics & SG_PHYS_DUFOUR);
   o->ltmask = 0;

@@ -20,9 +20,9 @@ using namespace sg;
 
 namespace {
 
-enum KernelId { K_TEMPIN = 0, K_BOUNDT, K_PRIM, K_PASSX, K_PASSY, K_PASSZ, K_FINAL, K_BOUNTT, K_DERIVS, K_NSCBC, K_COUNT };
+enum KernelId { K_TEMPIN = 0, K_BOUNDT, K_PRIM, K_PASSX, K_PASSY, K_PASSZ, K_FINAL, K_BOUNTT, K_DERIVS, K_NSCBC, K_RADN, K_COUNT };
 const char *kKernelNames[K_COUNT] = {"tempin", "boundt", "prim", "pass_x", "pass_y", "pass_z",
-                                     "final", "bountt", "derivs", "nscbc"};
+                                     "final", "bountt", "derivs", "nscbc", "radn"};
 
 }  // namespace
 
@@ -293,6 +293,7 @@ static int setup_params(sg_handle *h) {
   // the molecular-transport options act with mixture-averaged transport only (src/diffin.f:384-386)
   P.mixw = c.mixavg && (c.physics & SG_PHYS_MIXW) ? 1 : 0;
   P.mixp = c.mixavg && (c.physics & SG_PHYS_MIXP) ? 1 : 0;
+  P.radn = (c.physics & SG_PHYS_RADN) ? 1 : 0;
   P.soret = c.mixavg && (c.physics & SG_PHYS_SORET) ? 1 : 0;
   P.dufour = c.mixavg && (c.physics & SG_PHYS_DUFOUR) ? 1 : 0;
   // the thermal diffusion flags FLMTDR/FLMSOR/FLMDUF of src/diffin.f:388-407
@@ -366,6 +367,7 @@ static int init_launch(sg_handle *h, const int nw[3]) {
   L.precision = h->P.precision;
   L.bndx = h->P.bnd[0];
   L.ext = h->P.mixw || h->P.mixp || h->P.soret || h->P.dufour;
+  L.radn = h->P.radn;
   L.inflow = h->P.inflow_xl;
   L.mechfix = h->mechfix;
   return 0;
@@ -421,9 +423,20 @@ static void batch_touch(sg_batch *b) { b->dirty_members = 1; }
   X(ntint) X(olewis) X(ovwmol) X(rclind) X(rcsrif) X(rctroe) X(rgspec) X(rparam) X(tinthi) X(viscco)  \
   X(wilko1) X(wilko2) X(ncovis) X(ncocon) X(ncodif)
 
+// the thermal diffusion ratio fits and the radiation data, read from the
+// shared configuration by the thermal-diffusion and radiation kernels
+#define SG_TDR_FIELDS(X) X(ncotdr) X(wmltdr) X(tdrcco)
+#define SG_RADN_FIELDS(X) X(nsprad) X(nsprid) X(nkprad) X(trefrn) X(akprad)
+
 static bool same_mechanism(const sg_handle *a, const sg_handle *b) {
 #define CMP(f) if (memcmp(&a->cfg.f, &b->cfg.f, sizeof a->cfg.f) != 0) return false;
   SG_MECH_FIELDS(CMP)
+  if ((a->cfg.physics | b->cfg.physics) & (SG_PHYS_SORET | SG_PHYS_DUFOUR)) {
+    SG_TDR_FIELDS(CMP)
+  }
+  if ((a->cfg.physics | b->cfg.physics) & SG_PHYS_RADN) {
+    SG_RADN_FIELDS(CMP)
+  }
 #undef CMP
   // thermodynamic tables (the derivative row weights depend on the grid
   // spacing and stay per member)
@@ -485,6 +498,8 @@ static int batch_prepare(sg_batch *b) {
     b->L.precision = h0->P.precision;
     b->L.bndx = h0->P.bnd[0];
     b->L.ext = h0->P.mixw || h0->P.mixp || h0->P.soret || h0->P.dufour;
+    b->L.radn = 0;
+    for (int m = 0; m < nm; m++) b->L.radn |= b->members[m]->P.radn;
     b->L.mechfix = h0->mechfix;   // members share one mechanism
     b->L.inflow = 0;
     for (int m = 0; m < nm; m++) b->L.inflow |= b->members[m]->P.inflow_xl;
@@ -508,6 +523,7 @@ static int batch_capture(sg_batch *b) {
     if (b->L.inflow) launch_boundt(b->L, b->stream);
     launch_prim(b->L, b->stream);
     launch_pass(0, b->L, b->stream);
+    if (b->L.radn) launch_radn(b->L, b->stream);
     launch_pass(1, b->L, b->stream);
     launch_pass(2, b->L, b->stream);
     if (b->L.bndx) launch_nscbc(b->L, b->stream);
@@ -559,8 +575,13 @@ static bool mechanism_in_range(const sg_config *c) {
 
 static int validate_config(const sg_config *cfg) {
   if (!species_supported(cfg->nspec)) return SG_EINVAL;   // compiled species counts
-  // optional physics not computed here: radiation
-  if (cfg->physics & SG_PHYS_RADN) return SG_EINVAL;
+  // radiation data: species in range, at most SG_NKPRAD coefficients
+  if (cfg->physics & SG_PHYS_RADN) {
+    if (cfg->nsprad < 0 || cfg->nsprad > cfg->nspec) return SG_EINVAL;
+    for (int r = 0; r < cfg->nsprad; r++)
+      if (cfg->nsprid[r] < 0 || cfg->nsprid[r] >= cfg->nspec || cfg->nkprad[r] < 1 || cfg->nkprad[r] > SG_NKPRAD)
+        return SG_EINVAL;
+  }
   if (cfg->mixavg && (cfg->physics & (SG_PHYS_SORET | SG_PHYS_DUFOUR)) && (cfg->ncotdr < 1 || cfg->ncotdr > SG_NTRC))
     return SG_EINVAL;
   if (cfg->nstep < 0 || cfg->nstep > SG_MAX_STEP) return SG_EINVAL;
@@ -823,6 +844,7 @@ static int eval_rhs(sg_handle *h) {
   if (L.inflow) rc = timed(h, K_BOUNDT, [&] { launch_boundt(L, h->stream); });
   if (!rc) rc = timed(h, K_PRIM, [&] { launch_prim(L, h->stream); });
   if (!rc) rc = timed(h, K_PASSX, [&] { launch_pass(0, L, h->stream); });
+  if (!rc && L.radn) rc = timed(h, K_RADN, [&] { launch_radn(L, h->stream); });
   if (!rc) rc = timed(h, K_PASSY, [&] { launch_pass(1, L, h->stream); });
   if (!rc) rc = timed(h, K_PASSZ, [&] { launch_pass(2, L, h->stream); });
   if (!rc && L.bndx) rc = timed(h, K_NSCBC, [&] { launch_nscbc(L, h->stream); });

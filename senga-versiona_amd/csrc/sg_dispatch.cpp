@@ -15,6 +15,7 @@ template <int NS> void launch_bountt_t(const Launch &, hipStream_t);
 template <int NS> void launch_prim_t(const Launch &, hipStream_t);
 template <int NS> void launch_pass_t(int, const Launch &, hipStream_t);
 template <int NS> void launch_nscbc_t(const Launch &, hipStream_t);
+template <int NS> void launch_radn_t(const Launch &, hipStream_t);
 template <int NS> void launch_final_t(const Launch &, int, double *, hipStream_t);
 
 bool species_supported(int ns) {
@@ -59,6 +60,14 @@ void launch_prim(const Launch &L, hipStream_t st) {
 void launch_pass(int dir, const Launch &L, hipStream_t st) {
   switch (L.ns) {
 #define CASE(N) case N: launch_pass_t<N>(dir, L, st); break;
+    SG_NS_LIST(CASE)
+#undef CASE
+    default: break;
+  }
+}
+void launch_radn(const Launch &L, hipStream_t st) {
+  switch (L.ns) {
+#define CASE(N) case N: launch_radn_t<N>(L, st); break;
     SG_NS_LIST(CASE)
 #undef CASE
     default: break;

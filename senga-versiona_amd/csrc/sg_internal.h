@@ -35,6 +35,7 @@ struct Params {
   int mixavg;
   int mixw, mixp;              // molar-mass / pressure diffusion (mixture-averaged transport only)
   int soret, dufour;           // thermal diffusion and its heat flux, for the species in ltmask
+  int radn;                    // optically thin radiation
   uint32_t ltmask;             // light species (molar mass <= WMLTDR + 1e-3) when soret or dufour
   double alamda, rlamda, prantl, tdifgb, pdifgb;
 };
@@ -139,6 +140,7 @@ struct Launch {
   const Tables *tab;
   int ns, precision, bndx, inflow;
   int ext;          // a molecular-transport option on: the passes with those terms
+  int radn;         // some member has radiation on: the radiation kernel after the x pass
   int pass_nw[3];   // pass tile per direction: 0 wide (rows of 32), 1 narrow (rows of 16)
   int mechfix;      // the chemistry is the compiled mechanism (MechH2): unrolled step loop
 };
@@ -153,6 +155,8 @@ void launch_prim(const Launch &L, hipStream_t st);
 void launch_pass(int dir, const Launch &L, hipStream_t st);
 // NSCBC face corrections (bounded x only), before launch_final
 void launch_nscbc(const Launch &L, hipStream_t st);
+// optically thin radiation, added to the energy accumulator after the x pass
+void launch_radn(const Launch &L, hipStream_t st);
 // substage irk (1..5): LINCOM for irk < 5, FINCOM for irk = 5, with each
 // member's coefficients; mode 2 (irk = 0): write the RHS to out instead
 void launch_final(const Launch &L, int irk, double *out, hipStream_t st);

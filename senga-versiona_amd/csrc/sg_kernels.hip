@@ -1004,6 +1004,38 @@ __global__ void __launch_bounds__(BLK) k_tdr(const Member *__restrict__ mem, con
   }
 }
 
+// Optically thin radiation (src/radcal.f:36-111, called at src/rhscal.f:810):
+// the energy equation loses 4 sigma K_P (T^4 - T_ref^4), with the Planck mean
+// absorption coefficient K_P = sum over the radiating species r of a_r(T) p_r,
+// p_r = rho Y_r R_r T (the conserved rho Y_r, as RHSCAL holds it there).  Added
+// in FP64 to the energy accumulator the x pass has just written.
+template <int NS, class R>
+__global__ void __launch_bounds__(BLK) k_radn(const Member *__restrict__ mem, const uint32_t *__restrict__ map,
+                                              const sg_config *__restrict__ c) {
+  int lb;
+  const Member &M = member_at<L_POINT>(mem, map, blockIdx.x, lb);
+  const Params P = M.P;
+  if (!P.radn) return;
+  const Fields F = global_fields(M.F);
+  const long idx = (long)lb * BLK + threadIdx.x;
+  if (idx >= P.npts) return;
+  const long fs = P.fstride;
+  const double t = (double)((const R *)F.prim)[P_T * fs + idx];
+  double kp = 0.0;
+  for (int r = 0; r < c->nsprad; r++) {
+    const int nk = c->nkprad[r];
+    double pl = c->akprad[r][nk - 1];
+    for (int q = nk - 2; q >= 0; q--) pl = pl * t + c->akprad[r][q];
+    const int k = c->nsprid[r];
+    kp = kp + pl * (F.q[(5 + k) * fs + idx] * c->rgspec[k] * t);
+  }
+  const double t4 = t * t * t * t, tr = c->trefrn;
+  const double trfrth = tr * tr * tr * tr;
+  const double foursb = 4.0 * 5.67037321e-08;   // FOUR * STEFBO (src/com_senga2.h:221-222)
+  R *A = (R *)F.acc;
+  A[4 * fs + idx] += (R)(-(foursb * kp * (t4 - trfrth)));
+}
+
 // FP32 quotient and reciprocal from the hardware reciprocal (v_rcp_f32, about
 // 1 ulp; the quotient within 2 ulp) instead of the IEEE-exact FP32 division
 // sequence (~10 instructions each), for the FP32 transport and chemistry of
@@ -2494,6 +2526,12 @@ void launch_nscbc_t(const Launch &L, hipStream_t st) {
   else hipLaunchKernelGGL((k_nscbc<NS, double>), g, b, 0, st, L.mem, L.map[L_FACE2], L.c);
 }
 template <int NS>
+void launch_radn_t(const Launch &L, hipStream_t st) {
+  const dim3 g(L.nblk[L_POINT]), b(BLK);
+  if (L.precision) hipLaunchKernelGGL((k_radn<NS, float>), g, b, 0, st, L.mem, L.map[L_POINT], L.c);
+  else hipLaunchKernelGGL((k_radn<NS, double>), g, b, 0, st, L.mem, L.map[L_POINT], L.c);
+}
+template <int NS>
 void launch_final_t(const Launch &L, int irk, double *out, hipStream_t st) {
   const dim3 g(L.nblk[L_FINAL]), b(BLK);
   const uint32_t *map = L.map[L_FINAL];
@@ -2513,6 +2551,7 @@ template void launch_bountt_t<SG_NS>(const Launch &, hipStream_t);
 template void launch_prim_t<SG_NS>(const Launch &, hipStream_t);
 template void launch_pass_t<SG_NS>(int, const Launch &, hipStream_t);
 template void launch_nscbc_t<SG_NS>(const Launch &, hipStream_t);
+template void launch_radn_t<SG_NS>(const Launch &, hipStream_t);
 template void launch_final_t<SG_NS>(const Launch &, int, double *, hipStream_t);
 #endif
 

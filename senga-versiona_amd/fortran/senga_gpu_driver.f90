@@ -69,6 +69,10 @@ program senga_gpu_driver
     call read_diff(trim(rundir) // '/input/diff.dat', cfg, ierr)
     if (ierr /= 0) call fail('cannot read input/diff.dat')
   end if
+  if (ctl%nfradn == 1) then
+    call read_radn(trim(rundir) // '/input/radn.dat', cfg, ierr)
+    if (ierr /= 0) call fail('cannot read input/radn.dat')
+  end if
   call fill_config(ctl, cfg)
 
   if (cfgonly) then
@@ -84,7 +88,7 @@ contains
 
   subroutine run_case()
 
-  if (sg_abi_version() /= 3 .or. sg_config_size() /= c_sizeof(cfg)) call fail('library ABI mismatch')
+  if (sg_abi_version() /= 4 .or. sg_config_size() /= c_sizeof(cfg)) call fail('library ABI mismatch')
   ns = cfg%nspec
   npts = ctl%nx * ctl%ny * ctl%nz
   allocate(run(npts, 5 + ns))

@@ -14,11 +14,11 @@ module senga_gpu
 
   integer, parameter, public :: SG_MAX_SPEC = 18, SG_MAX_STEP = 128, SG_MAX_SSL = 10
   integer, parameter, public :: SG_MAX_BODY = 16, SG_MAX_FALL = 32
-  integer, parameter, public :: SG_NCOF = 7, SG_NTINT = 2, SG_NTRC = 4
+  integer, parameter, public :: SG_NCOF = 7, SG_NTINT = 2, SG_NTRC = 4, SG_NKPRAD = 6
   integer, parameter, public :: SG_OK = 0, SG_EINVAL = 1, SG_ENOMEM = 2, SG_EDEVICE = 3
   integer, parameter, public :: SG_ETEMPER = 4, SG_ETEMPIN = 5, SG_ESTATE = 6
   integer, parameter, public :: SG_BC_PERIODIC = 1, SG_BC_INFLOW3 = 13, SG_BC_OUTFLOW1 = 21
-  ! sg_config%physics bits: optional physics switched on in cont.dat (radiation refused)
+  ! sg_config%physics bits: optional physics switched on in cont.dat
   integer, parameter, public :: SG_PHYS_MIXW = 1, SG_PHYS_MIXP = 2, SG_PHYS_SORET = 4
   integer, parameter, public :: SG_PHYS_DUFOUR = 8, SG_PHYS_RADN = 16
 
@@ -81,6 +81,11 @@ module senga_gpu
     integer(c_int32_t) :: ncotdr, reserved2
     real(c_double) :: wmltdr
     real(c_double) :: tdrcco(SG_NTRC, SG_MAX_SPEC, SG_MAX_SPEC)
+    integer(c_int32_t) :: nsprad, reserved3
+    integer(c_int32_t) :: nsprid(SG_MAX_SPEC)
+    integer(c_int32_t) :: nkprad(SG_MAX_SPEC)
+    real(c_double) :: trefrn
+    real(c_double) :: akprad(SG_NKPRAD, SG_MAX_SPEC)
   end type sg_config_t
 
   public :: sg_abi_version, sg_config_size, sg_create, sg_destroy

@@ -41,7 +41,7 @@ module senga_io
     real(c_double) :: wmltdr, trefrn
   end type control_t
 
-  public :: read_cont, read_chem, read_diff, fill_config, zero_config
+  public :: read_cont, read_chem, read_diff, read_radn, fill_config, zero_config
   public :: read_dump, write_dump, dump_header
   public :: report_open, report_step, report_dump, stat_open, stat_step, write_out_res
 
@@ -510,6 +510,46 @@ contains
     ierr = 0
   end subroutine read_diff
 
+  ! Optically thin radiation data (src/radcin.f:76-126): the radiating species
+  ! (0-based indices in sg_config) and their Planck mean absorption
+  ! coefficient polynomials.
+  subroutine read_radn(path, cfg, ierr)
+    character(len=*), intent(in) :: path
+    type(sg_config_t), intent(inout) :: cfg
+    integer, intent(out) :: ierr
+    integer :: u, ns, i, n, jsp, nk
+    character(len=32) :: sym
+    open(newunit=u, file=path, status='old', form='formatted', iostat=ierr)
+    if (ierr /= 0) return
+    do i = 1, 6
+      read(u, *)
+    end do
+    read(u, '(I5)') ns
+    do i = 1, ns
+      read(u, '(I5,3X,A)') n, sym
+    end do
+    read(u, *)
+    read(u, *) cfg%nsprad
+    if (cfg%nsprad < 0 .or. cfg%nsprad > SG_MAX_SPEC) then
+      ierr = 1
+      close(u)
+      return
+    end if
+    do i = 1, cfg%nsprad
+      read(u, *) jsp, nk
+      if (nk < 1 .or. nk > SG_NKPRAD) then
+        ierr = 1
+        close(u)
+        return
+      end if
+      read(u, *) cfg%akprad(1:nk, i)
+      cfg%nsprid(i) = jsp - 1
+      cfg%nkprad(i) = nk
+    end do
+    close(u)
+    ierr = 0
+  end subroutine read_radn
+
   ! Grid, boundary conditions, default state and transport switch from cont.dat.
   subroutine fill_config(ctl, cfg)
     type(control_t), intent(in) :: ctl
@@ -521,7 +561,7 @@ contains
     cfg%mixavg = merge(1, 0, ctl%nfmavt == 1)
     ! optional physics that is switched on: the molecular-transport options
     ! act only with mixture-averaged transport (src/diffin.f:66,384-407).
-    ! sg_create refuses radiation.
+    ! sg_create refuses none of them.
     cfg%physics = 0
     if (ctl%nfmavt == 1) then
       if (ctl%nfmixw == 1) cfg%physics = ior(cfg%physics, SG_PHYS_MIXW)
@@ -532,6 +572,7 @@ contains
     if (ctl%nfradn == 1) cfg%physics = ior(cfg%physics, SG_PHYS_RADN)
     cfg%precision = 0
     cfg%wmltdr = ctl%wmltdr
+    cfg%trefrn = ctl%trefrn
     cfg%xgdlen = ctl%xgdlen
     cfg%ygdlen = ctl%ygdlen
     cfg%zgdlen = ctl%zgdlen

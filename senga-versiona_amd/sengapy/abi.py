@@ -18,6 +18,7 @@ MAX_STEP = 128
 MAX_SSL = 10
 MAX_BODY = 16
 MAX_FALL = 32
+NKPRAD = 6
 NCOF = 7
 NTINT = 2
 NTRC = 4
@@ -66,6 +67,8 @@ class Config(C.Structure):
         ("wilko1", (d * MAX_SPEC) * MAX_SPEC), ("wilko2", (d * MAX_SPEC) * MAX_SPEC),
         ("ncotdr", i32), ("reserved2", i32), ("wmltdr", d),
         ("tdrcco", ((d * NTRC) * MAX_SPEC) * MAX_SPEC),
+        ("nsprad", i32), ("reserved3", i32), ("nsprid", i32 * MAX_SPEC), ("nkprad", i32 * MAX_SPEC),
+        ("trefrn", d), ("akprad", (d * NKPRAD) * MAX_SPEC),
     ]
 
 
@@ -191,6 +194,16 @@ def config_from_case(case, precision=0) -> Config:
                     c.tdrcco[s][j][q] = dt.tdrcco[q, j, s]
         c.ncotdr = dt.ncotdr
     c.wmltdr = ctl.wmltdr
+    c.trefrn = ctl.trefrn
+    if ctl.nfradn == 1:
+        rd = case.radn
+        if rd is None:
+            raise ValueError("NFRADN = 1 needs the case's radn.dat")
+        c.nsprad = len(rd.nsprid)
+        for r, (k, nk) in enumerate(zip(rd.nsprid, rd.nkprad)):
+            c.nsprid[r], c.nkprad[r] = k, nk
+            for q in range(nk):
+                c.akprad[r][q] = rd.akprad[q, r]
     return c
 
 

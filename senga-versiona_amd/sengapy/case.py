@@ -47,10 +47,17 @@ class Case:
     dump: Dump
     chem_path: str
     diff_path: Optional[str]
+    radn_path: Optional[str] = None   # radn.dat, read when NFRADN = 1
 
     @property
     def nspec(self):
         return self.tables.nspec
+
+    @property
+    def radn(self):
+        """The radiation data (sengapy.radn) when radn.dat is given, else None."""
+        from .radn import read_radn
+        return read_radn(self.radn_path) if self.radn_path else None
 
     @property
     def shape(self):
@@ -64,6 +71,9 @@ class Case:
         _chem.write_chem(self.mech, os.path.join(rundir, "input", "chem.dat"))
         if self.diff_path:
             with open(self.diff_path) as f, open(os.path.join(rundir, "input", "diff.dat"), "w") as g:
+                g.write(f.read())
+        if self.radn_path:
+            with open(self.radn_path) as f, open(os.path.join(rundir, "input", "radn.dat"), "w") as g:
                 g.write(f.read())
         write_dump(os.path.join(rundir, "output", "dmpi0000000.dat"), self.dump)
 
@@ -286,7 +296,8 @@ def h2_nscbc(n=(128, 128, 128), length=(5.0e-3, 5.0e-3, 5.0e-3), seed=7, phi=0.7
     if falloff_variant:
         name = "h2_falloff" + name[2:]
     return Case(name, ctl, mech, tab, dif, Dump(st, 0.0, dt), chem_path,
-                os.path.join(DATA, "h2_9sp", "diff.dat") if mixavg else None)
+                os.path.join(DATA, "h2_9sp", "diff.dat") if mixavg else None,
+                os.path.join(DATA, "h2_9sp", "radn.dat"))
 
 
 def h2_kernel(n=(64, 64, 64), length=(4.0e-3, 4.0e-3, 4.0e-3), seed=11, phi=0.8, Tu=300.0,
@@ -316,7 +327,8 @@ def h2_kernel(n=(64, 64, 64), length=(4.0e-3, 4.0e-3, 4.0e-3), seed=11, phi=0.8,
     ctl.nfmavt = 1 if mixavg else 0
     return Case(f"h2_kernel_{nx}x{ny}x{nz}", ctl, mech, tab, dif, Dump(st, 0.0, dt),
                 os.path.join(DATA, "h2_9sp", "chem.dat"),
-                os.path.join(DATA, "h2_9sp", "diff.dat") if mixavg else None)
+                os.path.join(DATA, "h2_9sp", "diff.dat") if mixavg else None,
+                os.path.join(DATA, "h2_9sp", "radn.dat"))
 
 
 def h2_falloff_mechanism() -> _chem.Mechanism:

@@ -51,6 +51,16 @@ def build(name):
         c = C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=True)
         c.control.nfmixw = c.control.nfmixp = c.control.nfmsor = c.control.nfmduf = 1
         return c
+    if name == "h2_nscbc_radn_12x11x11":
+        # optically thin radiation from H2O (src/radcal.f:36-111)
+        c = C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=True)
+        c.control.nfradn = 1
+        return c
+    if name == "h2_per_radn_cle_11x12x11":
+        # radiation with constant-Lewis transport, periodic x
+        c = C.h2_nscbc(n=(11, 12, 11), length=(1.4e-3, 1.5e-3, 1.4e-3), mixavg=False, periodic_x=True)
+        c.control.nfradn = 1
+        return c
     if name == ADAPT:
         c = C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=True)
         c.control.nstpsw = 1
@@ -63,7 +73,8 @@ ADAPT = "h2_nscbc_mix_adapt_12x11x11"
 
 NAMES = ["periodic_1step_16x12x11", "h2_nscbc_mix_12x11x11", "h2_nscbc_cle_12x11x11", "h2_per_mix_11x12x11",
          "h2_falloff_nscbc_mix_12x11x11", "ch4_4step_16x12x11", "h2_nscbc_mixwp_12x11x11", "h2_per_mixwp_11x12x11",
-         "h2_nscbc_sordf_12x11x11", "h2_per_duf_11x12x11", "h2_nscbc_alltr_12x11x11"]
+         "h2_nscbc_sordf_12x11x11", "h2_per_duf_11x12x11", "h2_nscbc_alltr_12x11x11",
+         "h2_nscbc_radn_12x11x11", "h2_per_radn_cle_11x12x11"]
 OPS = {"ops_periodic_16x12x11": "periodic_1step_16x12x11", "ops_boundedx_12x11x11": "h2_nscbc_cle_12x11x11"}
 
 

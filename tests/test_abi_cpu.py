@@ -36,7 +36,7 @@ def test_exports_every_declared_symbol(lib):
 
 
 def test_abi_version_and_layout(lib):
-    assert lib.sg_abi_version() == 3
+    assert lib.sg_abi_version() == 4
     assert lib.sg_config_size() == C.sizeof(abi.Config)
 
 
@@ -59,26 +59,34 @@ def test_rejects_unsupported_configs(lib):
     assert lib.sg_create(None, C.byref(h)) == 1
 
 
-@pytest.mark.parametrize("switch,bit", [("nfradn", abi.PHYS_RADN)])
-def test_rejects_each_optional_physics_switch(lib, switch, bit, tmp_path):
-    """Radiation (cont.dat switch, src/contin.f:194-209) is not computed
-    here: a case that switches it on is refused by sg_create and sg_footprint
-    instead of run without it.  The switch reaches sg_config through both
-    readers."""
+def test_radiation_switch_and_data_checks(lib, tmp_path):
+    """Radiation (cont.dat NFRADN, src/contin.f:194-209; radn.dat,
+    src/radcin.f:76-126) is computed: the switch reaches sg_config through the
+    reader with the H2O absorption polynomial, sg_footprint accepts it, and
+    the radiation data is range-checked (species index, coefficient count,
+    radiating-species count) before it goes to the device."""
     c = CS.h2_nscbc(n=(12, 11, 11), length=(1e-3, 1e-3, 1e-3), mixavg=True)
-    setattr(c.control, switch, 1)
+    c.control.nfradn = 1
     p = tmp_path / "cont.dat"
     c.control.write(str(p))
-    r = cont.read_cont(str(p))
-    assert getattr(r, switch) == 1
+    assert cont.read_cont(str(p)).nfradn == 1
     cfg = abi.config_from_case(c)
-    assert cfg.physics == bit
-    h = C.c_void_p()
+    assert cfg.physics == abi.PHYS_RADN
+    assert cfg.nsprad == 1 and cfg.nsprid[0] == 2 and cfg.nkprad[0] == 6
+    assert cfg.akprad[0][0] == 6.869480e-04 and cfg.akprad[0][5] == -1.373456e-20
+    assert cfg.trefrn == c.control.trefrn
     out = C.c_size_t(0)
-    assert lib.sg_create(C.byref(cfg), C.byref(h)) == 1
-    assert lib.sg_footprint(C.byref(cfg), C.byref(out)) == 1
-    cfg.physics = 0
     assert lib.sg_footprint(C.byref(cfg), C.byref(out)) == 0
+    for field, value in (("nsprid", 9), ("nsprid", -1), ("nkprad", 0), ("nkprad", abi.NKPRAD + 1)):
+        bad = abi.Config.from_buffer_copy(cfg)
+        getattr(bad, field)[0] = value
+        assert lib.sg_footprint(C.byref(bad), C.byref(out)) == 1, (field, value)
+    bad = abi.Config.from_buffer_copy(cfg)
+    bad.nsprad = 10
+    assert lib.sg_footprint(C.byref(bad), C.byref(out)) == 1
+    c.radn_path = None
+    with pytest.raises(ValueError):
+        abi.config_from_case(c)
 
 
 def test_transport_options_need_mixture_averaging():
@@ -222,8 +230,8 @@ def test_serving_summary():
                                         ("nfmsor", abi.PHYS_SORET), ("nfmduf", abi.PHYS_DUFOUR)])
 def test_accepts_molecular_transport_options(lib, switch, bit, tmp_path):
     """The four molecular-transport options are computed: the switch reaches
-    sg_config through the reader, and sg_footprint accepts it (not with
-    radiation).  Soret and Dufour need the thermal diffusion ratio fits."""
+    sg_config through the reader, and sg_footprint accepts it.  Soret and
+    Dufour need the thermal diffusion ratio fits."""
     c = CS.h2_nscbc(n=(12, 11, 11), length=(1e-3, 1e-3, 1e-3), mixavg=True)
     setattr(c.control, switch, 1)
     p = tmp_path / "cont.dat"
@@ -233,8 +241,6 @@ def test_accepts_molecular_transport_options(lib, switch, bit, tmp_path):
     assert cfg.physics == bit
     out = C.c_size_t(0)
     assert lib.sg_footprint(C.byref(cfg), C.byref(out)) == 0
-    cfg.physics = bit | abi.PHYS_RADN
-    assert lib.sg_footprint(C.byref(cfg), C.byref(out)) == 1
     if bit in (abi.PHYS_SORET, abi.PHYS_DUFOUR):
         assert cfg.ncotdr == 4
         cfg.physics = bit

@@ -37,6 +37,7 @@ def _cases():
     from sengapy import case as C
     alltr = C.h2_nscbc(n=(16, 12, 12), length=(2e-3, 1.5e-3, 1.5e-3), mixavg=True, nsteps=2)
     alltr.control.nfmixw = alltr.control.nfmixp = alltr.control.nfmsor = alltr.control.nfmduf = 1
+    alltr.control.nfradn = 1
     return [("h2_mix", C.h2_nscbc(n=(16, 12, 12), length=(2e-3, 1.5e-3, 1.5e-3), mixavg=True, nsteps=2)),
             ("h2_cle", C.h2_nscbc(n=(16, 12, 12), length=(2e-3, 1.5e-3, 1.5e-3), mixavg=False, nsteps=2)),
             ("periodic", C.periodic_1step(n=(16, 12, 11), nsteps=2)),

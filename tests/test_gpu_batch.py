@@ -196,15 +196,17 @@ def test_batch_autotune_keeps_results():
 
 
 def test_batch_with_transport_options():
-    """Cases with the four molecular-transport options (the passes with those
-    terms and the thermal-diffusion kernel) batched: each member equals its
-    solo run bit for bit; a member without the options is refused."""
+    """Cases with the four molecular-transport options and radiation (the
+    passes with those terms, the thermal-diffusion and radiation kernels)
+    batched: each member equals its solo run bit for bit; a member without
+    the options is refused."""
     from sengapy import case as C
     from sengapy.abi import Batch, SgError, Solver
     cases = []
     for n, L in (((24, 16, 16), (3e-3, 2e-3, 2e-3)), ((32, 20, 16), (4e-3, 2.5e-3, 2e-3))):
         c = C.h2_nscbc(n=n, length=L, mixavg=True, nsteps=3)
         c.control.nfmixw = c.control.nfmixp = c.control.nfmsor = c.control.nfmduf = 1
+        c.control.nfradn = 1
         cases.append(c)
     print("batch with transport options", [c.shape for c in cases], flush=True)
 

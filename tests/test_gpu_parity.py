@@ -132,12 +132,13 @@ def _gpu_after(c, steps, precision):
     return run, T
 
 
-@pytest.mark.parametrize("options", ["mixwp", "sordf", "all"])
+@pytest.mark.parametrize("options", ["mixwp", "sordf", "radn", "all"])
 @pytest.mark.parametrize("periodic", [False, True])
 def test_transport_terms_against_oracle(periodic, options):
     """The molecular-transport options (cont.dat NFMIXW, NFMIXP, NFMSOR,
-    NFMDUF): molar-mass and pressure diffusion, Soret and Dufour, and all
-    four, through the passes with those terms, against the oracle (itself
+    NFMDUF): molar-mass and pressure diffusion, Soret and Dufour, optically
+    thin radiation (NFRADN), and all five, through the passes with those
+    terms and the radiation kernel, against the oracle (itself
     bit-exact against the reference's fixtures) on larger grids than the
     fixtures, after 6 steps: FP64 within the bar, the mixed form within
     1e-5.  A duct with NSCBC x faces and a triply periodic box."""
@@ -148,6 +149,8 @@ def test_transport_terms_against_oracle(periodic, options):
         c.control.nfmixw = c.control.nfmixp = 1
     if options in ("sordf", "all"):
         c.control.nfmsor = c.control.nfmduf = 1
+    if options in ("radn", "all"):
+        c.control.nfradn = 1
     print("case", c.name, options, "6 steps", flush=True)
     run_o, T_o = _oracle_after(c, 6)
     run, T = _gpu_after(c, 6, 0)

@@ -76,3 +76,31 @@ def test_transport_fixture_exercises_each_term(name, flag, oracle_built):
     rhs = o.rhs(1)
     assert np.array_equal(rhs[:4], fx["rhs"][:4])
     assert not np.array_equal(rhs[4:], fx["rhs"][4:])
+
+
+@pytest.mark.parametrize("name", ["h2_nscbc_radn_12x11x11", "h2_per_radn_cle_11x12x11"])
+def test_radiation_fixture_exercises_the_loss(name, oracle_built, tmp_path):
+    """The radiation fixtures pin the loss term: without it only the energy
+    right-hand side changes, by far more than the parity tolerance; with the
+    H2O absorption polynomial's constant term doubled in radn.dat (a planted
+    mistake in the data the oracle reads) the energy no longer matches."""
+    c, fx = golden_cases.load(name)
+    ref = fx["rhs"]
+    c.control.nfradn = 0
+    o = Oracle(c)
+    o.set_timestep(c.dump.tstep)
+    o.set_state(c.dump.state, 0.0)
+    rhs = o.rhs(1)
+    assert np.array_equal(np.delete(rhs, 4, axis=0), np.delete(ref, 4, axis=0))
+    assert np.abs(rhs[4] - ref[4]).max() / np.abs(ref[4]).max() > 1e-8
+    c.control.nfradn = 1
+    txt = open(c.radn_path).read().replace("6.869480D-04", "1.373896D-03")
+    p = tmp_path / "radn.dat"
+    p.write_text(txt)
+    c.radn_path = str(p)
+    o = Oracle(c)
+    o.set_timestep(c.dump.tstep)
+    o.set_state(c.dump.state, 0.0)
+    rhs = o.rhs(1)
+    assert np.array_equal(rhs[:4], ref[:4])
+    assert not np.array_equal(rhs[4], ref[4])
