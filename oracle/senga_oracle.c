@@ -10,8 +10,9 @@
  *
  * Supported configurations: y and z periodic; x periodic, or NSBCI3 inflow
  * at x-left with NSBCO1 outflow at x-right (src/bounds.f:610-682,1296-1458).
- * Constant-Lewis or basic mixture-averaged transport (Soret, Dufour, molar
- * mass and pressure diffusion terms off).  Radiation off.
+ * Constant-Lewis or mixture-averaged transport, the latter with any of the
+ * molar-mass diffusion, pressure diffusion, Soret and Dufour terms; optically
+ * thin radiation with either.
  */
 #include "senga_oracle.h"
 
