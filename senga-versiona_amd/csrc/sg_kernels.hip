@@ -295,6 +295,14 @@ __device__ __forceinline__ float fma_u(float a, float b, float c) {
   asm("v_fma_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
   return r;
 }
+// A uniform FP64 value into vector registers with one 64-bit move (the
+// compiler otherwise copies the two halves with two moves), for the leading
+// coefficient that starts each Horner chain.
+__device__ __forceinline__ double vmov_u(double c) {
+  double r;
+  asm("v_mov_b64 %0, %1" : "=v"(r) : "s"(c));
+  return r;
+}
 
 // The temperature interval it when every active lane of the wave holds the
 // same one, else -1; wave-uniform (from ballots), so the polynomials below
@@ -314,12 +322,12 @@ __device__ __forceinline__ double horner(const double (&a)[SG_NTINT][SG_NCPMAX],
   static_assert(SG_NTINT == 2, "two temperature intervals");
   const int u = wave_interval(it);
   if (u >= 0) {
-    double f = a[u][SG_NCPMAX - 1];
+    double f = vmov_u(a[u][SG_NCPMAX - 1]);
 #pragma unroll
     for (int m = SG_NCPMAX - 2; m >= 0; m--) f = fma_u(f, x, a[u][m]);
     return f;
   }
-  double f0 = a[0][SG_NCPMAX - 1], f1 = a[1][SG_NCPMAX - 1];
+  double f0 = vmov_u(a[0][SG_NCPMAX - 1]), f1 = vmov_u(a[1][SG_NCPMAX - 1]);
 #pragma unroll
   for (int m = SG_NCPMAX - 2; m >= 0; m--) {
     f0 = fma_u(f0, x, a[0][m]);
@@ -333,7 +341,7 @@ __device__ __forceinline__ double horner(const double (&a)[SG_NTINT][SG_NCPMAX],
 // coefficients instead costs register copies and selects for every one).
 __device__ __forceinline__ double horner_c(const double (&a)[SG_NTINT][SG_NCPMAX], const double (&c0)[SG_NTINT],
                                            int it, double x) {
-  double f0 = a[0][SG_NCPMAX - 1], f1 = a[1][SG_NCPMAX - 1];
+  double f0 = vmov_u(a[0][SG_NCPMAX - 1]), f1 = vmov_u(a[1][SG_NCPMAX - 1]);
 #pragma unroll
   for (int m = SG_NCPMAX - 2; m >= 0; m--) {
     f0 = fma_u(f0, x, a[0][m]);
@@ -346,7 +354,7 @@ __device__ __forceinline__ double horner_c(const double (&a)[SG_NTINT][SG_NCPMAX
 // Gibbs function gbh(it)/T - gbs(it) ln T - poly(it)(T), both intervals formed,
 // one chosen (ovt = 1/T).
 __device__ __forceinline__ double gibbs_of(const Tables *tb, int s, int it, double t, double ovt, double lt) {
-  double f0 = tb->gb[s][0][SG_NCPMAX - 1], f1 = tb->gb[s][1][SG_NCPMAX - 1];
+  double f0 = vmov_u(tb->gb[s][0][SG_NCPMAX - 1]), f1 = vmov_u(tb->gb[s][1][SG_NCPMAX - 1]);
 #pragma unroll
   for (int m = SG_NCPMAX - 2; m >= 0; m--) {
     f0 = fma_u(f0, t, tb->gb[s][0][m]);
@@ -414,7 +422,7 @@ __device__ __forceinline__ void batch_recip(double (&a)[N]) {
 
 // Transport fits, zero-padded to SG_NTRC terms.
 __device__ __forceinline__ double trfit(const double *a, double x) {
-  double f = a[SG_NTRC - 1];
+  double f = vmov_u(a[SG_NTRC - 1]);
 #pragma unroll
   for (int m = SG_NTRC - 2; m >= 0; m--) f = fma_u(f, x, a[m]);
   return f;
@@ -764,13 +772,20 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
     transp = fornow;
   } else {
     transp = log(t / P.tdifgb);
-    double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+    // the divisions X_k / lambda_k share one (batch_recip; a few ulp from the
+    // reference's quotients)
+    double c1 = 0.0, c2 = 0.0, c3 = 0.0, ictr[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      double ctr = exp_t(trfit(c->condco[s], transp), xt);
+      const double ctr = exp_t(trfit(c->condco[s], transp), xt);
+      ictr[s] = ctr;
+      c1 = c1 + (ry[s] * c->ovwmol[s]) * ctr;
+    }
+    batch_recip<NS>(ictr);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
       double fornow = ry[s] * c->ovwmol[s];
-      c1 = c1 + fornow * ctr;
-      c2 = c2 + fornow / ctr;
+      c2 = c2 + fornow * ictr[s];
       c3 = c3 + fornow;
     }
     c3 = 1.0 / c3;
@@ -870,30 +885,27 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
     for (int s = 0; s < NS; s++) yw2p[s] = yw2[s] * ppd;
     if (tb->diffsym) {
-      // D_ij = D_ji: one fit per unordered pair, its term added to both
-      // sums.  Visiting pairs (a, b >= a) row by row delivers the terms of
-      // every sum in increasing species order, the reference's order.
-      double cs[NS], self[NS];
+      // D_ij = D_ji: one fit per unordered pair a < b, its term added to both
+      // sums.  Visiting the pairs row by row delivers the terms of every sum
+      // in increasing species order, the reference's order.  The reference
+      // sums over every j and subtracts the j = k term; the sums here skip
+      // it (no fit and exp for the 9 self pairs; the same sum up to the
+      // rounding of that addition and subtraction).
+      double cs[NS];
 #pragma unroll
       for (int s = 0; s < NS; s++) cs[s] = 0.0;
 #pragma unroll
       for (int a = 0; a < NS; a++) {
 #pragma unroll
-        for (int b = a; b < NS; b++) {
+        for (int b = a + 1; b < NS; b++) {
           const double ictd = exp_t(-trfit(c->diffco[a][b], transp), xt);
-          const double tab = yw2p[b] * ictd;
-          cs[a] = cs[a] + tab;
-          if (b == a) {
-            self[a] = tab;
-          } else {
-            const double tba = yw2p[a] * ictd;
-            cs[b] = cs[b] + tba;
-          }
+          cs[a] = cs[a] + yw2p[b] * ictd;
+          cs[b] = cs[b] + yw2p[a] * ictd;
         }
       }
       double c2[NS];
 #pragma unroll
-      for (int s = 0; s < NS; s++) c2[s] = (cs[s] - self[s]) * wmomix;
+      for (int s = 0; s < NS; s++) c2[s] = cs[s] * wmomix;
       batch_recip<NS>(c2);
 #pragma unroll
       for (int s = 0; s < NS; s++) fld(pr + prim_rd(P, s), bo) = d * (c1t - yt[s]) * c2[s];
@@ -915,19 +927,20 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     SG_PHASE();
     // Wilke viscosity (src/rhscal.f:3854-3891).  sqrt(mu_s / mu_j) is formed
-    // as sqrt(mu_s) / sqrt(mu_j), with sqrt(mu_s) = exp(fit_s / 2) and its
-    // reciprocal exp(-fit_s / 2) (two exps instead of exp, square root and
-    // division), mu_s = sqrt(mu_s)^2; the divisions mu_s / sum_s share one
-    // (batch_recip).
+    // as sqrt(mu_s) / sqrt(mu_j), with sqrt(mu_s) = exp(fit_s / 2) (one exp
+    // instead of exp and square root) and mu_s = sqrt(mu_s)^2; the
+    // reciprocals 1/sqrt(mu_j) share one division, and so do the divisions
+    // mu_s / sum_s (batch_recip).
     double mus[NS], sq[NS], rsq[NS], yw[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       const double hf = 0.5 * trfit(c->viscco[s], transp);
       sq[s] = exp_t(hf, xt);
-      rsq[s] = exp_t(-hf, xt);
+      rsq[s] = sq[s];
       mus[s] = sq[s] * sq[s];
       yw[s] = y[s] * c->ovwmol[s];
     }
+    batch_recip<NS>(rsq);
     double c1 = 0.0, wc2[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
