@@ -1374,27 +1374,27 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       return f;
     };
     if (tb->diffsym) {
-      float cs[NS], self[NS];
+      // as in the FP64 kernel: one fit per unordered pair a < b, the pressure
+      // ratio folded into the pair weights, no self pairs
+      float cs[NS], yw2p[NS];
 #pragma unroll
-      for (int s = 0; s < NS; s++) cs[s] = 0.0f;
+      for (int s = 0; s < NS; s++) {
+        cs[s] = 0.0f;
+        yw2p[s] = yw2[s] * ppd;
+      }
 #pragma unroll
       for (int a = 0; a < NS; a++) {
 #pragma unroll
-        for (int b = a; b < NS; b++) {
-          const float ictd = __expf(-fit(tb->difff[a][b])) * ppd;
-          const float tab = yw2[b] * ictd;
-          cs[a] = cs[a] + tab;
-          if (b == a) {
-            self[a] = tab;
-          } else {
-            cs[b] = cs[b] + yw2[a] * ictd;
-          }
+        for (int b = a + 1; b < NS; b++) {
+          const float ictd = __expf(-fit(tb->difff[a][b]));
+          cs[a] = cs[a] + yw2p[b] * ictd;
+          cs[b] = cs[b] + yw2p[a] * ictd;
         }
       }
 #pragma unroll
       for (int s = 0; s < NS; s++) {
         float c1 = c1t - yt[s];
-        float c2 = (cs[s] - self[s]) * wmomix;
+        float c2 = cs[s] * wmomix;
         fld(pr + prim_rd(P, s), b4) = qf(df * c1, c2);
       }
     } else {
