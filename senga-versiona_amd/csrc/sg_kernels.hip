@@ -866,25 +866,20 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
     for (int s = 0; s < NS; s++) fld(pr + prim_rd(P, s), bo) = transp * c->olewis[s];
     fld(pr + P_MU * fs, bo) = transp * P.prantl;
   } else {
-    // binary diffusion (src/rhscal.f:1311-1346)
-    double yt[NS];
-#pragma unroll
-    for (int s = 0; s < NS; s++) yt[s] = y[s] + 1.0e-12;
+    // binary diffusion (src/rhscal.f:1311-1346), with Y_j + tol formed again
+    // where it is used rather than held through the pair loop
     double c1t = 0.0;
 #pragma unroll
-    for (int js = 0; js < NS; js++) c1t = c1t + yt[js];
+    for (int js = 0; js < NS; js++) c1t = c1t + (y[js] + 1.0e-12);
     // The reference's term (Y_j + tol) W_j^-1 / CTRANS, CTRANS = exp(f) p_g / p,
     // is formed as ((Y_j + tol) W_j^-1 (p / p_g)) * exp(-f): no division or
     // pressure product per pair (a few ulp from the reference's operation
     // order), and the species' final divisions share one (batch_recip).
     const double ppd = pres / P.pdifgb;
-    double yw2[NS];
-#pragma unroll
-    for (int s = 0; s < NS; s++) yw2[s] = yt[s] * c->ovwmol[s];
-    double yw2p[NS];
-#pragma unroll
-    for (int s = 0; s < NS; s++) yw2p[s] = yw2[s] * ppd;
     if (tb->diffsym) {
+      double yw2p[NS];
+#pragma unroll
+      for (int s = 0; s < NS; s++) yw2p[s] = ((y[s] + 1.0e-12) * c->ovwmol[s]) * ppd;
       // D_ij = D_ji: one fit per unordered pair a < b, its term added to both
       // sums.  Visiting the pairs row by row delivers the terms of every sum
       // in increasing species order, the reference's order.  The reference
@@ -908,8 +903,14 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       for (int s = 0; s < NS; s++) c2[s] = cs[s] * wmomix;
       batch_recip<NS>(c2);
 #pragma unroll
-      for (int s = 0; s < NS; s++) fld(pr + prim_rd(P, s), bo) = d * (c1t - yt[s]) * c2[s];
+      for (int s = 0; s < NS; s++) fld(pr + prim_rd(P, s), bo) = d * (c1t - (y[s] + 1.0e-12)) * c2[s];
     } else {
+      double yt[NS], yw2[NS];
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        yt[s] = y[s] + 1.0e-12;
+        yw2[s] = yt[s] * c->ovwmol[s];
+      }
 #pragma unroll
       for (int s = 0; s < NS; s++) {
         double c2 = 0.0, self = 0.0;
