@@ -217,6 +217,70 @@ static int timed(sg_handle *h, int kid, Fn fn) {
 }
 
 // Stencil coefficients (src/dfinit.f:61-477), flags and ranges (src/bcinit.f).
+// Whether the mixed form's FP32 rates of the mechanism stay finite and keep
+// every term FP64 resolves at temperature T (src/chrate.f:177-725; the Gibbs
+// functions as the point kernels form them).  FP32 holds exp(x) for x in
+// about (-87, 88).  Rate coefficients that underflow to zero are harmless
+// where they only multiply (the FP64 term is as negligible), so the bounds
+// are: every forward and reverse exponent below 70 (room for the products
+// with concentrations); for a falloff step, its forward exponent above -85
+// and its low-pressure exponent in (-80, 70) (the reduced pressure divides
+// by the one and the corrected coefficient carries the other into the
+// reverse rate), their difference within 80; the Troe and SRI temperature
+// exponentials within +-70.
+static bool fp32_rates_fit(const sg_config &c, const Tables &t, double T) {
+  const double lt = log(T), ovt = 1.0 / T;
+  double g[SG_MAX_SPEC];
+  for (int s = 0; s < c.nspec; s++) {
+    int it = 0;
+    while (T > c.tinthi[s][it] && it < c.ntint[s] - 1) it++;
+    double f = t.gb[s][it][SG_NCPMAX - 1];
+    for (int m = SG_NCPMAX - 2; m >= 0; m--) f = f * T + t.gb[s][it][m];
+    g[s] = t.gbh[s][it] * ovt - t.gbs[s][it] * lt - f;
+  }
+  for (int st = 0; st < c.nstep; st++) {
+    const double ef = c.rparam[st][0] + c.rparam[st][1] * lt - c.rparam[st][2] * ovt;
+    if (!(ef < 70.0)) return false;
+    auto fall = [&](const double *r) {
+      const double e4 = r[0] + r[1] * lt - r[2] * ovt;
+      return ef > -85.0 && e4 > -80.0 && e4 < 70.0 && fabs(e4 - ef) < 80.0;
+    };
+    if (c.mllist[st] && !fall(c.rclind[c.mllist[st] - 1])) return false;
+    if (c.mtlist[st]) {
+      const double *r = c.rctroe[c.mtlist[st] - 1];
+      if (!fall(r) || !(fabs(T * r[4]) < 70.0) || !(fabs(r[5] * ovt) < 70.0) || !(fabs(T * r[6]) < 70.0))
+        return false;
+    }
+    if (c.mslist[st]) {
+      const double *r = c.rcsrif[c.mslist[st] - 1];
+      if (!fall(r) || !(fabs(r[4] * ovt) < 70.0) || !(fabs(T * r[5]) < 70.0)) return false;
+    }
+    if (c.mglist[st]) {
+      double e = ef;
+      for (int q = 0; q < c.nsslen[st]; q++) e += c.diffmu[st][q] * g[c.nsspec[st][q]];
+      if (!(e < 70.0)) return false;
+    }
+  }
+  return true;
+}
+
+// The temperature range around 1000 K in which fp32_rates_fit holds, in
+// steps of 1 K between 20 K and 10000 K (an empty range when it fails at
+// 1000 K: every point then forms its rates in FP64).
+static void fp32_rate_range(const sg_config &c, const Tables &t, double *lo, double *hi) {
+  if (!fp32_rates_fit(c, t, 1000.0)) {
+    *lo = 1.0;
+    *hi = 0.0;
+    return;
+  }
+  double T = 1000.0;
+  while (T > 20.0 && fp32_rates_fit(c, t, T - 1.0)) T -= 1.0;
+  *lo = T;
+  T = 1000.0;
+  while (T < 10000.0 && fp32_rates_fit(c, t, T + 1.0)) T += 1.0;
+  *hi = T;
+}
+
 static int setup_params(sg_handle *h) {
   const sg_config &c = h->cfg;
   Params &P = h->P;
@@ -653,6 +717,7 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   }
   h->mechfix = mech_compiled(h->cfg, h->tab);
   int rc = setup_params(h);
+  fp32_rate_range(h->cfg, h->tab, &h->P.chlo, &h->P.chhi);
   if (!rc && h->P.fstride * (long)sizeof(double) >= (1L << 32)) rc = SG_EINVAL;  // 32-bit field offsets
   if (rc) { delete h; return rc; }
   for (int r = 0; r < SG_NROW; r++) {

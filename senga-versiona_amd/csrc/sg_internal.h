@@ -38,6 +38,9 @@ struct Params {
   int radn;                    // optically thin radiation
   uint32_t ltmask;             // light species (molar mass <= WMLTDR + 1e-3) when soret or dufour
   double alamda, rlamda, prantl, tdifgb, pdifgb;
+  // mixed form: the temperatures at which every rate exponent of the
+  // mechanism fits FP32's exponential (outside, the rates are formed in FP64)
+  double chlo, chhi;
 };
 
 // Thermodynamic tables derived from sg_config when a handle is created.  The

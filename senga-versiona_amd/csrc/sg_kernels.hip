@@ -566,11 +566,11 @@ __device__ __forceinline__ void for_steps(const M &m, Fn f) {
 
 // One step in FP64: rate coefficients with falloff, forward and reverse
 // products, added to rate[] (src/chrate.f:177-747).  cyv(s) and gbv(s) give
-// the point's molar concentration and Gibbs function of species s.
-template <int NS, class M, class CY, class GB>
+// the point's molar concentration and Gibbs function of species s, ex(x) the
+// exponential.
+template <int NS, class M, class CY, class GB, class EX>
 __device__ __forceinline__ void chem_step64(const M &m, int st, double t, double lt, double ovt,
-                                            const double *ry, CY cyv, GB gbv, double *rate,
-                                            const double *xt) {
+                                            const double *ry, CY cyv, GB gbv, double *rate, EX ex) {
   const double ysmall = 1.0e-30, ydenom = 1.0e-15;
   bool flthrd = true;
   double s3 = 0.0;
@@ -580,12 +580,12 @@ __device__ __forceinline__ void chem_step64(const M &m, int st, double t, double
     for (int s = 0; s < NS; s++) s3 = s3 + ry[s] * (m.ovw(s) * m.eff(mb - 1, s));
   }
   double s2 = m.rp(st, 0) + m.rp(st, 1) * lt - m.rp(st, 2) * ovt;
-  double s1 = exp_t(s2, xt);
+  double s1 = ex(s2);
   const int ml = m.ml(st);
   if (ml != 0) {
     const int j = ml - 1;
     double s4 = m.rl(j, 0) + m.rl(j, 1) * lt - m.rl(j, 2) * ovt;
-    s4 = exp_t(s4, xt);
+    s4 = ex(s4);
     double preduc = s3 * s4 / s1;
     double fornow = m.rl(j, 3) * preduc / (1.0 + preduc);  // rl[3] pre-rounded to REAL
     s1 = s1 * fornow;
@@ -597,10 +597,10 @@ __device__ __forceinline__ void chem_step64(const M &m, int st, double t, double
     const int j = mt - 1;
     double omalph = 1.0 - m.rt(j, 3);
     double s4 = m.rt(j, 0) + m.rt(j, 1) * lt - m.rt(j, 2) * ovt;
-    s4 = exp_t(s4, xt);
+    s4 = ex(s4);
     double preduc = s3 * s4 / s1;
     double trats1 = t * m.rt(j, 4), trats2 = -m.rt(j, 5) * ovt, trats3 = t * m.rt(j, 6);
-    double ftcent = omalph * exp_t(trats3, xt) + m.rt(j, 3) * exp_t(trats1, xt) + exp_t(trats2, xt);
+    double ftcent = omalph * ex(trats3) + m.rt(j, 3) * ex(trats1) + ex(trats2);
     ftcent = log10(ftcent);
     double cfactr = m.rt(j, 7) + m.rt(j, 8) * ftcent;
     double enfact = m.rt(j, 9) + m.rt(j, 10) * ftcent;
@@ -608,7 +608,7 @@ __device__ __forceinline__ void chem_step64(const M &m, int st, double t, double
     fornow = fornow / (enfact - m.rt(j, 11) * fornow);
     fornow = 1.0 + fornow * fornow;
     double fbroad = ftcent / fornow;
-    fbroad = exp_t(fbroad * m.clnten(), xt);
+    fbroad = ex(fbroad * m.clnten());
     fornow = fbroad * preduc / (1.0 + preduc);
     s1 = s1 * fornow;
     s2 = log(s1);
@@ -618,15 +618,15 @@ __device__ __forceinline__ void chem_step64(const M &m, int st, double t, double
   if (ms != 0) {
     const int j = ms - 1;
     double s4 = m.rs(j, 0) + m.rs(j, 1) * lt - m.rs(j, 2) * ovt;
-    s4 = exp_t(s4, xt);
+    s4 = ex(s4);
     double preduc = s3 * s4 / s1;
     double fornow = m.rs(j, 4) * ovt;
-    double fbroad = m.rs(j, 3) * exp_t(fornow, xt);
+    double fbroad = m.rs(j, 3) * ex(fornow);
     fornow = t * m.rs(j, 5);
-    fbroad = fbroad + exp_t(fornow, xt);
+    fbroad = fbroad + ex(fornow);
     fornow = log10(preduc);
     fornow = 1.0 / (1.0 + log10(fornow));
-    fbroad = m.rs(j, 6) * exp_t(fornow * log(fbroad), xt);
+    fbroad = m.rs(j, 6) * ex(fornow * log(fbroad));
     fornow = fbroad * preduc / (1.0 + preduc);
     s1 = s1 * fornow;
     s2 = log(s1);
@@ -640,13 +640,13 @@ __device__ __forceinline__ void chem_step64(const M &m, int st, double t, double
   for (int q = 0; q < m.nrc(st); q++) {
     double fornow = cyv(m.nrcp(st, q));
     fornow = fornow > ysmall ? fornow : ysmall;
-    fornow = exp_t(m.crs(st, q) * log(fornow), xt);
+    fornow = ex(m.crs(st, q) * log(fornow));
     fornow = fornow / (1.0 + ydenom * fornow);
     s1 = s1 * fornow;
   }
   if (m.mg(st) != 0) {
     for (int q = 0; q < m.nss(st); q++) s2 = s2 + m.dmu(st, q) * gbv(m.nssp(st, q));
-    s2 = exp_t(s2, xt);
+    s2 = ex(s2);
     for (int q = 0; q < m.nps(st); q++) {
       double fornow = cyv(m.npsp(st, q));
       fornow = fornow > 0.0 ? fornow : 0.0;
@@ -655,7 +655,7 @@ __device__ __forceinline__ void chem_step64(const M &m, int st, double t, double
     for (int q = 0; q < m.npc(st); q++) {
       double fornow = cyv(m.npcp(st, q));
       fornow = fornow > ysmall ? fornow : ysmall;
-      fornow = exp_t(m.cps(st, q) * log(fornow), xt);
+      fornow = ex(m.cps(st, q) * log(fornow));
       fornow = fornow / (1.0 + ydenom * fornow);
       s2 = s2 * fornow;
     }
@@ -824,7 +824,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       for (int s = 0; s < NS; s++) cy[s] = ry[s] * MT::ovw(s);
       for_steps(MT{}, [&](int st) {
         chem_step64<NS>(MT{}, st, t, lt, ovt, ry, [&](int s) { return cy[s]; }, [&](int s) { return gibbs[s]; },
-                        rate, xt);
+                        rate, [&](double x) { return exp_t(x, xt); });
       });
     } else {
       // Molar concentrations and Gibbs functions of this point in LDS, so
@@ -840,7 +840,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       const RtMech mr{c, tb};
       for_steps(mr, [&](int st) {
         chem_step64<NS>(mr, st, t, lt, ovt, ry, [&](int s) { return lcy[s][tl]; }, [&](int s) { return lgb[s][tl]; },
-                        rate, xt);
+                        rate, [&](double x) { return exp_t(x, xt); });
       });
     }
 #pragma unroll
@@ -972,6 +972,66 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k
       fc[F_RR * nf + fq] = s7 * ovd;
     }
   }
+}
+
+// Mixed form, after the point kernel: FP32 holds exp(x) only for x in about
+// (-87, 88), and outside the mechanism's range [P.chlo, P.chhi] (worked out
+// on the host) a rate coefficient could underflow to zero before a falloff
+// division, or overflow before its product with small concentrations, and
+// give NaN (or lose a reverse rate) where FP64 gives finite values.  The
+// points whose temperature lies outside that range get their rates formed
+// again here in FP64 (chem_step64 with the library exp, from the FP64 rho Y_k
+// and the stored FP32 temperature) and stored in FP32; a block with no such
+// point returns after reading its temperatures.
+template <int NS, bool FIX>
+__global__ void __launch_bounds__(BLK) k_rate64(const Member *__restrict__ mem, const uint32_t *__restrict__ map,
+                                                const sg_config *__restrict__ c, const Tables *__restrict__ tb) {
+  int lb;
+  const Member &M = member_at<L_POINT>(mem, map, blockIdx.x, lb);
+  const Params P = M.P;
+  const Fields F = global_fields(M.F);
+  const long idx = (long)lb * BLK + threadIdx.x;
+  const long fs = P.fstride;
+  const double t = idx < P.npts ? (double)((const float *)F.prim)[P_T * fs + idx] : 1000.0;
+  const bool out = idx < P.npts && !(t >= P.chlo && t <= P.chhi);
+  if (!__syncthreads_or(out)) return;
+  if (!out) return;   // no barrier below
+  double ry[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) ry[s] = F.q[(5 + s) * fs + idx];
+  const double lt = log(t), ovt = 1.0 / t;
+  double gibbs[NS], rate[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    gibbs[s] = gibbs_of(tb, s, locate(c, s, t), t, ovt, lt);
+    rate[s] = 0.0;
+  }
+  auto ex = [](double x) { return exp(x); };
+  if constexpr (FIX) {
+    using MT = CtMech<MechH2>;
+    for_steps(MT{}, [&](int st) {
+      chem_step64<NS>(MT{}, st, t, lt, ovt, ry, [&](int s) { return ry[s] * MT::ovw(s); },
+                      [&](int s) { return gibbs[s]; }, rate, ex);
+    });
+  } else {
+    // species lists index the point's values at run time: from LDS, one
+    // slot per thread (as in k_prim)
+    __shared__ double lcy[NS][BLK], lgb[NS][BLK];
+    const int tl = threadIdx.x;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      lcy[s][tl] = ry[s] * c->ovwmol[s];
+      lgb[s][tl] = gibbs[s];
+    }
+    const RtMech mr{c, tb};
+    for_steps(mr, [&](int st) {
+      chem_step64<NS>(mr, st, t, lt, ovt, ry, [&](int s) { return lcy[s][tl]; }, [&](int s) { return lgb[s][tl]; },
+                      rate, ex);
+    });
+  }
+  float *rt = (float *)F.rate;
+#pragma unroll
+  for (int s = 0; s < NS; s++) rt[s * fs + idx] = (float)rate[s];
 }
 
 // Soret / Dufour, after the point kernel: ln(T/TDIFGB), the thermal
@@ -2489,6 +2549,13 @@ static void launch_prim_k(const Launch &L, hipStream_t st) {
 template <int NS>
 void launch_prim_t(const Launch &L, hipStream_t st) {
   launch_prim_k<NS>(L, st);
+  if (L.precision) {   // mixed form: FP64 rates where FP32 cannot hold them
+    const dim3 g(L.nblk[L_POINT]), b(BLK);
+    bool fix = false;
+    if constexpr (NS == MechH2::nspec) fix = L.mechfix != 0;
+    if (fix) hipLaunchKernelGGL((k_rate64<NS, NS == MechH2::nspec>), g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
+    else hipLaunchKernelGGL((k_rate64<NS, false>), g, b, 0, st, L.mem, L.map[L_POINT], L.c, L.tab);
+  }
   if (L.ext) {   // Soret / Dufour quantities of the members that need them
     const dim3 g(L.nblk[L_POINT]), b(BLK);
     if (L.precision) hipLaunchKernelGGL((k_tdr<NS, float>), g, b, 0, st, L.mem, L.map[L_POINT], L.c);

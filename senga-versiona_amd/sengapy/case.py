@@ -421,6 +421,87 @@ def ch4_flame(n=(24, 24, 24), length=None, seed=3, phi=0.8, Tu=300.0, uprime=1.0
                 os.path.join(DATA, "ch4_4step", "chem.dat"), None)
 
 
+def ch4_16sp_mechanism() -> _chem.Mechanism:
+    return _chem.read_chem(os.path.join(DATA, "ch4_16sp", "chem.dat"))
+
+
+def ch4_16sp_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), seed=5, phi=0.8, Tu=300.0,
+                   uin=0.39, uprime=0.5, cfl=0.3, nsteps=10, periodic_x=False, transport=True,
+                   radiation=True) -> Case:
+    """Methane-air flame front with the reference's 16-species, 35-step
+    mechanism (data/ch4_16sp/chem.dat, src/casefiles/25step: third-body and
+    Lindemann steps), its mixture-averaged transport fits with thermal
+    diffusion ratios and its radiation data (four radiating species): an erf
+    front along x between unburnt CH4 + (2/phi)(O2 + 3.76 N2) at Tu and its
+    complete-combustion products at the adiabatic temperature, Gaussian bumps
+    of the intermediates in the front, seeded Passot-Pouquet turbulence, 1 atm.
+    An NSCBC duct (inflow at x-left with the unburnt state, outflow at
+    x-right), or triply periodic with two fronts.  transport: the four
+    molecular-transport options of the reference's own cont.dat for this case
+    (molar-mass and pressure diffusion, Soret, Dufour, WMLTDR = 5); radiation:
+    NFRADN = 1."""
+    nx, ny, nz = n
+    mech = ch4_16sp_mechanism()
+    tab = _chem.derive(mech)
+    data = os.path.join(DATA, "ch4_16sp")
+    dif = _diff.read_diff(os.path.join(data, "diff.dat"), mech.wmolar)
+    idx = {s: i for i, s in enumerate(mech.species)}
+    W = mech.wmolar
+    nO2 = 2.0 / phi
+
+    def massfrac(mol):
+        y = np.zeros(mech.nspec)
+        for sp, nm in mol.items():
+            y[idx[sp]] = nm * W[idx[sp]]
+        return y / y.sum()
+
+    Yu = massfrac({"CH4": 1.0, "O2": nO2, "N2": 3.76 * nO2})
+    Yb = massfrac({"CO2": 1.0, "H2O": 2.0, "O2": max(nO2 - 2.0, 0.0), "N2": 3.76 * nO2})
+    Tb = _ic.adiabatic_temperature(tab, Yu, Yb, Tu)
+    d = tuple(length[i] / (n[i] - 1) for i in range(3))
+    x = _ic.grid(nx, length[0])
+    shape = (nz, ny, nx)
+    delta = 0.3e-3
+    if periodic_x:
+        period = nx * d[0]
+        c = 0.5 * (np.vectorize(math.erf)((x - 0.3 * period) / delta)
+                   - np.vectorize(math.erf)((x - 0.7 * period) / delta))
+    else:
+        c = _ic.erf_profile(x, 0.5 * length[0], delta)
+    C = np.broadcast_to(c, shape)
+    T = Tu + (Tb - Tu) * C
+    Y = [Yu[i] * (1.0 - C) + Yb[i] * C for i in range(mech.nspec)]
+    bump = np.exp(-((C - 0.5) / 0.2) ** 2)
+    for sp, amp in (("H2", 1e-3), ("O", 1e-3), ("OH", 3e-3), ("H", 1e-4), ("HO2", 1e-4), ("H2O2", 2e-5),
+                    ("CO", 5e-3), ("CH2O", 1e-4), ("HCO", 1e-5), ("CH3", 5e-4), ("CH3O", 1e-5)):
+        Y[idx[sp]] = Y[idx[sp]] + amp * bump
+    Y[idx["N2"]] = Y[idx["N2"]] - (sum(Y) - 1.0)
+    u, v, w = _ic.solenoidal_field(shape, (d[2], d[1], d[0]), uprime, 4.0 * delta, seed,
+                                   spectrum="passot-pouquet")
+    if not periodic_x:
+        taper = np.sin(np.pi * x / length[0]) ** 2
+        u = u * taper; v = v * taper; w = w * taper
+        u = u + uin
+    st = _ic.conservative_state(tab, T, np.full(shape, 1.0e5), Y, u, v, w)
+    dt = _f32(cfl * min(d) / sound_speed_max(tab, st, T))
+    ctl = Control(length[0], length[1], length[2], nx, ny, nz, dt, 1, nsteps, 0, prin=1.0e5,
+                  trin=Tu, urin=uin if not periodic_x else 0.0, yrin=[float(y) for y in Yu])
+    per = (1, [0, 0, 0, 0], [0.0] * 4)
+    if periodic_x:
+        ctl.bc = [per] * 6
+    else:
+        ctl.bc = [(13, [1, 0, 0, 0], [uin, 0.0, 0.0, 0.0]),
+                  (21, [0, 0, 0, 0], [1.0e5, 0.287, 1.0e-3, 0.0]), per, per, per, per]
+    ctl.nfmavt = 1
+    if transport:
+        ctl.nfmixw = ctl.nfmixp = ctl.nfmsor = ctl.nfmduf = 1
+    ctl.wmltdr = 5.0
+    ctl.nfradn = 1 if radiation else 0
+    name = f"ch4_16sp_{'per' if periodic_x else 'nscbc'}_{nx}x{ny}x{nz}"
+    return Case(name, ctl, mech, tab, dif, Dump(st, 0.0, dt), os.path.join(data, "chem.dat"),
+                os.path.join(data, "diff.dat"), os.path.join(data, "radn.dat"))
+
+
 # ---------------------------------------------------------------------------
 # Case sets for batched and served runs.  Each spec is a small dict drawn from
 # a seeded generator; make_case(spec) builds the initial dump on demand.

@@ -61,6 +61,14 @@ def build(name):
         c = C.h2_nscbc(n=(11, 12, 11), length=(1.4e-3, 1.5e-3, 1.4e-3), mixavg=False, periodic_x=True)
         c.control.nfradn = 1
         return c
+    if name == "ch4_16sp_nscbc_12x11x11":
+        # the reference's 16-species, 35-step methane mechanism (src/casefiles/25step)
+        # with every option of that case's cont.dat: mixture-averaged transport,
+        # molar-mass and pressure diffusion, Soret, Dufour, and radiation from
+        # four species; NSCBC duct
+        return C.ch4_16sp_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3))
+    if name == "ch4_16sp_per_11x12x11":
+        return C.ch4_16sp_nscbc(n=(11, 12, 11), length=(1.4e-3, 1.5e-3, 1.4e-3), periodic_x=True)
     if name == ADAPT:
         c = C.h2_nscbc(n=(12, 11, 11), length=(1.5e-3, 1.4e-3, 1.4e-3), mixavg=True)
         c.control.nstpsw = 1
@@ -74,7 +82,8 @@ ADAPT = "h2_nscbc_mix_adapt_12x11x11"
 NAMES = ["periodic_1step_16x12x11", "h2_nscbc_mix_12x11x11", "h2_nscbc_cle_12x11x11", "h2_per_mix_11x12x11",
          "h2_falloff_nscbc_mix_12x11x11", "ch4_4step_16x12x11", "h2_nscbc_mixwp_12x11x11", "h2_per_mixwp_11x12x11",
          "h2_nscbc_sordf_12x11x11", "h2_per_duf_11x12x11", "h2_nscbc_alltr_12x11x11",
-         "h2_nscbc_radn_12x11x11", "h2_per_radn_cle_11x12x11"]
+         "h2_nscbc_radn_12x11x11", "h2_per_radn_cle_11x12x11", "ch4_16sp_nscbc_12x11x11",
+         "ch4_16sp_per_11x12x11"]
 OPS = {"ops_periodic_16x12x11": "periodic_1step_16x12x11", "ops_boundedx_12x11x11": "h2_nscbc_cle_12x11x11"}
 
 

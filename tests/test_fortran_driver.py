@@ -42,7 +42,8 @@ def _cases():
             ("h2_cle", C.h2_nscbc(n=(16, 12, 12), length=(2e-3, 1.5e-3, 1.5e-3), mixavg=False, nsteps=2)),
             ("periodic", C.periodic_1step(n=(16, 12, 11), nsteps=2)),
             # the molecular-transport switches and the thermal diffusion ratios
-            ("h2_alltr", alltr)]
+            ("h2_alltr", alltr),
+            ("ch4_16sp", C.ch4_16sp_nscbc(n=(16, 12, 12), length=(2e-3, 1.5e-3, 1.5e-3), nsteps=2))]
 
 
 def test_fortran_config_equals_python_config(tmp_path):

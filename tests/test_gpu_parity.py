@@ -70,7 +70,8 @@ def test_rhs_and_steps_match_reference(name):
 @pytest.mark.parametrize("kind,n,steps", [("periodic_1step", (32, 32, 32), 10),
                                           ("h2_nscbc_mix", (40, 24, 24), 4),
                                           ("h2_nscbc_cle", (24, 24, 24), 4),
-                                          ("h2_kernel", (24, 20, 16), 4)])
+                                          ("h2_kernel", (24, 20, 16), 4),
+                                          ("ch4_16sp", (32, 20, 16), 3)])
 def test_multistep_against_oracle(kind, n, steps):
     """Larger grids than the fixtures: GPU vs the oracle after several steps."""
     from pyoracle import Oracle
@@ -86,6 +87,11 @@ def test_multistep_against_oracle(kind, n, steps):
         spec = C.kernel_specs(1, seed=13)[0]
         spec["n"] = n
         c = C.make_case(spec)
+    elif kind == "ch4_16sp":
+        # the reference's 16-species, 35-step methane mechanism with every
+        # transport option and radiation: the NS = 16 kernels, the
+        # table-driven step loop, the passes with the extra transport terms
+        c = C.ch4_16sp_nscbc(n=n, length=(3.1e-3, 1.9e-3, 1.5e-3), nsteps=steps)
     else:
         c = C.h2_nscbc(n=n, length=(2.4e-3, 2.4e-3, 2.4e-3), mixavg=False, nsteps=steps)
     o = Oracle(c)
@@ -253,7 +259,8 @@ def test_adaptive_step_matches_reference():
     assert abs(o.timestep()[0] - tstep) <= 1e-7 * tstep
 
 
-@pytest.mark.parametrize("kind,n,steps", [("h2_nscbc_mix", (40, 24, 24), 20), ("periodic_1step", (32, 32, 32), 20)])
+@pytest.mark.parametrize("kind,n,steps", [("h2_nscbc_mix", (40, 24, 24), 20), ("periodic_1step", (32, 32, 32), 20),
+                                          ("ch4_16sp", (32, 20, 16), 10)])
 def test_mixed_precision_against_fp64(kind, n, steps):
     """Mixed mode (FP32 primitives, transport, chemistry, derivatives and RHS;
     FP64 registers and temperature iteration) against the FP64 path and
@@ -265,6 +272,8 @@ def test_mixed_precision_against_fp64(kind, n, steps):
     print("case", kind, n, steps, flush=True)
     if kind == "periodic_1step":
         c = C.periodic_1step(n=n, nsteps=steps)
+    elif kind == "ch4_16sp":
+        c = C.ch4_16sp_nscbc(n=n, length=(3.1e-3, 1.9e-3, 1.5e-3), nsteps=steps)
     else:
         c = C.h2_nscbc(n=n, length=(4e-3, 2.4e-3, 2.4e-3), mixavg=True, nsteps=steps)
     out = {}

@@ -63,7 +63,8 @@ def test_adaptive_step_bitexact(oracle_built):
 @pytest.mark.parametrize("name,flag", [("h2_nscbc_mixwp_12x11x11", "nfmixw"), ("h2_nscbc_mixwp_12x11x11", "nfmixp"),
                                        ("h2_nscbc_sordf_12x11x11", "nfmsor"), ("h2_nscbc_sordf_12x11x11", "nfmduf"),
                                        ("h2_per_duf_11x12x11", "nfmduf"), ("h2_nscbc_alltr_12x11x11", "nfmixw"),
-                                       ("h2_nscbc_alltr_12x11x11", "nfmsor")])
+                                       ("h2_nscbc_alltr_12x11x11", "nfmsor"), ("ch4_16sp_nscbc_12x11x11", "nfmduf"),
+                                       ("ch4_16sp_per_11x12x11", "nfmixp")])
 def test_transport_fixture_exercises_each_term(name, flag, oracle_built):
     """The molecular-transport fixtures pin each term they switch on: with
     that switch off the oracle no longer reproduces the reference's
@@ -78,7 +79,7 @@ def test_transport_fixture_exercises_each_term(name, flag, oracle_built):
     assert not np.array_equal(rhs[4:], fx["rhs"][4:])
 
 
-@pytest.mark.parametrize("name", ["h2_nscbc_radn_12x11x11", "h2_per_radn_cle_11x12x11"])
+@pytest.mark.parametrize("name", ["h2_nscbc_radn_12x11x11", "h2_per_radn_cle_11x12x11", "ch4_16sp_nscbc_12x11x11"])
 def test_radiation_fixture_exercises_the_loss(name, oracle_built, tmp_path):
     """The radiation fixtures pin the loss term: without it only the energy
     right-hand side changes, by far more than the parity tolerance; with the
