@@ -161,19 +161,23 @@ def test_fortran_chem_reader_rejects_out_of_range_indices(tmp_path, what):
     assert "cannot read input/chem.dat" in r.stdout
 
 
-def test_both_readers_read_the_reference_formatted_dump(tmp_path):
-    """A dump the reference itself wrote: src/casefiles/21step/dmpi0000000.dat
-    (formatted, 1001 x 1 x 1 points, 9 species; kept as the fixture
-    tests/golden/ref21step_dmpi0000000.dat).  The Python formatted reader and
-    the Fortran reader (the driver's --convert-dump mode: formatted read as
+@pytest.mark.parametrize("fname,nspec,etime,tstep", [
+    ("ref21step_dmpi0000000.dat", 9, 1.394146750614353e-2, 1.413965038003083e-8),
+    ("ref25step_dmpi0000000.dat", 16, 8.576785249268155e-3, 1.8955182896609245e-8)])
+def test_both_readers_read_the_reference_formatted_dump(tmp_path, fname, nspec, etime, tstep):
+    """Dumps the reference itself wrote: src/casefiles/21step/dmpi0000000.dat
+    and src/casefiles/25step/dmpi0000000.dat (formatted, 1001 x 1 x 1 points,
+    9 and 16 species; kept as the fixtures tests/golden/ref21step_... and
+    ref25step_dmpi0000000.dat).  The Python formatted reader and the Fortran
+    reader (the driver's --convert-dump mode: formatted read as
     src/indata.f:1449-1473, unformatted write as src/output.f:306-316) give the
-    same values bit for bit, and the state is a 1-D flame: sum_k rho Y_k =
+    same values bit for bit, and each state is a 1-D flame: sum_k rho Y_k =
     rho to the dump's precision, positive density."""
     from sengapy.dump import read_dump, read_dump_formatted
     exe = _driver()
-    src = os.path.join(HERE, "golden", "ref21step_dmpi0000000.dat")
+    src = os.path.join(HERE, "golden", fname)
     d = read_dump_formatted(src)
-    assert d.state.shape == (14, 1, 1, 1001)
+    assert d.state.shape == (5 + nspec, 1, 1, 1001)
     out = tmp_path / "dmpi.bin"
     r = subprocess.run([exe, "--convert-dump", src, str(out)], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -183,7 +187,7 @@ def test_both_readers_read_the_reference_formatted_dump(tmp_path):
     rho = d.state[0]
     assert rho.min() > 0
     assert np.max(np.abs(d.state[5:].sum(axis=0) / rho - 1.0)) < 1e-11
-    assert d.etime == pytest.approx(1.394146750614353e-2) and d.tstep == pytest.approx(1.413965038003083e-8)
+    assert d.etime == pytest.approx(etime) and d.tstep == pytest.approx(tstep)
 
 
 @pytest.mark.gpu

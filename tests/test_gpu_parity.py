@@ -7,6 +7,7 @@ multiply-adds, and ROCm's exp/log differ from glibc's by an ulp; single RHS
 evaluations are therefore compared at 1e-10 too (observed differences are far
 smaller), temperatures at 1e-12.
 """
+import os
 import sys
 
 import numpy as np
@@ -16,6 +17,7 @@ import golden_cases
 from conftest import assert_state_close, rel_err
 
 pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 TOL = 1e-10
 
@@ -163,6 +165,40 @@ def test_transport_terms_against_oracle(periodic, options):
     assert_state_close("fp64 vs oracle", run, T, run_o, T_o, TOL, 1e-12, tol_pw=PW_TOL64)
     run, T = _gpu_after(c, 6, 1)
     assert_state_close("mixed vs oracle", run, T, run_o, T_o, 1e-5, 1e-5)
+
+
+def test_reference_methane_flame_against_oracle():
+    """The reference's own converged 16-species methane flame
+    (src/casefiles/25step/dmpi0000000.dat, the fixture
+    ref25step_dmpi0000000.dat: a stoichiometric 1-D flame at 1 atm, 300 to
+    2189 K, written by the reference): its 64 points across the flame front
+    extruded over an 11 x 11 cross-section, with a small seeded transverse
+    velocity so that every direction carries derivatives, in an NSCBC duct
+    with every transport option of that case and radiation; 3 RK steps, FP64
+    within the bar and the mixed form within 1e-5 against the oracle."""
+    from sengapy import case as C
+    from sengapy.dump import read_dump_formatted
+    d = read_dump_formatted(os.path.join(HERE, "golden", "ref25step_dmpi0000000.dat"))
+    nx, ny, nz = 64, 11, 11
+    dx = 1.0e-2 / 1000
+    c = C.ch4_16sp_nscbc(n=(nx, ny, nz), length=((nx - 1) * dx, (ny - 1) * dx, (nz - 1) * dx), phi=1.0,
+                         nsteps=3)
+    prof = d.state[:, 0, 0, 192:192 + nx]
+    st = np.broadcast_to(prof[:, None, None, :], (prof.shape[0], nz, ny, nx)).copy()
+    rng = np.random.default_rng(25)
+    pert = 0.2 * rng.standard_normal((2, nz, ny, 1))
+    st[2] += st[0] * pert[0]
+    st[3] += st[0] * pert[1]
+    c.dump.state = st
+    T = C._case_temperature(c.tables, st)
+    c.dump.tstep = C._f32(0.3 * dx / C.sound_speed_max(c.tables, st, T))
+    c.control.tstep = c.dump.tstep
+    print("case reference methane flame", c.shape, "T", float(T.min()), float(T.max()), flush=True)
+    run_o, T_o = _oracle_after(c, 3)
+    run, Tg = _gpu_after(c, 3, 0)
+    assert_state_close("fp64 vs oracle", run, Tg, run_o, T_o, TOL, 1e-12, tol_pw=PW_TOL64)
+    run, Tg = _gpu_after(c, 3, 1)
+    assert_state_close("mixed vs oracle", run, Tg, run_o, T_o, 1e-5, 1e-5)
 
 
 def test_h2_workload_full_size_one_step():
