@@ -100,10 +100,16 @@ struct Fields {
 // mass), written when molar-mass or pressure diffusion is on; P_LNT:
 // ln(T/TDIFGB), written when Soret or Dufour is on
 enum PrimSlot { P_U = 0, P_V, P_W, P_E, P_T, P_P, P_LAM, P_MU, P_HMIX, P_LNW, P_LNP, P_OVW, P_LNT, P_NSCAL };
-// face slots: scalars then per-species
-enum FaceSlot { F_CP = 0, F_RR, F_DPDX, F_DPDY, F_DPDZ, F_DRDX, F_NSCAL };
-// per species: F_DYDX at F_NSCAL + s; then the NSCBC corrections of the
-// 5 + ns right-hand sides at F_NSCAL + ns + v (written by k_nscbc)
+// face slots: scalars then per-species.  Besides the NSCBC data proper (cp,
+// mixture gas constant, pressure and density gradients) the passes copy
+// into them, at the x-face points, every value the NSCBC kernel reads
+// (velocity, T, E, stored p, rho, the velocity gradients it uses), so that
+// it reads contiguous face arrays instead of one cache line per value.
+enum FaceSlot { F_CP = 0, F_RR, F_DPDX, F_DPDY, F_DPDZ, F_DRDX, F_U, F_V, F_W, F_T, F_E, F_P, F_RHO,
+                F_DUX, F_DVX, F_DWX, F_DUY, F_DVY, F_DUZ, F_DWZ, F_NSCAL };
+// per species: dY/dx at F_NSCAL + s, h at face_h, Y at face_y, the
+// chemical rate at face_rate; then the NSCBC corrections of the 5 + ns
+// right-hand sides at face_corr + v (written by k_nscbc)
 
 inline __host__ __device__ long prim_y(const Params &P, int s) { return (long)(P_NSCAL + s) * P.fstride; }
 inline __host__ __device__ long prim_rd(const Params &P, int s) { return (long)(P_NSCAL + P.ns + s) * P.fstride; }
@@ -113,8 +119,11 @@ inline __host__ __device__ long prim_h(const Params &P, int s) { return (long)(P
 inline __host__ __device__ long prim_tdr(const Params &P, int s) { return (long)(P_NSCAL + 3 * P.ns + s) * P.fstride; }
 inline __host__ __device__ long prim_tt(const Params &P, int s) { return (long)(P_NSCAL + 4 * P.ns + s) * P.fstride; }
 inline __host__ __device__ int nprim(int ns, int tdr) { return P_NSCAL + (tdr ? 5 : 3) * ns; }
-inline __host__ __device__ int nface(int ns) { return F_NSCAL + ns + 5 + ns; }
-inline __host__ __device__ int face_corr(int ns) { return F_NSCAL + ns; }
+inline __host__ __device__ int face_h(int ns, int s) { return F_NSCAL + ns + s; }
+inline __host__ __device__ int face_y(int ns, int s) { return F_NSCAL + 2 * ns + s; }
+inline __host__ __device__ int face_rate(int ns, int s) { return F_NSCAL + 3 * ns + s; }
+inline __host__ __device__ int face_corr(int ns) { return F_NSCAL + 4 * ns; }
+inline __host__ __device__ int nface(int ns) { return face_corr(ns) + 5 + ns; }
 
 // Substage launches cover any number of cases ("members") with one grid per
 // kernel kind.  The grid of a kind is the concatenation of every member's own

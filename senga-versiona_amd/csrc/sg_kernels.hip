@@ -1719,6 +1719,16 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
   const bool zadb_any = (i == 0 && P.fadb[0]) || (i == P.nx - 1 && P.fadb[1]) ||
                         (j == 0 && P.fadb[2]) || (j == P.ny - 1 && P.fadb[3]) ||
                         (k == 0 && P.fadb[4]) || (k == P.nz - 1 && P.fadb[5]);
+  // x-face point of an NSCBC case: copies of what the NSCBC kernel reads go
+  // to the face arrays (in the x pass only the edge tiles hold such points;
+  // the test and the address are formed where they are used, so that they
+  // hold no registers through the pass)
+  constexpr bool FACES = DIR != 0 || XP == 2;
+  auto facew = [&]() { return FACES && active && (P.inflow_xl || P.outflow_xr) && (i == 0 || i == P.nx - 1); };
+  const long fnf = (long)P.ny * P.nz;
+  auto fcw = [&](int slot) -> double & {
+    return F.face[(i == 0 ? 0 : (long)nface(P.ns) * fnf) + (long)slot * fnf + (long)k * P.ny + j];
+  };
   // rho u_d stays resident in its own LDS tile for the whole pass: products
   // with it (e rho u_d, Y rho u_d and the momentum fluxes rho u_d u_b) are
   // formed inside the stencils from two staged tiles instead of being staged
@@ -1820,6 +1830,12 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
     const R e0 = ctrf(0);
     aE = -R(0.5) * (e0 * dm + dem + md0 * de);
     pin(aE);
+    if (DIR == 0 && facew()) {
+      fcw(F_E) = (double)e0;
+      fcw(F_T) = (double)ctrf(1);
+      fcw(F_P) = (double)pp[idx];
+      fcw(F_RHO) = Q[idx];
+    }
   }
   SG_PHASE();
   // ---------------- heat conduction (src/rhscal.f:427-797)
@@ -1846,6 +1862,21 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
     flux[a] = D1Pf(a);
     pin(flux[a]);
     SG_PHASE();
+  }
+  if (facew()) {
+    if (DIR == 0) {
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        fcw(F_U + a) = (double)u0[a];
+        fcw(F_DUX + a) = (double)du[a];
+      }
+    } else if (DIR == 1) {
+      fcw(F_DUY) = (double)du[0];
+      fcw(F_DVY) = (double)du[1];
+    } else {
+      fcw(F_DUZ) = (double)du[0];
+      fcw(F_DWZ) = (double)du[2];
+    }
   }
   next();
   // ---------------- p - p_ref and mu under one barrier
@@ -1890,11 +1921,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
     aM[DIR] = aM[DIR] - dp;
     aE = aE - u0[DIR] * dp - p0 * du[DIR];
     pin(aM[DIR]); pin(aE);
-    if (active && (P.inflow_xl || P.outflow_xr) && (i == 0 || i == P.nx - 1)) {
-      long nf = (long)P.ny * P.nz, fq = (long)k * P.ny + j;
-      double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
-      fc[(DIR == 0 ? F_DPDX : (DIR == 1 ? F_DPDY : F_DPDZ)) * nf + fq] = dp;
-    }
+    if (facew()) fcw(DIR == 0 ? F_DPDX : (DIR == 1 ? F_DPDY : F_DPDZ)) = dp;
   }
   // ---------------- separable viscous terms (src/rhsvel.f:1462-2305)
 #pragma unroll
@@ -1975,6 +2002,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
           if (BND) { int pq[5]; pkc(pq); ds7s = d1_rows<R>(P, pq, DIR, wq, fqp); }
           else ds7s = d1<false, R>(P, S, DIR, atp);
           SG_PHASE();
+          // this stencil read the other buffer, which the next stage
+          // overwrites: every wave's reads end before any wave's stores
+          // (the barrier of this stage ordered only the stage before)
+          __syncthreads();
         }
         pin(tdr0); pin(ttc); pin(dtt); pin(ds7s);
         next();
@@ -2027,10 +2058,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
     dYz[sp] = zdif ? R(0) : dY;
     yv[sp] = yc;
     pin(aY[sp]); pin(dYz[sp]); pin(yv[sp]); pin(vt); pin(ucor); pin(aE);
-    if (DIR == 0 && BND && active && (i == 0 || i == P.nx - 1)) {
-      long nf = (long)P.ny * P.nz, fq = (long)k * P.ny + j;
-      double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
-      fc[(F_NSCAL + sp) * nf + fq] = dY;
+    if (DIR == 0 && BND && facew()) {
+      fcw(F_NSCAL + sp) = dY;
+      fcw(face_h(P.ns, sp)) = (double)hc;
+      fcw(face_y(P.ns, sp)) = (double)yc;
+      fcw(face_rate(P.ns, sp)) = (double)((const R *)F.rate)[sp * fs + idx];
     }
   }
   {
@@ -2088,38 +2120,37 @@ __global__ void __launch_bounds__(NSCBC_BLK) k_nscbc(const Member *__restrict__ 
   const int k = (int)(fq / P.ny), j = (int)(fq - (long)k * P.ny);
   const int i = side ? P.nx - 1 : 0;
   const long idx = ((long)k * P.ny + j) * P.nx + i;
-  const long fs = P.fstride;
-  const R *pr = (const R *)F.prim;
-  const R u = pr[P_U * fs + idx], v = pr[P_V * fs + idx], w = pr[P_W * fs + idx];
-  const R *G = (const R *)F.grad;
-  auto gr = [&](int q) { return G[q * fs + idx]; };
-  const R dux = gr(0), dvy = gr(5), dwz = gr(10);
+  // the face point's values, from the copies the passes wrote into the face
+  // arrays (contiguous over the face); each held in the working precision
+  // exactly as the passes read it
+  const double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf) + fq;
+  auto fv = [&](int slot) { return (R)fc[slot * nf]; };
+  const R u = fv(F_U), v = fv(F_V), w = fv(F_W);
+  const R dux = fv(F_DUX), dvy = fv(F_DVY), dwz = fv(F_DWZ);
   R rD = R(0), rU = R(0), rV = R(0), rW = R(0), rE = R(0);
   R rY[NS];
 #pragma unroll
   for (int s = 0; s < NS; s++) rY[s] = R(0);
   constexpr bool BND = true;
   {
-    const long nf = (long)P.ny * P.nz, fq = (long)k * P.ny + j;
-    const double *fc = F.face + (i == 0 ? 0 : (long)nface(P.ns) * nf);
-    const double strg = fc[F_CP * nf + fq], strr = fc[F_RR * nf + fq];
-    const double strt = pr[P_T * fs + idx];
-    const double strd = F.q[idx];
+    const double strg = fc[F_CP * nf], strr = fc[F_RR * nf];
+    const double strt = fv(F_T);
+    const double strd = fc[F_RHO * nf];
     double gam1 = strg - strr;
-    const double stre = pr[P_E * fs + idx] - gam1 * strt;
+    const double stre = fv(F_E) - gam1 * strt;
     gam1 = strr / gam1;
     const double ovgm = 1.0 / gam1;
     double fornow = strg * gam1 * strt;
     const double acou = sqrt(fornow), ova2 = 1.0 / fornow;
     double strh[NS];
 #pragma unroll
-    for (int s = 0; s < NS; s++) strh[s] = pr[prim_h(P, s) + idx] - strg * strt * c->rgspec[s] / strr;
+    for (int s = 0; s < NS; s++) strh[s] = fv(face_h(P.ns, s)) - strg * strt * c->rgspec[s] / strr;
     if (i == 0 && P.inflow_xl) {
       // NSBCI3, src/bounds.f:610-682 (imposed values have zero time derivative)
       // drho/dx at the face (src/rhscal.f:3966-3990), one-sided closure row
       const Sten Sx = make_sten<BND>(i, P.nx, 1);
       const double drdx = d1<BND>(P, Sx, 0, [&](int kk) { return F.q[idx + Sx.o[kk]]; });
-      double bcl1 = dux, bcl2 = drdx, bcl3 = gr(1), bcl4 = gr(2), bcl5 = fc[F_DPDX * nf + fq];
+      double bcl1 = dux, bcl2 = drdx, bcl3 = fv(F_DVX), bcl4 = fv(F_DWX), bcl5 = fc[F_DPDX * nf];
       fornow = strd * acou * bcl1;
       bcl1 = 0.5 * (u - acou) * (bcl5 - fornow);
       bcl2 = u * (bcl2 - bcl5 * ova2);
@@ -2134,23 +2165,23 @@ __global__ void __launch_bounds__(NSCBC_BLK) k_nscbc(const Member *__restrict__ 
       rE = rE - bcl2 * stre - bcl3 * strd * v - bcl4 * strd * w - bcl5 * (ova2 * stre + u / acou + ovgm);
 #pragma unroll
       for (int s = 0; s < NS; s++) {
-        double bcly = ((const R *)F.rate)[s * fs + idx] / strd - 0.0 - u * fc[(F_NSCAL + s) * nf + fq];
+        double bcly = fv(face_rate(P.ns, s)) / strd - 0.0 - u * fc[(F_NSCAL + s) * nf];
         rE = rE - bcly * strd * strh[s];
       }
     }
     if (i == P.nx - 1 && P.outflow_xr) {
       // NSBCO1, src/bounds.f:1296-1458 (pointwise in/outflow switch and local-Mach beta off)
       const double bet = 0.050;
-      const double strp = (double)pr[P_P * fs + idx] + P.pofs;
-      const double t51 = -v * fc[F_DPDY * nf + fq] - w * fc[F_DPDZ * nf + fq];
+      const double strp = (double)fv(F_P) + P.pofs;
+      const double t51 = -v * fc[F_DPDY * nf] - w * fc[F_DPDZ * nf];
       const double t52 = -strp * (dvy + dwz);
-      const double t2b = -v * gr(4) - w * gr(8);
+      const double t2b = -v * fv(F_DUY) - w * fv(F_DUZ);
       const double tt1 = t51 + t52 * (gam1 + 1.0) - strd * acou * t2b;
       double sorp = 0.0;
 #pragma unroll
-      for (int s = 0; s < NS; s++) sorp = sorp + strh[s] * ((const R *)F.rate)[s * fs + idx];
+      for (int s = 0; s < NS; s++) sorp = sorp + strh[s] * fv(face_rate(P.ns, s));
       sorp = -sorp * gam1;
-      double bcl1 = 0.5 * (u - acou) * (fc[F_DPDX * nf + fq] - strd * acou * dux);
+      double bcl1 = 0.5 * (u - acou) * (fc[F_DPDX * nf] - strd * acou * dux);
       bcl1 = 0.5 * sorp + P.cobcxr * acou * (strp - P.pinfxr) + 0.5 * (1.0 - bet) * tt1 - bcl1;
       rD = rD - bcl1 * ova2;
       rU = rU - bcl1 * ova2 * (u - acou);
@@ -2158,7 +2189,7 @@ __global__ void __launch_bounds__(NSCBC_BLK) k_nscbc(const Member *__restrict__ 
       rW = rW - bcl1 * ova2 * w;
       rE = rE - bcl1 * (ova2 * stre - u / acou + ovgm);
 #pragma unroll
-      for (int s = 0; s < NS; s++) rY[s] = rY[s] - bcl1 * ova2 * pr[prim_y(P, s) + idx];
+      for (int s = 0; s < NS; s++) rY[s] = rY[s] - bcl1 * ova2 * fv(face_y(P.ns, s));
     }
   }
   double *cr = F.face + (long)side * nface(P.ns) * nf + (long)face_corr(P.ns) * nf + fq;
