@@ -77,12 +77,12 @@ def kernel_bytes(name, ns, bounded, nx=128):
         return 2 * f * ((6 + ns) + 2 * (5 + ns)) / nx
     if name == "prim":          # reads stage (5+N), T, index; writes primitives, rate, index
         return f * (5 + ns + 1) + 4 + f * (nprim + ns) + 4
-    if name == "pass_x":        # rho, 3 momenta, primitives, rate -> accumulators, 4 gradients
-        return f * (4 + nprim + ns) + f * (5 + ns) + f * 4
+    if name == "pass_x":        # rho, 3 momenta, primitives, rate -> accumulators, d mu/dx
+        return f * (4 + nprim + ns) + f * (5 + ns) + f
     if name in ("pass_y", "pass_z"):   # accumulators read-modify-write
-        return f * (4 + nprim) + 2 * f * (5 + ns) + f * 4
-    if name == "final":         # accumulators, 12 gradients, u,v,w,mu, run (r/w), stage (w)
-        return f * (5 + ns) + f * 12 + f * 4 + 2 * f * (5 + ns) + f * (5 + ns)
+        return f * (4 + nprim) + 2 * f * (5 + ns) + f
+    if name == "final":         # accumulators, 3 viscosity gradients, u,v,w,mu, run (r/w), stage (w)
+        return f * (5 + ns) + f * 3 + f * 4 + 2 * f * (5 + ns) + f * (5 + ns)
     return 0
 
 

@@ -675,7 +675,7 @@ static size_t case_bytes(const Params &P, size_t *main_block) {
   // FP64 registers; primitives, rates, accumulators and gradients in the
   // working precision (FP32 in the mixed mode)
   const size_t esz = P.precision ? sizeof(float) : sizeof(double);
-  const long nreg = (5 + ns) * 2, nwork = nprim(ns, P.soret || P.dufour) + ns + (5 + ns) + 12;
+  const long nreg = (5 + ns) * 2, nwork = nprim(ns, P.soret || P.dufour) + ns + (5 + ns) + 3;
   size_t bytes = sizeof(double) * (size_t)fs * nreg + esz * (size_t)fs * nwork;
   bytes = ((bytes + 255) / 256) * 256;
   const size_t itn = ((sizeof(uint32_t) * (size_t)fs + 255) / 256) * 256;
@@ -737,7 +737,7 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   const size_t total = case_bytes(P, &h->bytes);
   h->lbytes = total - h->bytes;
   size_t bytes = sizeof(double) * (size_t)fs * (5 + ns) * 2 +
-                 esz * (size_t)fs * (nprim(ns, P.soret || P.dufour) + ns + (5 + ns) + 12);
+                 esz * (size_t)fs * (nprim(ns, P.soret || P.dufour) + ns + (5 + ns) + 3);
   bytes = ((bytes + 255) / 256) * 256;
   const size_t itn = ((sizeof(uint32_t) * (size_t)fs + 255) / 256) * 256;
   const size_t facesz = sizeof(double) * 2 * (size_t)nface(ns) * P.ny * P.nz;
@@ -755,7 +755,7 @@ int sg_create(const sg_config *cfg, sg_handle **out) {
   F.prim = (double *)b; b += esz * nprim(ns, P.soret || P.dufour) * fs;   // element type per precision
   F.rate = (double *)b; b += esz * ns * fs;
   F.acc = (double *)b; b += esz * (5 + ns) * fs;
-  F.grad = (double *)b; b += esz * 12 * fs;
+  F.grad = (double *)b; b += esz * 3 * fs;
   b = (char *)h->base + bytes;
   F.itndex = (uint32_t *)b; b += itn;
   F.face = (double *)b; b += ((facesz + 255) / 256) * 256;

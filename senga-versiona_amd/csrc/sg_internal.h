@@ -88,7 +88,7 @@ struct Fields {
   double *prim;   // primitives: see PrimSlot, then Y[ns], rhoD[ns], h[ns]
   double *rate;   // chemical source (ns), kg/m^3/s
   double *acc;    // RHS accumulators (5+ns) + VT (divergence of the correction velocity)
-  double *grad;   // du/dx,dv/dx,dw/dx,dmu/dx, du/dy,..,dmu/dy, du/dz,..,dmu/dz (12)
+  double *grad;   // dmu/dx, dmu/dy, dmu/dz (the velocity gradients the final kernel forms itself)
   uint32_t *itndex;  // packed temperature interval indices (1 bit per species)
   double *face;   // NSCBC face data, see FaceSlot, [face 0/1][slot][ny*nz]
   int *err;       // [0]=error code, [1..3] = failing point (1-based)

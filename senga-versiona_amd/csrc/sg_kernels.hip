@@ -1934,13 +1934,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
     pin(aM[a]);
   }
   pin(aE);
-  if (active) {
-    R *G = (R *)F.grad + (long)DIR * 4 * fs;
-    G[idx] = du[0];
-    G[fs + idx] = du[1];
-    G[2 * fs + idx] = du[2];
-    G[3 * fs + idx] = dmu;
-  }
+  if (active) ((R *)F.grad)[(long)DIR * fs + idx] = dmu;   // d mu / d x_DIR for the final kernel
   // ---------------- species (src/rhscal.f:1030-3944)
   R vt = 0.0, ucor = 0.0;
   R aY[NS], dYz[NS], yv[NS];
@@ -2225,7 +2219,7 @@ template <bool BND, int NS, class R>
 __device__ __forceinline__ void final_point(const Params &P, const Fields &F, const sg_config *__restrict__ c,
                                             int i, int j, int k, int mode, double lhs, double rhsc, double erc,
                                             double *__restrict__ out, R uxy, R vxy, R uxz, R wxz, R vyz, R wyz,
-                                            R u, R v, R w) {
+                                            R u, R v, R w, R dux, R dvy, R dwz, R sxy, R sxz, R syz) {
   const long idx = ((long)k * P.ny + j) * P.nx + i;
   const long fs = P.fstride;
   constexpr int ES = (int)sizeof(R);      // bytes per element of the FP32/FP64 fields
@@ -2243,8 +2237,7 @@ __device__ __forceinline__ void final_point(const Params &P, const Fields &F, co
     return (m == 0 && fl[2 * dir]) || (m == n - 1 && fl[2 * dir + 1]);
   };
   const R fthd = (R)P.fthd, tthd = (R)P.tthd;
-  const R dux = gr(0), dvy = gr(5), dwz = gr(10);
-  const R dmux = gr(3), dmuy = gr(7), dmuz = gr(11);
+  const R dmux = gr(0), dmuy = gr(1), dmuz = gr(2);
   // normal stresses with viscosity gradients
   {
     R sxx = fthd * dux - tthd * (dvy + dwz);
@@ -2264,7 +2257,7 @@ __device__ __forceinline__ void final_point(const Params &P, const Fields &F, co
     rE = rE + w * t;
   }
   {  // xy pair
-    const R S7 = gr(4) + gr(1);   // du/dy + dv/dx
+    const R S7 = sxy;   // du/dy + dv/dx
     const R ma = uxy, mb = vxy;
     R t = fz(P.fvsn, 0) ? R(0) : tthd * mu * mb;
     rU = rU - t; rE = rE - u * t;
@@ -2277,7 +2270,7 @@ __device__ __forceinline__ void final_point(const Params &P, const Fields &F, co
     rE = rE + mu * S7 * S7;
   }
   {  // xz pair
-    const R S7 = gr(8) + gr(2);   // du/dz + dw/dx
+    const R S7 = sxz;   // du/dz + dw/dx
     const R ma = uxz, mb = wxz;
     R t = fz(P.fvsn, 0) ? R(0) : tthd * mu * mb;
     rU = rU - t; rE = rE - u * t;
@@ -2290,7 +2283,7 @@ __device__ __forceinline__ void final_point(const Params &P, const Fields &F, co
     rE = rE + mu * S7 * S7;
   }
   {  // yz pair
-    const R S7 = gr(9) + gr(6);   // dv/dz + dw/dy
+    const R S7 = syz;   // dv/dz + dw/dy
     const R ma = vyz, mb = wyz;
     R t = fz(P.fvsn, 1) ? R(0) : tthd * mu * mb;
     rV = rV - t; rE = rE - v * t;
@@ -2398,6 +2391,9 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
   __shared__ R tile[FT::GN];
   const int t = threadIdx.x;
   if (BND && t < SG_NROW * 7) mtab[t] = (R)(&F.tab->mixw[0][0])[t];   // ordered by the first staging barrier
+  // first-derivative row weights W1 of the x closure rows (the passes' table)
+  __shared__ R wtab1[BND ? SG_NROW * 5 : 1];
+  if (BND && t < SG_NROW * 5) wtab1[t] = (R)F.tab->roww[t / 5][t % 5];
   const int nx = P.nx, ny = P.ny, nz = P.nz;
   const int nbx = (nx + FX - 1) / FX, nby = (ny + FY - 1) / FY;
   const int x0 = (lb % nbx) * FX, y0 = ((lb / nbx) % nby) * FY, z0 = (lb / (nbx * nby)) * FZ;
@@ -2478,9 +2474,32 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
   auto mix_yz = [&](int B) {
     return dmix<false, R>(P, S0, 1, 2, [&](int k1, int k2) { return tile[B + (k1 - 8) * GX + (k2 - 8) * GX * GY]; });
   };
+  // first derivatives of the staged field: x with the closure rows where the
+  // wave holds them (the passes' rows and weights), y and z interior (the
+  // supported cases are periodic there)
+  auto d1x = [&](int B) {
+    if (xedge) {
+      int pq[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        pq[q] = c_row_offsets.w[row][q];
+        asm volatile("" : "+v"(pq[q]));
+      }
+      return d1_rows<R>(P, pq, 0, [&](int q) { return wtab1[BND ? row * 5 + q : 0]; },
+                        [&](int q) { return tile[B + q]; });
+    }
+    return d1<false, R>(P, S0, 0, [&](int kk) { return tile[B + kk - 8]; });
+  };
+  auto d1y = [&](int B) { return d1<false, R>(P, S0, 1, [&](int kk) { return tile[B + (kk - 8) * GX]; }); };
+  auto d1z = [&](int B) { return d1<false, R>(P, S0, 2, [&](int kk) { return tile[B + (kk - 8) * GX * GY]; }); };
   // the derivatives, and each point's own u, v, w read back from the tile
-  // (the values a global load would give) for the per-point part
+  // (the values a global load would give) for the per-point part.  The
+  // velocity gradients the viscous terms use are formed here from the tile
+  // too, instead of being written by the passes and read back: du/dx, dv/dy,
+  // dw/dz and the sums du/dy + dv/dx, du/dz + dw/dx, dv/dz + dw/dy, each
+  // added in that order
   R uxy[NPT], uxz[NPT], vxy[NPT], vyz[NPT], wxz[NPT], wyz[NPT], uc[NPT], vc[NPT], wc[NPT];
+  R dux[NPT], dvy[NPT], dwz[NPT], sxy[NPT], sxz[NPT], syz[NPT];
   fetch(pr + P_U * fs);
   store();
   __syncthreads();
@@ -2491,6 +2510,10 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
     uxy[p] = mix_x(base(p), 1);
     uxz[p] = mix_x(base(p), 2);
     pin(uxy[p]); pin(uxz[p]);
+    dux[p] = d1x(base(p));
+    sxy[p] = d1y(base(p));
+    sxz[p] = d1z(base(p));
+    pin(dux[p]); pin(sxy[p]); pin(sxz[p]);
   }
   __syncthreads();
   store();
@@ -2502,6 +2525,10 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
     vxy[p] = mix_x(base(p), 1);
     vyz[p] = mix_yz(base(p));
     pin(vxy[p]); pin(vyz[p]);
+    sxy[p] = sxy[p] + d1x(base(p));
+    dvy[p] = d1y(base(p));
+    syz[p] = d1z(base(p));
+    pin(sxy[p]); pin(dvy[p]); pin(syz[p]);
   }
   __syncthreads();
   store();
@@ -2512,13 +2539,17 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
     wxz[p] = mix_x(base(p), 2);
     wyz[p] = mix_yz(base(p));
     pin(wxz[p]); pin(wyz[p]);
+    sxz[p] = sxz[p] + d1x(base(p));
+    syz[p] = syz[p] + d1y(base(p));
+    dwz[p] = d1z(base(p));
+    pin(sxz[p]); pin(syz[p]); pin(dwz[p]);
   }
 #pragma unroll
   for (int p = 0; p < NPT; p++) {
     const int k = z0 + lz0 + p * FT::ZS;
     if (i < nx && j < ny && k < nz)
       final_point<BND, NS, R>(P, F, c, i, j, k, mode, lhs, rhsc, erc, out, uxy[p], vxy[p], uxz[p], wxz[p],
-                              vyz[p], wyz[p], uc[p], vc[p], wc[p]);
+                              vyz[p], wyz[p], uc[p], vc[p], wc[p], dux[p], dvy[p], dwz[p], sxy[p], sxz[p], syz[p]);
   }
 }
 
