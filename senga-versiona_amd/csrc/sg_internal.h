@@ -105,7 +105,7 @@ enum PrimSlot { P_U = 0, P_V, P_W, P_E, P_T, P_P, P_LAM, P_MU, P_HMIX, P_LNW, P_
 // into them, at the x-face points, every value the NSCBC kernel reads
 // (velocity, T, E, stored p, rho, the velocity gradients it uses), so that
 // it reads contiguous face arrays instead of one cache line per value.
-enum FaceSlot { F_CP = 0, F_RR, F_DPDX, F_DPDY, F_DPDZ, F_DRDX, F_U, F_V, F_W, F_T, F_E, F_P, F_RHO,
+enum FaceSlot { F_CP = 0, F_RR, F_DPDX, F_DPDY, F_DPDZ, F_U, F_V, F_W, F_T, F_E, F_P, F_RHO,
                 F_DUX, F_DVX, F_DWX, F_DUY, F_DVY, F_DUZ, F_DWZ, F_NSCAL };
 // per species: dY/dx at F_NSCAL + s, h at face_h, Y at face_y, the
 // chemical rate at face_rate; then the NSCBC corrections of the 5 + ns
