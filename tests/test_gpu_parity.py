@@ -328,6 +328,22 @@ def test_mixed_precision_against_fp64(kind, n, steps):
     assert_state_close("mixed vs oracle", run32, T32, run_o, T_o, 1e-5, 1e-5)
 
 
+def test_mixed_cold_gas_takes_fp64_rates():
+    """The mixed form's FP32 rates hold the compiled H2-air mechanism's rate
+    coefficients only above about 206 K; an H2-air duct whose unburnt gas is
+    at 180 K has every cold point outside that range, where the rates are
+    formed again in FP64 (the compiled-mechanism instantiation of that
+    kernel).  After 5 steps: finite, and within 1e-5 of the oracle."""
+    from sengapy import case as C
+    c = C.h2_nscbc(n=(40, 24, 24), length=(4e-3, 2.4e-3, 2.4e-3), mixavg=True, Tu=180.0, nsteps=5)
+    print("case", c.name, "Tu 180 K, 5 steps", flush=True)
+    run_o, T_o = _oracle_after(c, 5)
+    run, Tg = _gpu_after(c, 5, 1)
+    assert np.isfinite(run).all() and np.isfinite(Tg).all()
+    assert float(Tg.min()) < 206.0
+    assert_state_close("mixed vs oracle", run, Tg, run_o, T_o, 1e-5, 1e-5)
+
+
 def test_mixed_renormalisation():
     """precision 2: the mixed form with the FP64 species renormalisation at the
     end of every step.  After 20 steps sum_k rho Y_k equals rho to round-off
