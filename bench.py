@@ -458,6 +458,9 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=64,
                     help="uneven H2-air cases of the batched run (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--batch-small-cases", type=int, default=64,
+                    help="cases of the second batched run, on small grids {16,24,32}^3 (0: skip)")
+    ap.add_argument("--batch-small-steps", type=int, default=20)
     ap.add_argument("--serve-requests", type=int, default=16,
                     help="flame-kernel requests of the served run (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=40)
@@ -492,12 +495,16 @@ def main():
         print(json.dumps(out), flush=True)
         return
     bspecs = Cs.batch_specs(a.batch_cases) if a.batch_cases > 0 else []
+    bsspecs = (Cs.batch_specs(a.batch_small_cases, seed=77, sizes=(16, 24, 32))
+               if a.batch_small_cases > 0 else [])
     sspecs = Cs.kernel_specs(a.serve_requests, rate=a.serve_rate, nsteps=a.serve_steps) if a.serve_requests > 0 else []
     lspecs = ([dict(kind="nscbc", n=(a.large_n,) * 3, phi=0.7, uprime_over_sl=5.0, Tu=300.0, seed=7, nsteps=100)]
               if a.large_n > 0 else [])
-    built = make_cases(lspecs + bspecs + sspecs) if bspecs or sspecs or lspecs else []
+    allspecs = lspecs + bspecs + bsspecs + sspecs
+    built = make_cases(allspecs) if allspecs else []
     lcases, built = built[:len(lspecs)], built[len(lspecs):]
-    bcases, scases = built[:len(bspecs)], built[len(bspecs):]
+    bcases, built = built[:len(bspecs)], built[len(bspecs):]
+    bscases, scases = built[:len(bsspecs)], built[len(bsspecs):]
     n = a.n
     t0 = time.perf_counter()
     c = Cs.h2_nscbc(n=(n, n, n), length=(5.0e-3, 5.0e-3, 5.0e-3), mixavg=True, nsteps=a.steps)
@@ -594,6 +601,13 @@ def main():
         out["batched"]["workload"] = (f"{len(bcases)} H2-air NSCBC cases, n^3 with n from {{48,64,96,128}}, "
                                       "phi~U(0.5,1.2), u'/S_L~U(1,10), T_u~U(300,700) K; "
                                       f"{a.batch_steps} RK steps timed, rates per 100-step case")
+    if bscases:
+        # where batching pays: cases too small to fill the GPU alone, whose
+        # solo steps are bound by launch latency
+        out["batched_small"] = batched_run(abi, h, bscases, a.batch_small_steps)
+        out["batched_small"]["workload"] = (f"{len(bscases)} H2-air NSCBC cases, n^3 with n from {{16,24,32}}, "
+                                            "phi~U(0.5,1.2), u'/S_L~U(1,10), T_u~U(300,700) K; "
+                                            f"{a.batch_small_steps} RK steps timed, rates per 100-step case")
     if scases:
         out["serving"] = serve_run(scases, sspecs, a.serve_steps, a.serve_point_budget, a.serve_chunk)
         out["serving"]["workload"] = (f"{len(scases)} H2-air flame-kernel requests, Poisson {a.serve_rate}/s, "
