@@ -13,7 +13,8 @@ Beside the headline figure, "batched" advances 64 uneven cases together
 RK step) against the same cases stepped one after another, and "serving"
 pushes a Poisson stream of flame-kernel requests through the continuous-
 batching loop (sengapy.serve) against sequential FP64 solves, with p50/p99
-time to solution.
+time to solution.  "batched_small" and "serving_small" repeat both on cases
+of 16-32^3, which are too small to fill the GPU alone.
 
 Output: one JSON line.  roofline: the dominant kernel's algorithmic HBM bytes
 per launch over its mean duration (HIP events on the library's stream);
@@ -468,6 +469,10 @@ def main():
     ap.add_argument("--serve-point-budget", type=float, default=6e6,
                     help="grid points in flight beyond which the serving loop queues requests")
     ap.add_argument("--serve-chunk", type=int, default=5, help="RK steps between admission points")
+    ap.add_argument("--serve-small-requests", type=int, default=48,
+                    help="requests of the second served run, on small grids {16,24,32}^3 (0: skip)")
+    ap.add_argument("--serve-small-rate", type=float, default=200.0)
+    ap.add_argument("--serve-small-steps", type=int, default=100)
     ap.add_argument("--large-n", type=int, default=256,
                     help="grid of the second FP64/mixed throughput figure (0: skip)")
     ap.add_argument("--large-steps", type=int, default=5)
@@ -497,14 +502,17 @@ def main():
     bspecs = Cs.batch_specs(a.batch_cases) if a.batch_cases > 0 else []
     bsspecs = (Cs.batch_specs(a.batch_small_cases, seed=77, sizes=(16, 24, 32))
                if a.batch_small_cases > 0 else [])
+    ssspecs = (Cs.kernel_specs(a.serve_small_requests, seed=99, rate=a.serve_small_rate, sizes=(16, 24, 32),
+                               nsteps=a.serve_small_steps) if a.serve_small_requests > 0 else [])
     sspecs = Cs.kernel_specs(a.serve_requests, rate=a.serve_rate, nsteps=a.serve_steps) if a.serve_requests > 0 else []
     lspecs = ([dict(kind="nscbc", n=(a.large_n,) * 3, phi=0.7, uprime_over_sl=5.0, Tu=300.0, seed=7, nsteps=100)]
               if a.large_n > 0 else [])
-    allspecs = lspecs + bspecs + bsspecs + sspecs
+    allspecs = lspecs + bspecs + bsspecs + sspecs + ssspecs
     built = make_cases(allspecs) if allspecs else []
     lcases, built = built[:len(lspecs)], built[len(lspecs):]
     bcases, built = built[:len(bspecs)], built[len(bspecs):]
-    bscases, scases = built[:len(bsspecs)], built[len(bsspecs):]
+    bscases, built = built[:len(bsspecs)], built[len(bsspecs):]
+    scases, sscases = built[:len(sspecs)], built[len(sspecs):]
     n = a.n
     t0 = time.perf_counter()
     c = Cs.h2_nscbc(n=(n, n, n), length=(5.0e-3, 5.0e-3, 5.0e-3), mixavg=True, nsteps=a.steps)
@@ -614,6 +622,12 @@ def main():
                                       f"n^3 with n in 48..160, {a.serve_steps} RK steps each, "
                                       f"240 GB HBM budget, {a.serve_point_budget:.3g} points in flight, "
                                       f"{a.serve_chunk}-step chunks, time to solution = arrival to downloaded state")
+    if sscases:
+        out["serving_small"] = serve_run(sscases, ssspecs, a.serve_small_steps, a.serve_point_budget, a.serve_chunk)
+        out["serving_small"]["workload"] = (f"{len(sscases)} H2-air flame-kernel requests, Poisson "
+                                            f"{a.serve_small_rate}/s, n^3 with n in {{16,24,32}}, "
+                                            f"{a.serve_small_steps} RK steps each, {a.serve_point_budget:.3g} "
+                                            f"points in flight, {a.serve_chunk}-step chunks")
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
