@@ -2479,12 +2479,14 @@ __global__ void __launch_bounds__(BLK) k_final(const Member *__restrict__ mem, c
   // supported cases are periodic there)
   auto d1x = [&](int B) {
     if (xedge) {
-      int pq[5];
+      // a first derivative reads entries 0..9 (P1, M1) of the packed row,
+      // words 0..2; all three loads are issued before the fences, which
+      // would otherwise wait for each load in turn
+      int pq[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-      for (int q = 0; q < 5; q++) {
-        pq[q] = c_row_offsets.w[row][q];
-        asm volatile("" : "+v"(pq[q]));
-      }
+      for (int q = 0; q < 3; q++) pq[q] = c_row_offsets.w[row][q];
+#pragma unroll
+      for (int q = 0; q < 3; q++) asm volatile("" : "+v"(pq[q]));
       return d1_rows<R>(P, pq, 0, [&](int q) { return wtab1[BND ? row * 5 + q : 0]; },
                         [&](int q) { return tile[B + q]; });
     }
