@@ -2320,6 +2320,12 @@ __device__ __forceinline__ void final_point(const Params &P, const Fields &F, co
     for (int vv2 = 0; vv2 < 5 + NS; vv2++) out[vv2 * fs + idx] = Rv[vv2];
     return;   // no register update
   }
+  // The solution register of every variable is loaded before the first
+  // store: the stores could alias the loads as far as the compiler knows, so
+  // a load after a store would cost a memory round trip per variable.
+  double old[5 + NS];
+#pragma unroll
+  for (int vv2 = 0; vv2 < 5 + NS; vv2++) old[vv2] = fld(F.run + vv2 * fs, bo8);
   if (sizeof(R) == 4 && mode == 1 && P.renorm) {
     // mixed form with renormalisation: FINCOM, then the species of the new
     // solution rescaled in FP64 so that they sum to the new density
@@ -2327,12 +2333,11 @@ __device__ __forceinline__ void final_point(const Params &P, const Fields &F, co
 #pragma unroll
     for (int vv2 = 0; vv2 < 5 + NS; vv2++) {
       const int slot = vv2 < 5 ? vv2 : 5;
-      const double old = fld(F.run + vv2 * fs, bo8);
       if (i >= P.rklo[slot]) {
         if (F.erk) fld(F.erk + vv2 * fs, bo8) += erc * (double)Rv[vv2];
-        nw[vv2] = old + lhs * (double)Rv[vv2];
+        nw[vv2] = old[vv2] + lhs * (double)Rv[vv2];
       } else {
-        nw[vv2] = old;
+        nw[vv2] = old[vv2];
       }
     }
     if (i >= P.rklo[5]) {
@@ -2357,12 +2362,11 @@ __device__ __forceinline__ void final_point(const Params &P, const Fields &F, co
     double *runv = F.run + vv2 * fs, *qv = F.q + vv2 * fs;
     if (i >= P.rklo[slot]) {
       if (F.erk) fld(F.erk + vv2 * fs, bo8) += erc * (double)Rv[vv2];   // *ERR += RKERR * R
-      double old = fld(runv, bo8);
       if (mode == 0) {
-        fld(runv, bo8) = old + lhs * (double)Rv[vv2];
-        fld(qv, bo8) = old + rhsc * (double)Rv[vv2];
+        fld(runv, bo8) = old[vv2] + lhs * (double)Rv[vv2];
+        fld(qv, bo8) = old[vv2] + rhsc * (double)Rv[vv2];
       } else {
-        double nw = old + lhs * (double)Rv[vv2];
+        double nw = old[vv2] + lhs * (double)Rv[vv2];
         fld(runv, bo8) = nw;
         fld(qv, bo8) = nw;
       }
