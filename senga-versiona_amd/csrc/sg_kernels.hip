@@ -2067,26 +2067,34 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && XP != 2 && !EXT ? 6 : 4))) 
   }
   if (!active) return;
   // ---------------- store
+  // Every value the stores add to (the chemical rates in x, the accumulators
+  // in y and z) is loaded before the first store: the stores could alias
+  // those loads as far as the compiler knows, so a load after a store would
+  // cost a memory round trip per variable.
   R *A = (R *)F.acc;
+  R ay[NS];
+#pragma unroll
+  for (int sp = 0; sp < NS; sp++) ay[sp] = aY[sp] - yv[sp] * vt - dYz[sp] * ucor;
+  R old[5 + NS];
   if (DIR == 0) {
+#pragma unroll
+    for (int sp = 0; sp < NS; sp++) old[5 + sp] = ((const R *)F.rate)[sp * fs + idx];
     A[idx] = aD;
     A[fs + idx] = aM[0];
     A[2 * fs + idx] = aM[1];
     A[3 * fs + idx] = aM[2];
     A[4 * fs + idx] = aE;
   } else {
-    A[idx] += aD;
-    A[fs + idx] += aM[0];
-    A[2 * fs + idx] += aM[1];
-    A[3 * fs + idx] += aM[2];
-    A[4 * fs + idx] += aE;
+#pragma unroll
+    for (int v = 0; v < 5 + NS; v++) old[v] = A[v * fs + idx];
+    A[idx] = old[0] + aD;
+    A[fs + idx] = old[1] + aM[0];
+    A[2 * fs + idx] = old[2] + aM[1];
+    A[3 * fs + idx] = old[3] + aM[2];
+    A[4 * fs + idx] = old[4] + aE;
   }
 #pragma unroll
-  for (int sp = 0; sp < NS; sp++) {
-    const R ay = aY[sp] - yv[sp] * vt - dYz[sp] * ucor;
-    if (DIR == 0) A[(5 + sp) * fs + idx] = ((const R *)F.rate)[sp * fs + idx] + ay;
-    else A[(5 + sp) * fs + idx] += ay;
-  }
+  for (int sp = 0; sp < NS; sp++) A[(5 + sp) * fs + idx] = old[5 + sp] + ay[sp];
 }
 
 // ---------------------------------------------------------------------------
