@@ -75,10 +75,13 @@ __device__ __forceinline__ void coords(const Params &P, long idx, int &i, int &j
 static_assert(SG_NTINT == 2 && SG_MAX_SPEC <= 32, "one index bit per species in a 32-bit word");
 __device__ __forceinline__ int itint_of(uint32_t idx, int s) { return (int)((idx >> s) & 1u); }
 
+// Interval of species s at temperature t: the first interval whose upper
+// bound is not below t, the last one otherwise.  With at most two intervals
+// (sg_create checks 1 <= ntint <= SG_NTINT) that is one comparison; written
+// as a per-lane search loop, the uniform bound was fetched by a vector load
+// and waited on inside the loop, once per species.
 __device__ __forceinline__ int locate(const sg_config *c, int s, double t) {
-  int it = 0;
-  while (t > c->tinthi[s][it] && it < c->ntint[s] - 1) it++;
-  return it;
+  return (c->ntint[s] > 1 && t > c->tinthi[s][0]) ? 1 : 0;
 }
 
 __device__ void set_error(int *err, int code, int i, int j, int k) {
